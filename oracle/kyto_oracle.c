@@ -1,0 +1,466 @@
+/*
+ * ORACLE — test infrastructure only. A CPU restatement, in plain C, of kyto's CPU
+ * path for pca() and neighbors(). Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker; the product
+ * (kyto_amd) never links or calls it.
+ *
+ * Build: oracle/Makefile (gcc, -ffp-contract=off so every f64 sum is evaluated in
+ * the order written, as Rust evaluates it: no fused multiply-add).
+ *
+ * Each function cites the reference file:line it restates (paths relative to the
+ * reference checkout).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ Omega ---- */
+
+/* SplitMix64's output function: crates/kyto-exec/src/pca.rs:946-950. */
+uint64_t ko_mix(uint64_t v) {
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
+    return v ^ (v >> 31);
+}
+
+/* Omega's row of one cell: crates/kyto-exec/src/pca.rs:936-943. 53 bits as a
+ * multiple of 2^-52 in [0, 2), shifted to [-1, 1). */
+void ko_omega_row(uint64_t seed, uint64_t cell, double *out, size_t l) {
+    uint64_t state = ko_mix(seed ^ ko_mix(cell + 0x9E3779B97F4A7C15ULL));
+    for (size_t j = 0; j < l; j++) {
+        state += 0x9E3779B97F4A7C15ULL;
+        out[j] = (double)(ko_mix(state) >> 11) * 0x1p-52 - 1.0;
+    }
+}
+
+/* ------------------------------------------------------------ input layer ---- */
+
+/* The fixed tree over 8 f64 lanes: crates/kyto-exec/src/kernels.rs:290-299. */
+static inline double combine_lanes(const double a[8]) {
+    double h0 = a[0] + a[4], h1 = a[1] + a[5], h2 = a[2] + a[6], h3 = a[3] + a[7];
+    double q0 = h0 + h2, q1 = h1 + h3;
+    return q0 + q1;
+}
+
+/* kyto's log1p: crates/kyto-exec/src/kernels/log1p.rs:1-13. Its own test
+ * (log1p.rs, `within_one_ulp_of_libm_and_rounds_the_f64_value`) asserts the result
+ * equals the f64 ln_1p rounded once to f32; this restates that definition. */
+float ko_log1p(float x) { return (float)log1p((double)x); }
+
+/* normalize_total(target_sum) then log1p on a counts CSR.
+ * Row total: Kernels::row_sums, 8 f64 lanes (kernels.rs:130-138, lane_sum);
+ * divisor: f32(total) / f32(target_sum), 0 -> 1 (elementwise.rs:262-269);
+ * value: f32(count) / divisor (kernels.rs DivideRows, f32 division);
+ * then log1p (above). */
+void ko_normalize_log1p(const uint64_t *indptr, const float *counts, size_t n_rows,
+                        double target_sum, float *normalized, float *out) {
+    #pragma omp parallel for schedule(static)
+    for (size_t r = 0; r < n_rows; r++) {
+        double lanes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        uint64_t s = indptr[r], e = indptr[r + 1];
+        for (uint64_t i = s; i < e; i++) lanes[(i - s) % 8] += (double)counts[i];
+        double total = combine_lanes(lanes);
+        float divisor = (float)total / (float)target_sum;
+        if (divisor == 0.0f) divisor = 1.0f;
+        for (uint64_t i = s; i < e; i++) {
+            float v = counts[i] / divisor;
+            if (normalized) normalized[i] = v;
+            out[i] = ko_log1p(v);
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- PCA folds ---- */
+/* The row-by-row fold kyto's split fold must equal bit for bit:
+ * crates/kyto-exec/src/pca.rs:1018-1093 (`reference_fold`), the arms of
+ * `fold_rows` pca.rs:588-677. One leaf over the rows in cell order. */
+
+/* Pass 0 (Sketch arm, pca.rs:588-622): n_cells, gene sums, Omega's column sums,
+ * and X^T Omega (genes x l). `cells` gives each row's table position. */
+void ko_pca_sketch(const uint64_t *indptr, const uint32_t *indices, const float *values,
+                   size_t n_rows, const uint64_t *cells, uint64_t seed, size_t l,
+                   uint64_t *n_cells, double *sums, double *omega_sums, double *sketch) {
+    double *omega = (double *)malloc(sizeof(double) * (l ? l : 1));
+    for (size_t r = 0; r < n_rows; r++) {
+        ko_omega_row(seed, cells[r], omega, l);
+        *n_cells += 1;
+        for (size_t j = 0; j < l; j++) omega_sums[j] += omega[j];
+        for (uint64_t i = indptr[r]; i < indptr[r + 1]; i++) {
+            size_t g = indices[i];
+            double v = (double)values[i];
+            sums[g] += v;
+            double *t = sketch + g * l;
+            for (size_t j = 0; j < l; j++) t[j] += v * omega[j];
+        }
+    }
+    free(omega);
+}
+
+/* Passes 1..q (Product arm, pca.rs:623-645 with `project` :548-586): each row's
+ * w = -block_means + sum value * block[gene] (values in gene order), then
+ * product[gene] += value * w, and sums += w. */
+void ko_pca_product(const uint64_t *indptr, const uint32_t *indices, const float *values,
+                    size_t n_rows, const double *block, size_t w, const double *block_means,
+                    double *product, double *sums) {
+    double *proj = (double *)malloc(sizeof(double) * (w ? w : 1));
+    for (size_t r = 0; r < n_rows; r++) {
+        for (size_t j = 0; j < w; j++) proj[j] = -block_means[j];
+        for (uint64_t i = indptr[r]; i < indptr[r + 1]; i++) {
+            double v = (double)values[i];
+            const double *b = block + (size_t)indices[i] * w;
+            for (size_t j = 0; j < w; j++) proj[j] += v * b[j];
+        }
+        for (uint64_t i = indptr[r]; i < indptr[r + 1]; i++) {
+            double v = (double)values[i];
+            double *t = product + (size_t)indices[i] * w;
+            for (size_t j = 0; j < w; j++) t[j] += v * proj[j];
+        }
+        for (size_t j = 0; j < w; j++) sums[j] += proj[j];
+    }
+    free(proj);
+}
+
+/* The last pass (Rows arm, pca.rs:649-677): each row's (x - mu) V as
+ * -mean_projection + sum value * loadings[gene], and each gene's centred squares
+ * over its stored values and their count. */
+void ko_pca_rows(const uint64_t *indptr, const uint32_t *indices, const float *values,
+                 size_t n_rows, const double *loadings, size_t k, const double *mean_projection,
+                 const double *means, double *rows, double *squares, uint64_t *stored) {
+    for (size_t r = 0; r < n_rows; r++) {
+        double *out = rows + r * k;
+        for (size_t j = 0; j < k; j++) out[j] = -mean_projection[j];
+        for (uint64_t i = indptr[r]; i < indptr[r + 1]; i++) {
+            size_t g = indices[i];
+            double v = (double)values[i];
+            const double *b = loadings + g * k;
+            for (size_t j = 0; j < k; j++) out[j] += v * b[j];
+            double c = v - means[g];
+            squares[g] += c * c;
+            stored[g] += 1;
+        }
+    }
+}
+
+/* --------------------------------------------------------------- neighbours ---- */
+
+/* Exact keys: kernels.rs:422-448 (`pair_sum`): component j's difference (or
+ * product) in f64 into lane j % 8, then combine_lanes. */
+double ko_squared_distance(const float *a, const float *b, size_t w) {
+    double lanes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t j = 0; j < w; j++) {
+        double d = (double)a[j] - (double)b[j];
+        lanes[j % 8] += d * d;
+    }
+    return combine_lanes(lanes);
+}
+
+double ko_dot(const float *a, const float *b, size_t w) {
+    double lanes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t j = 0; j < w; j++) lanes[j % 8] += (double)a[j] * (double)b[j];
+    return combine_lanes(lanes);
+}
+
+/* cosine_distance: neighbors.rs:440-445. */
+static inline double cosine_distance(double dot, double left, double right) {
+    if (left == 0.0 || right == 0.0) return 1.0;
+    double v = 1.0 - dot / (left * right);
+    return v < 0.0 ? 0.0 : (v > 2.0 ? 2.0 : v);
+}
+
+/* `Nearest`: the k smallest by (key, row), ties to the earlier row:
+ * neighbors.rs:448-480 (`offer` :463-474). */
+typedef struct {
+    size_t k, len;
+    double *keys;
+    uint32_t *rows;
+} nearest_t;
+
+static inline int before(double ek, uint32_t er, double key, uint32_t row) {
+    return ek < key || (ek == key && er < row);
+}
+
+static void nearest_offer(nearest_t *n, double key, uint32_t row) {
+    if (n->len == n->k) {
+        if (before(n->keys[n->k - 1], n->rows[n->k - 1], key, row)) return;
+        n->len--;
+    }
+    size_t at = 0;
+    while (at < n->len && before(n->keys[at], n->rows[at], key, row)) at++;
+    for (size_t i = n->len; i > at; i--) {
+        n->keys[i] = n->keys[i - 1];
+        n->rows[i] = n->rows[i - 1];
+    }
+    n->keys[at] = key;
+    n->rows[at] = row;
+    n->len++;
+}
+
+/* Exposed for the known-answer test of neighbors.rs:555-561. */
+size_t ko_nearest_offer_many(size_t k, const double *keys, const uint32_t *rows, size_t m,
+                             double *out_keys, uint32_t *out_rows) {
+    nearest_t n = {k, 0, out_keys, out_rows};
+    for (size_t i = 0; i < m; i++) nearest_offer(&n, keys[i], rows[i]);
+    return n.len;
+}
+
+/* One row's key to another: `Space::keys` neighbors.rs:389-401. `norms` is NULL
+ * for Euclidean. */
+static inline double space_key(const float *pts, size_t w, const double *norms, size_t q,
+                               size_t p) {
+    if (!norms) return ko_squared_distance(pts + q * w, pts + p * w, w);
+    return cosine_distance(ko_dot(pts + q * w, pts + p * w, w), norms[q], norms[p]);
+}
+
+/* Each row's Euclidean norm for cosine (`Space::new` neighbors.rs:336-350). */
+void ko_norms(const float *pts, size_t n, size_t w, double *norms) {
+    for (size_t r = 0; r < n; r++) norms[r] = sqrt(ko_dot(pts + r * w, pts + r * w, w));
+}
+
+/* `search_row`: neighbors.rs:521-536 (and `brute_force` :484-517, which gives each
+ * row this result): the k nearest other rows of `queries[i]`, ranked by (key, row).
+ * Output k entries per query (keys and rows). */
+void ko_search_rows(const float *pts, size_t n, size_t w, const double *norms, size_t k,
+                    const uint32_t *queries, size_t nq, double *keys, uint32_t *rows) {
+    #pragma omp parallel for schedule(dynamic, 8)
+    for (size_t i = 0; i < nq; i++) {
+        size_t q = queries[i];
+        nearest_t nn = {k, 0, keys + i * k, rows + i * k};
+        for (size_t p = 0; p < n; p++) {
+            if (p == q) continue;
+            nearest_offer(&nn, space_key(pts, w, norms, q, p), (uint32_t)p);
+        }
+    }
+}
+
+/* ---- neighbors_exact.rs: the f32 matrix-product filter + exact re-rank. ---- */
+
+#define QUERY_ROWS 128      /* neighbors_exact.rs:33 */
+#define POINT_ROWS 1024     /* neighbors_exact.rs:36 */
+#define CANDIDATE_LIMIT 4096 /* neighbors_exact.rs:38 */
+static const double UNIT_ROUNDOFF = 1.0 / 16777216.0; /* :41 */
+static const double LARGEST_NORM = 1e15;              /* :43 */
+
+/* The least f32 at or above `value`: neighbors_exact.rs:247-254. */
+static float round_up(double value) {
+    float r = (float)value;
+    if ((double)r < value) r = nextafterf(r, INFINITY);
+    return r;
+}
+float ko_round_up(double v) { return round_up(v); }
+
+typedef struct {
+    size_t k;
+    double slack;
+    float smallest[64];
+    size_t n_smallest;
+    float threshold;
+    float *ckey;
+    uint32_t *crow;
+    size_t ncand, cap, compact_at;
+    int overflowed;
+} query_t;
+
+/* `Query::offer`: neighbors_exact.rs:200-243. */
+static void query_offer(query_t *s, float key, uint32_t row) {
+    if (s->overflowed) return;
+    if (s->n_smallest < s->k || key < s->smallest[s->k - 1]) {
+        if (s->n_smallest < s->k) s->smallest[s->n_smallest++] = key;
+        size_t at = s->n_smallest - 1;
+        while (at > 0 && s->smallest[at - 1] > key) {
+            s->smallest[at] = s->smallest[at - 1];
+            at--;
+        }
+        s->smallest[at] = key;
+        if (s->n_smallest == s->k)
+            s->threshold = round_up((double)s->smallest[s->k - 1] + 2.0 * s->slack);
+    }
+    if (key <= s->threshold) {
+        if (s->ncand == s->cap) {
+            s->cap = s->cap ? 2 * s->cap : 128;
+            s->ckey = (float *)realloc(s->ckey, s->cap * sizeof(float));
+            s->crow = (uint32_t *)realloc(s->crow, s->cap * sizeof(uint32_t));
+        }
+        s->ckey[s->ncand] = key;
+        s->crow[s->ncand] = row;
+        s->ncand++;
+        if (s->ncand >= s->compact_at) {
+            size_t j = 0;
+            for (size_t i = 0; i < s->ncand; i++)
+                if (s->ckey[i] <= s->threshold) {
+                    s->ckey[j] = s->ckey[i];
+                    s->crow[j] = s->crow[i];
+                    j++;
+                }
+            s->ncand = j;
+            if (s->ncand > CANDIDATE_LIMIT) {
+                s->overflowed = 1;
+                s->ncand = 0;
+            } else {
+                s->compact_at = 2 * s->ncand > 64 ? 2 * s->ncand : 64;
+            }
+        }
+    }
+}
+
+/* The f32 dot products of a block of rows with a tile of rows (kernels/products.rs:21;
+ * the bits are not fixed there, and the slack proof makes the result independent
+ * of them). Contraction is allowed in this one function. */
+__attribute__((optimize("fp-contract=fast")))
+static void f32_products(const float *qs, size_t nq, const float *ps, size_t np, size_t w,
+                         float *out) {
+    for (size_t i = 0; i < nq; i++) {
+        const float *a = qs + i * w;
+        for (size_t j = 0; j < np; j++) {
+            const float *b = ps + j * w;
+            float s = 0.0f;
+            for (size_t c = 0; c < w; c++) s += a[c] * b[c];
+            out[i * np + j] = s;
+        }
+    }
+}
+
+/* Exact neighbours through the filter: neighbors_exact.rs:50-74 (`exact`),
+ * `Prepared::new` :92-180 (Euclidean and cosine), `search_block` :257-307, `scan`
+ * :313-334, `rank` :338-355; the fallback is `search_row`. Returns 1 when the
+ * filter ran, 0 when norms forced brute force (`Prepared::new` -> None). */
+int ko_knn_exact(const float *pts, size_t n, size_t w, int cosine, size_t k, double *keys_out,
+                 uint32_t *rows_out, uint64_t *fallback_rows) {
+    double factor = 2.0 * ((double)w + 8.0) * UNIT_ROUNDOFF;
+    double underflow = 4.0 * ((double)w + 8.0) * (double)FLT_MIN;
+    float *rows = (float *)malloc(sizeof(float) * n * w);
+    float *offsets = NULL;
+    double *slack = (double *)malloc(sizeof(double) * n);
+    double *norms = NULL;
+    if (!cosine) {
+        double *mean = (double *)calloc(w, sizeof(double));
+        for (size_t r = 0; r < n; r++)
+            for (size_t c = 0; c < w; c++) mean[c] += (double)pts[r * w + c];
+        for (size_t c = 0; c < w; c++) mean[c] /= (double)n;
+        double *squares = (double *)malloc(sizeof(double) * n);
+        double largest = 0.0;
+        for (size_t r = 0; r < n; r++) {
+            double sq = 0.0;
+            for (size_t c = 0; c < w; c++) {
+                float v = (float)((double)pts[r * w + c] - mean[c]);
+                rows[r * w + c] = v;
+                sq += (double)v * (double)v;
+            }
+            squares[r] = sq;
+            if (sq > largest) largest = sq;
+        }
+        free(mean);
+        largest = sqrt(largest);
+        if (!isfinite(largest) || largest > LARGEST_NORM) {
+            free(squares); free(rows); free(slack);
+            uint32_t *all = (uint32_t *)malloc(sizeof(uint32_t) * n);
+            for (size_t r = 0; r < n; r++) all[r] = (uint32_t)r;
+            ko_search_rows(pts, n, w, NULL, k, all, n, keys_out, rows_out);
+            free(all);
+            if (fallback_rows) *fallback_rows = n;
+            return 0;
+        }
+        offsets = (float *)malloc(sizeof(float) * n);
+        for (size_t r = 0; r < n; r++) {
+            double t = sqrt(squares[r]) + largest;
+            slack[r] = factor * t * t + underflow;
+            offsets[r] = (float)squares[r];
+        }
+        free(squares);
+    } else {
+        norms = (double *)malloc(sizeof(double) * n);
+        ko_norms(pts, n, w, norms);
+        for (size_t r = 0; r < n; r++) {
+            for (size_t c = 0; c < w; c++)
+                rows[r * w + c] = norms[r] == 0.0 ? 0.0f : (float)((double)pts[r * w + c] / norms[r]);
+            slack[r] = factor + underflow;
+        }
+    }
+    uint64_t fallbacks = 0;
+    size_t n_blocks = (n + QUERY_ROWS - 1) / QUERY_ROWS;
+    #pragma omp parallel reduction(+ : fallbacks)
+    {
+        float *products = (float *)malloc(sizeof(float) * QUERY_ROWS * POINT_ROWS);
+        query_t states[QUERY_ROWS];
+        #pragma omp for schedule(dynamic, 1)
+        for (size_t b = 0; b < n_blocks; b++) {
+            size_t first = b * QUERY_ROWS, last = first + QUERY_ROWS < n ? first + QUERY_ROWS : n;
+            size_t nq = last - first;
+            for (size_t i = 0; i < nq; i++) {
+                memset(&states[i], 0, sizeof(query_t));
+                states[i].k = k;
+                states[i].slack = slack[first + i];
+                states[i].threshold = INFINITY;
+                states[i].compact_at = 64;
+            }
+            for (size_t start = 0; start < n; start += POINT_ROWS) {
+                size_t end = start + POINT_ROWS < n ? start + POINT_ROWS : n, np = end - start;
+                f32_products(rows + first * w, nq, rows + start * w, np, w, products);
+                for (size_t i = 0; i < nq; i++) {
+                    size_t query = first + i;
+                    float *kk = products + i * np;
+                    if (!cosine) {
+                        float o = offsets[query];
+                        for (size_t j = 0; j < np; j++) kk[j] = (o + offsets[start + j]) - 2.0f * kk[j];
+                    } else {
+                        for (size_t j = 0; j < np; j++) {
+                            float v = 1.0f - kk[j];
+                            kk[j] = v < 0.0f ? 0.0f : (v > 2.0f ? 2.0f : v);
+                        }
+                    }
+                    /* `scan`: offer every key at most the threshold, except the query. */
+                    for (size_t j = 0; j < np; j++) {
+                        size_t row = start + j;
+                        if (kk[j] <= states[i].threshold && row != query)
+                            query_offer(&states[i], kk[j], (uint32_t)row);
+                    }
+                }
+            }
+            for (size_t i = 0; i < nq; i++) {
+                size_t query = first + i;
+                query_t *s = &states[i];
+                double *ok = keys_out + query * k;
+                uint32_t *orow = rows_out + query * k;
+                nearest_t nn = {k, 0, ok, orow};
+                int good = !s->overflowed;
+                for (size_t c = 0; good && c < s->ncand; c++) {
+                    if (s->ckey[c] > s->threshold) continue;
+                    double exact = space_key(pts, w, norms, query, s->crow[c]);
+                    if (fabs(exact - (double)s->ckey[c]) > s->slack) good = 0;
+                    else nearest_offer(&nn, exact, s->crow[c]);
+                }
+                if (!good || nn.len != k) {
+                    uint32_t qq = (uint32_t)query;
+                    ko_search_rows(pts, n, w, norms, k, &qq, 1, ok, orow);
+                    fallbacks++;
+                }
+                free(s->ckey);
+                free(s->crow);
+            }
+        }
+        free(products);
+    }
+    if (fallback_rows) *fallback_rows = fallbacks;
+    free(rows); free(offsets); free(slack); free(norms);
+    return 1;
+}
+
+int ko_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void ko_set_threads(int t) {
+#ifdef _OPENMP
+    omp_set_num_threads(t);
+#else
+    (void)t;
+#endif
+}
