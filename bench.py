@@ -1,0 +1,174 @@
+"""Benchmark of the embedding path: pca(50) + neighbors(15) on one MI355X.
+
+A step is one pass of the path over the resident layer: the streamed block-Krylov
+PCA (q product passes, FP64 dense steps on the host, embedding left on the device),
+then the kNN graph of the embedding, downloaded to the host. The layer is the
+synthetic 1,000,000 cells x 2,000 genes log1p CSR (20 cell types, ~8% density),
+uploaded before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--workload 1m] [--search exact|learned]
+                    [--precision bf16|fp32] [--q 4] [--no-cpu-baseline]
+
+Prints ONE JSON line (the driver's contract). --gpus N with N > 1 exits: this path
+runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+PEAKS_TFLOPS = {"bf16": 2500.0, "fp32": 157.3, "fp64": 78.6}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(layer, embedding, n_genes, q, threads):
+    """The oracle (kyto's CPU path restated) on a bounded sample of the workload:
+    PCA on the first `pca_cells` cells (its cost is linear in cells), and the filtered
+    exact kNN for `knn_queries` query rows against all rows (linear in queries)."""
+    import oracle
+
+    n = len(layer.indptr) - 1
+    pca_cells = min(n, 50_000)
+    ip = layer.indptr[: pca_cells + 1]
+    nnz = int(ip[-1])
+    t0 = time.perf_counter()
+    oracle.pca(ip, layer.indices[:nnz], layer.values[:nnz], n_genes, 50, seed=0, q=q)
+    pca_s = time.perf_counter() - t0
+    knn_queries = min(n, 4096)
+    t0 = time.perf_counter()
+    oracle.knn_exact_range(embedding, 14, 0, knn_queries, threads=threads)
+    knn_s = time.perf_counter() - t0
+    pca_rate = pca_cells / pca_s
+    knn_rate = knn_queries / knn_s
+    value = 1.0 / (1.0 / pca_rate + 1.0 / knn_rate)
+    return {
+        "value": value,
+        "unit": "cells/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle pca(50, q={q}) fold on 1 thread over the first {pca_cells} cells "
+                   f"({pca_s:.1f} s, {pca_rate:.0f} cells/s) + oracle filtered exact kNN(15) of "
+                   f"{knn_queries} queries against all {n} rows on {threads} threads "
+                   f"({knn_s:.1f} s, {knn_rate:.0f} queries/s); value = 1/(1/pca + 1/knn)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="1m")
+    ap.add_argument("--search", default="exact", choices=["exact", "learned"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
+    ap.add_argument("--q", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this path runs on one GPU: every run here uses one device; "
+                                   "see DESIGN.md for how it would shard"}))
+        sys.exit(2)
+
+    import kyto_amd
+    from kyto_amd import synthetic as S
+
+    w = S.WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    layer = S.workload(args.workload)
+    log(f"workload {w.name}: {w.n_cells} x {w.n_genes}, nnz {layer.nnz} (density {layer.density:.4f}), "
+        f"generated in {time.perf_counter() - t0:.1f} s")
+
+    dev = kyto_amd.Device(0)
+    csr = dev.upload_csr(layer.indptr, layer.indices, layer.values, w.n_genes)
+    log("layer resident on", dev.info())
+    n = w.n_cells
+    k = 15
+
+    def step():
+        res = dev.pca(csr, 50, seed=0, power_iterations=args.q, download_embedding=False)
+        emb = res.device_embedding
+        if args.search == "exact":
+            g = dev.neighbors(emb, k, precision=args.precision)
+        else:
+            raise SystemExit("learned search is not built yet")
+        return res, g, emb
+
+    for i in range(args.warmup):
+        res, g, emb = step()
+        log(f"warmup {i}: pca {res.timing['total_ms']:.1f} ms, knn {g.info['total_ms']:.1f} ms")
+        emb.free()
+    times, pcas, knns = [], [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        res, g, emb = step()
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        pcas.append(res.timing)
+        knns.append(g.info)
+        log(f"step {i}: {dt * 1e3:.1f} ms (pca {res.timing['total_ms']:.1f} ms, knn {g.info['total_ms']:.1f} ms, "
+            f"search {g.info['search_ms']:.1f}, fallback rows {g.info['fallback']})")
+        if i < args.steps - 1:
+            emb.free()
+    ms = 1e3 * float(np.mean(times))
+    value = n / (ms / 1e3)
+    # Dominant kernel: the candidate key search, 2 n^2 d flops per full search.
+    d = 50
+    search_ms = float(np.mean([x["search_ms"] for x in knns]))
+    flops = 2.0 * n * n * d
+    achieved = flops / (search_ms / 1e3) / 1e12
+    peak = PEAKS_TFLOPS[args.precision]
+    embedding = emb.download()
+    emb.free()
+    out = {
+        "metric": "cells/s pca(50)+neighbors(15) @1M×2k",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic",
+        "config": {"workload": f"{w.n_cells} cells x {w.n_genes} genes log1p CSR, {w.n_types} cell types, "
+                               f"density {layer.density:.4f}, nnz {layer.nnz}",
+                   "pca": f"n_components=50, block 60, q={args.q}, seed 0",
+                   "neighbors": f"n_neighbors=15, euclidean, search={args.search}, candidates {args.precision}"},
+        "roofline": {"bound": "mfma" if args.precision == "bf16" else "valu", "kernel": "candidate key search",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None},
+        "breakdown": {
+            "pca_ms": float(np.mean([p["total_ms"] for p in pcas])),
+            "pca_pass_ms": pcas[-1]["pass_ms"],
+            "pca_dense_ms": float(np.mean([p["dense_ms"] for p in pcas])),
+            "knn_ms": float(np.mean([x["total_ms"] for x in knns])),
+            "knn_search_ms": search_ms,
+            "knn_rerank_ms": float(np.mean([x["rerank_ms"] for x in knns])),
+            "knn_fallback_ms": float(np.mean([x["fallback_ms"] for x in knns])),
+            "knn_fallback_rows": int(knns[-1]["fallback"]),
+            "knn_candidates": int(knns[-1]["candidates"]),
+        },
+    }
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(layer, embedding, w.n_genes, args.q, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

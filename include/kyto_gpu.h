@@ -1,0 +1,216 @@
+/*
+ * kyto_gpu.h — C ABI of libkyto_gpu.so, the MI355X (gfx950) embedding path behind
+ * kyto's pca() and neighbors().
+ *
+ * Conventions (those of kyto-gpu's DeviceKernels, crates/kyto-gpu/src/launch.rs:42-93):
+ * - every call returns an int status: KY_OK (0), or a kyto error code (KY_E215 for
+ *   invalid arguments or data, KY_E501 when no kernel serves the request, KY_E503 when
+ *   cancelled, KY_E504 when the device is not found, KY_E505 for a device failure,
+ *   launch.rs:62-69, errors.toml:790-845); ky_last_error() gives the message of the
+ *   calling thread's last failure;
+ * - host buffers are borrowed and caller-owned; outputs are written in place;
+ * - no C++ exception crosses the ABI;
+ * - a context is used from one host thread at a time (calls on it are serialised on
+ *   its stream; each call returns when its results are in the caller's buffers).
+ */
+#ifndef KYTO_GPU_H
+#define KYTO_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KY_OK 0
+#define KY_E215 215 /* invalid argument or data (KytoError::InvalidArgument) */
+#define KY_E501 501 /* no kernel for the placed step */
+#define KY_E503 503 /* cancelled */
+#define KY_E504 504 /* device not found */
+#define KY_E505 505 /* device failure (KytoError::DeviceFailure) */
+
+#define KY_METRIC_EUCLIDEAN 0
+#define KY_METRIC_COSINE 1
+
+/* Key precision of the candidate search (the exact re-rank is always f64). */
+#define KY_PRECISION_FP32 0 /* plain FP32 keys */
+#define KY_PRECISION_BF16 1 /* BF16 MFMA keys, FP32 accumulation */
+
+typedef struct ky_ctx ky_ctx;     /* one device, one stream */
+typedef struct ky_csr ky_csr;     /* a CSR layer resident in device memory */
+typedef struct ky_dense ky_dense; /* a row-major f32 matrix resident in device memory */
+typedef struct ky_ivf ky_ivf;     /* an IVF index (k-means lists + router) on the device */
+
+/* The calling thread's last error message ("" if none). Replaces the
+ * KytoError::DeviceFailure payload (launch.rs:62-69). */
+const char *ky_last_error(void);
+
+/* Number of visible devices (kyto_gpu::probe, crates/kyto-gpu/src/probe.rs:19-33). */
+int ky_device_count(int *count);
+
+/* Open / close a device context (DeviceKernels::open, launch.rs:81-93). */
+int ky_open(int device, ky_ctx **out);
+int ky_close(ky_ctx *ctx);
+/* Device name and total memory, for explain()/provenance (device.rs:24-64). */
+int ky_device_info(ky_ctx *ctx, char *name, size_t name_len, uint64_t *total_bytes);
+
+/* ---------------------------------------------------------------- CSR layers ---- */
+
+/* Upload a CSR layer (cells x genes, indices strictly increasing within a row) and
+ * keep it resident for every pass. Replaces the per-call SparseRows copies of
+ * launch_matrix.rs:31-84 and the host CSC of launch_matrix.rs:405-424: the device
+ * keeps the rows and a chunked column-major copy built on the device.
+ * KY_E215 if a value is not finite or outside 1e-10..1e10 in magnitude
+ * (pca/gpu.rs:103-105,120: GPU_RANGE), an index is out of range, or indptr is not
+ * a row pointer; KY_E505 past 2^32 stored values ("pass a smaller memory_limit",
+ * pca/gpu.rs:62-70). */
+int ky_csr_upload(ky_ctx *ctx, const uint64_t *indptr, const uint32_t *indices,
+                  const float *values, uint64_t n_rows, uint32_t n_cols, ky_csr **out);
+int ky_csr_free(ky_csr *csr);
+int ky_csr_shape(const ky_csr *csr, uint64_t *n_rows, uint32_t *n_cols, uint64_t *nnz);
+
+/* Device-resident dense matrices (the PCA embedding, X_pca). */
+int ky_dense_upload(ky_ctx *ctx, const float *host, uint64_t rows, uint32_t cols, ky_dense **out);
+int ky_dense_download(const ky_dense *m, float *host);
+int ky_dense_shape(const ky_dense *m, uint64_t *rows, uint32_t *cols);
+int ky_dense_free(ky_dense *m);
+
+/* ------------------------------------------------------------- PCA passes ---- */
+/* One pass each over the resident layer, as kyto's GPU fold computes a leaf
+ * (crates/kyto-exec/src/pca/gpu.rs:149-219), here with exact f32 x f32 products
+ * accumulated in f64 and fixed-order (deterministic) reductions. */
+
+/* Pass 0 (pca/gpu.rs:159-178; pca.rs:588-622): X^T Omega (genes x l, row-major),
+ * the gene sums and Omega's column sums, Omega drawn on the device per cell from
+ * (seed, cell_offset + row) by kyto's SplitMix64 rows (pca.rs:936-950) and rounded
+ * to f32 (its column sums from the rounded values). */
+int ky_pca_sketch(ky_ctx *ctx, const ky_csr *csr, uint64_t seed, uint64_t cell_offset,
+                  uint32_t l, double *sketch, double *gene_sums, double *omega_sums);
+
+/* Passes 1..q (pca/gpu.rs:179-191; pca.rs:623-645): each row's centred projection
+ * w = x Q - block_means (rounded to f32), then X^T W (genes x w) and the sum of the
+ * rows w (f64). Q is genes x w row-major, w <= 64 x 16. Replaces sparse_dense +
+ * sparse_transposed_dense (launch_matrix.rs:183-238). */
+int ky_pca_product(ky_ctx *ctx, const ky_csr *csr, const double *q, uint32_t w,
+                   const double *block_means, double *product, double *sums);
+
+/* The last pass (pca/gpu.rs:192-214; pca.rs:649-677): the embedding rows
+ * (x - mu) V = x V - mean_projection (n x k, f32, left on the device in *embedding
+ * when it is not NULL, and copied to `host_embedding` when that is not NULL), each
+ * gene's centred squares over its stored values and their count. */
+int ky_pca_project(ky_ctx *ctx, const ky_csr *csr, const double *v, uint32_t k,
+                   const double *mean_projection, const double *means, float *host_embedding,
+                   ky_dense **embedding, double *squares, uint64_t *stored);
+
+/* The whole pca() (PcaAccumulator with GPU folds, pca.rs:302-862): the passes above
+ * with kyto's end_pass / extend_basis / rayleigh_ritz in FP64 on the host
+ * (pca_dense.rs:105-174). n_components >= 1; power_iterations = q (kyto's constant
+ * is 9, physical_plan.rs:73). Outputs: embedding (device, and host if not NULL),
+ * loadings f32 genes x k, variance and variance_ratio f64 k, means f64 genes.
+ * `passes_out` (may be NULL) receives the product passes run (fewer than q when the
+ * basis spans the data early). KY_E215 for fewer than 2 cells or n_components above
+ * min(cells, genes) (pca.rs:876-920). */
+typedef struct ky_pca_timing {
+    double upload_ms;   /* unused by ky_pca (the layer is already resident) */
+    double pass_ms[16]; /* device time of each pass */
+    double dense_ms;    /* host FP64 steps */
+    double total_ms;
+    uint32_t n_passes;
+} ky_pca_timing;
+
+int ky_pca(ky_ctx *ctx, const ky_csr *csr, uint32_t n_components, uint64_t seed,
+           uint32_t power_iterations, float *host_embedding, ky_dense **embedding,
+           float *loadings, double *variance, double *variance_ratio, double *means,
+           uint32_t *passes_out, ky_pca_timing *timing);
+
+/* ------------------------------------------------------------- neighbours ---- */
+
+/* Statistics of a search. */
+typedef struct ky_knn_info {
+    uint64_t candidates;    /* candidates kept per query (m) */
+    uint64_t certified;     /* rows whose candidates were proven to hold the k nearest */
+    uint64_t fallback;      /* rows searched again exactly against every row */
+    double search_ms, rerank_ms, fallback_ms, total_ms;
+} ky_knn_info;
+
+/* Exact neighbours of every row of `points` (its first `n_components` columns), as
+ * kyto's exact search ranks them (neighbors.rs:226-312): for each row the
+ * n_neighbors - 1 nearest other rows by (f64 key in kyto's 8-lane order, row), the
+ * row itself excluded by position. Candidates come from the device's key search at
+ * `precision`; each is re-ranked with the exact f64 key on the device, rows whose
+ * candidates cannot be shown to hold their neighbours (neighbors_gpu.rs:50-54,
+ * 132-159) are searched again exactly on the device. Output: CSR-ordered graph
+ * (neighbors.rs:412-435): per row, indices increasing (n x (n_neighbors-1) u32) and
+ * f32 distances (sqrt of the f64 key for Euclidean). Replaces nearest_keys
+ * (launch_matrix.rs:253-286) and neighbors_gpu::exact (neighbors_gpu.rs:70-127). */
+int ky_neighbors(ky_ctx *ctx, const ky_dense *points, uint32_t n_components,
+                 uint32_t n_neighbors, int metric, int precision, uint32_t *indices,
+                 float *distances, ky_knn_info *info);
+
+/* The candidate search alone (DeviceKernels::nearest_keys, launch_matrix.rs:253-286):
+ * for each of the rows [first, first + count) of `points`, its m nearest other rows
+ * by the approximate key at `precision`, ascending (keys f32 count x m, rows u32
+ * count x m). */
+int ky_nearest_keys(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uint64_t first,
+                    uint64_t count, uint32_t m, int precision, float *keys, uint32_t *rows);
+
+/* ------------------------------------------------------ learned IVF router ---- */
+/* No counterpart in the reference (its approximate search is an HNSW index,
+ * neighbors_hnsw.rs): an IVF index over the embedding with a learned router. */
+
+typedef struct ky_ivf_params {
+    uint32_t n_lists;       /* k-means lists (4096) */
+    uint32_t kmeans_iters;  /* Lloyd iterations */
+    uint64_t seed;
+    uint32_t hidden;        /* router hidden width (256) */
+    uint32_t train_queries; /* sampled training queries (200,000) */
+    uint32_t batch;         /* 4096 */
+    uint32_t epochs;
+    float lr;               /* Adam learning rate (1e-3) */
+    uint32_t label_k;       /* neighbours whose lists label a query (n_neighbors - 1) */
+} ky_ivf_params;
+
+typedef struct ky_ivf_info {
+    double kmeans_ms, labels_ms, train_ms;
+    float final_loss;
+    float label_recall_top;  /* share of label mass in the router's top-probe lists */
+} ky_ivf_info;
+
+int ky_ivf_build(ky_ctx *ctx, const ky_dense *points, const ky_ivf_params *params,
+                 ky_ivf **out, ky_ivf_info *info);
+int ky_ivf_free(ky_ivf *ivf);
+
+typedef struct ky_ivf_search_info {
+    uint64_t fallback;      /* queries below the confidence threshold searched exactly */
+    double route_ms, search_ms, total_ms;
+} ky_ivf_search_info;
+
+/* Approximate neighbours: each row's top `n_probe` lists (from the router when
+ * `use_router` is 1, else the nearest centroids), their points re-ranked exactly
+ * (f64 keys, (key, row) order), rows whose router confidence (probability mass of
+ * the probed lists) is below `confidence` searched exactly instead. Output as
+ * ky_neighbors. */
+int ky_ivf_search(ky_ctx *ctx, const ky_ivf *ivf, const ky_dense *points, uint32_t n_neighbors,
+                  uint32_t n_probe, int use_router, float confidence, uint32_t *indices,
+                  float *distances, ky_ivf_search_info *info);
+
+/* -------------------------------------------- host-only FP64 dense steps ---- */
+/* The FP64 steps of the PCA on the host, exported so they can be checked without a
+ * device. Row-major matrices. */
+
+/* extend_basis (pca_dense.rs:105-129): the orthonormal directions `block` (rows x bw)
+ * adds to `basis` (rows x bc); writes up to bw columns to `out` (rows x bw, the
+ * first *kept columns used) and *kept. */
+int ky_dense_extend_basis(const double *basis, uint32_t rows, uint32_t bc, const double *block,
+                          uint32_t bw, double *out, uint32_t *kept);
+
+/* rayleigh_ritz (pca_dense.rs:141-174): eigenvalues (k) and loadings (rows x k). */
+int ky_dense_rayleigh_ritz(const double *basis, const double *products, uint32_t rows,
+                           uint32_t cols, uint32_t k, double *eigenvalues, double *loadings);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KYTO_GPU_H */

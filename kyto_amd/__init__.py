@@ -1,0 +1,301 @@
+"""kyto_amd: the MI355X embedding path of kyto — pca() then neighbors() — behind
+kyto's plan API (crates/kyto-py/src/atlas.rs:822-966, 1622-1720).
+
+    import kyto_amd as ky
+    out = (ky.scan(layer)                       # a CSR layer (cells x genes)
+             .pca(n_components=50, seed=0)      # atlas.rs:822-856
+             .neighbors(n_neighbors=15)         # atlas.rs:857-966
+             .on(device="gpu")                  # atlas.rs:1622-1636
+             .collect(memory_limit="32GB"))     # atlas.rs:1691-1720
+    out["X_pca"], out["distances"]
+
+Chained methods only build a plan; collect() runs it on the device. There is no CPU
+path: on(device="cpu") is refused with KY-E501 and a missing native library raises.
+"""
+from __future__ import annotations
+
+import re
+from dataclasses import dataclass, field, replace
+from typing import Any
+
+import numpy as np
+
+from ._native import KytoError, KY_E215, KY_E501
+from .device import CsrLayer, DenseMatrix, Device, Graph, IvfIndex, PcaOutput, device_count
+
+__all__ = ["scan", "Atlas", "Device", "KytoError", "Graph", "PcaOutput", "device_count", "CsrLayer", "DenseMatrix",
+           "IvfIndex"]
+
+# crates/kyto-plan/src/physical_plan.rs:73
+PCA_POWER_ITERATIONS = 9
+
+
+def _invalid(function, parameter, value, problem, accepted) -> KytoError:
+    return KytoError(KY_E215, f"{function}: {parameter}={value!r}: {problem} (accepted: {accepted})")
+
+
+def _integer(function, parameter, value, minimum):
+    if isinstance(value, bool) or not isinstance(value, (int, np.integer)):
+        raise _invalid(function, parameter, value, "not an int", f"an int of at least {minimum}")
+    if value < minimum:
+        raise _invalid(function, parameter, value, f"below {minimum}", f"an int of at least {minimum}")
+    return int(value)
+
+
+_UNITS = {"b": 1, "kb": 1e3, "mb": 1e6, "gb": 1e9, "tb": 1e12, "kib": 2**10, "mib": 2**20, "gib": 2**30,
+          "tib": 2**40}
+
+
+def parse_memory(limit) -> int | None:
+    """`memory_limit=` as kyto's terminals take it: bytes, or '32GB' / '512MiB'."""
+    if limit is None:
+        return None
+    if isinstance(limit, (int, np.integer)) and not isinstance(limit, bool):
+        if limit <= 0:
+            raise _invalid("collect", "memory_limit", limit, "not positive", "a positive size")
+        return int(limit)
+    m = re.fullmatch(r"\s*([0-9]*\.?[0-9]+)\s*([a-zA-Z]*)\s*", str(limit))
+    if not m or m.group(2).lower() not in _UNITS or float(m.group(1)) <= 0:
+        raise _invalid("collect", "memory_limit", limit, "not a size", "an int of bytes or a string like '32GB'")
+    return int(float(m.group(1)) * _UNITS[m.group(2).lower()])
+
+
+@dataclass(frozen=True)
+class _Layer:
+    indptr: np.ndarray
+    indices: np.ndarray
+    values: np.ndarray
+    n_genes: int
+
+    @property
+    def n_cells(self):
+        return len(self.indptr) - 1
+
+    @property
+    def nnz(self):
+        return int(self.indptr[-1])
+
+
+def _as_layer(x) -> _Layer:
+    if hasattr(x, "indptr") and hasattr(x, "indices"):
+        values = getattr(x, "values", None)
+        if values is None:
+            values = getattr(x, "data")
+        n_genes = getattr(x, "n_genes", None)
+        if n_genes is None:
+            n_genes = x.shape[1]
+        return _Layer(np.ascontiguousarray(x.indptr, np.uint64), np.ascontiguousarray(x.indices, np.uint32),
+                      np.ascontiguousarray(values, np.float32), int(n_genes))
+    if isinstance(x, tuple) and len(x) == 4:
+        ip, ix, v, g = x
+        return _Layer(np.ascontiguousarray(ip, np.uint64), np.ascontiguousarray(ix, np.uint32),
+                      np.ascontiguousarray(v, np.float32), int(g))
+    raise _invalid("scan", "layer", type(x).__name__, "not a CSR layer",
+                   "an object with indptr/indices/data (or values) and shape/n_genes, or (indptr, indices, values, n_genes)")
+
+
+@dataclass(frozen=True)
+class _Pca:
+    layer: str | None
+    into: str
+    n_components: int
+    seed: int
+    genes: Any = None
+
+
+@dataclass(frozen=True)
+class _Neighbors:
+    embedding: str | None
+    n_neighbors: int
+    n_components: int | None
+    metric: str
+    search: str
+    seed: int
+    into: str | None
+    precision: str
+
+
+@dataclass(frozen=True)
+class Atlas:
+    """A lazy plan over CSR layers (kyto's Atlas for the embedding path)."""
+
+    layers: dict
+    default_layer: str
+    steps: tuple = ()
+    device: Any = "auto"
+
+    # ----------------------------------------------------------------- steps --
+
+    def pca(self, *, layer=None, into=None, n_components=None, seed=None, genes=None) -> "Atlas":
+        """atlas.rs:822-856: n_components defaults to 50, seed is required."""
+        if seed is None:
+            raise _invalid("pca", "seed", None, "a seed is required (R7)", "an int of at least 0")
+        seed = _integer("pca", "seed", seed, 0)
+        n_components = 50 if n_components is None else _integer("pca", "n_components", n_components, 1)
+        name = layer if layer is not None else self.default_layer
+        if name not in self.layers:
+            raise KytoError(109, f"pca: layer {name!r} is not a layer of the plan")
+        if genes is not None:
+            genes = np.asarray(genes, bool)
+            if genes.shape != (self.layers[name].n_genes,):
+                raise _invalid("pca", "genes", "mask", "not one flag per gene", "a boolean mask over the genes")
+            if not genes.any():
+                raise _invalid("pca", "genes", "mask", "the predicate is true for none of the genes reaching pca",
+                               f"a predicate true for at least {n_components} genes (n_components)")
+        return replace(self, steps=self.steps + (_Pca(name, into or "X_pca", n_components, seed, genes),))
+
+    def neighbors(self, embedding=None, /, *, n_neighbors=None, n_components=None, metric="euclidean",
+                  search="exact", seed=None, into=None, precision="bf16") -> "Atlas":
+        """atlas.rs:857-966: n_neighbors (default 15) counts the cell itself; search
+        'exact' is exact (candidates at `precision`, re-ranked and certified exactly),
+        'approximate' is the learned IVF router."""
+        n_neighbors = 15 if n_neighbors is None else _integer("neighbors", "n_neighbors", n_neighbors, 0)
+        if n_neighbors < 2:
+            raise _invalid("neighbors", "n_neighbors", n_neighbors, "it counts the cell itself",
+                           "an int of at least 2")
+        if n_components is not None:
+            n_components = _integer("neighbors", "n_components", n_components, 1)
+        seed = 0 if seed is None else _integer("neighbors", "seed", seed, 0)
+        if metric not in ("euclidean", "cosine"):
+            raise _invalid("neighbors", "metric", metric, "unknown", "'euclidean' or 'cosine'")
+        if search not in ("exact", "approximate"):
+            raise _invalid("neighbors", "search", search, "unknown", "'exact' or 'approximate'")
+        if precision not in ("fp32", "bf16"):
+            raise _invalid("neighbors", "precision", precision, "unknown", "'fp32' or 'bf16'")
+        if embedding is None:
+            pcas = [s for s in self.steps if isinstance(s, _Pca)]
+            embedding = pcas[-1].into if pcas else "X_pca"
+        if not any(isinstance(s, _Pca) and s.into == embedding for s in self.steps):
+            raise KytoError(109, f"neighbors: embedding {embedding!r} is not a cell embedding of the plan")
+        return replace(self, steps=self.steps + (
+            _Neighbors(embedding, n_neighbors, n_components, metric, search, seed, into, precision),))
+
+    def on(self, device="auto") -> "Atlas":
+        """atlas.rs:1622-1636: 'auto', 'gpu', 'gpu:N', an int N, or 'cpu'."""
+        if isinstance(device, (int, np.integer)) and not isinstance(device, bool):
+            ordinal = int(device)
+        elif isinstance(device, str) and device in ("auto", "gpu", "cuda", "cpu"):
+            ordinal = device
+        elif isinstance(device, str) and re.fullmatch(r"(gpu|cuda):\d+", device):
+            ordinal = int(device.split(":")[1])
+        else:
+            raise _invalid("on", "device", device, "unknown device", "'auto', 'gpu', 'gpu:N', 'cpu' or an int")
+        return replace(self, device=ordinal)
+
+    def explain(self) -> str:
+        lines = []
+        for s in reversed(self.steps):
+            if isinstance(s, _Pca):
+                lines.append(f'PCA "{s.layer}" -> "{s.into}" n_components={s.n_components} seed={s.seed} '
+                             f"[device: {self.device}]")
+            else:
+                lines.append(f'NEIGHBORS "{s.embedding}" n_neighbors={s.n_neighbors} metric={s.metric} '
+                             f"search={s.search} precision={s.precision} [device: {self.device}]")
+        lines.append(f"SCAN ({', '.join(self.layers)})")
+        return "\n".join(lines)
+
+    # -------------------------------------------------------------- terminal --
+
+    def collect(self, *, memory_limit=None, threads=None, progress=None) -> dict:
+        """Run the plan on the device; returns a dict of the outputs (embeddings,
+        loadings, variances, graphs)."""
+        limit = parse_memory(memory_limit)
+        if threads is not None:
+            _integer("collect", "threads", threads, 1)
+        if self.device == "cpu":
+            raise KytoError(KY_E501, "this build runs pca and neighbors on the GPU only: on(device='gpu')")
+        ordinal = 0 if self.device in ("auto", "gpu", "cuda") else int(self.device)
+        out: dict = {}
+        dev = Device(ordinal)
+        resident: dict[str, DenseMatrix] = {}
+        try:
+            for step in self.steps:
+                if isinstance(step, _Pca):
+                    layer = self.layers[step.layer]
+                    _fits(layer, step, limit)
+                    res, emb = _run_pca(dev, layer, step)
+                    resident[step.into] = emb
+                    out[step.into] = res.embedding
+                    out[f"{step.into}_loadings"] = res.loadings
+                    out[f"{step.into}_variance"] = res.variance
+                    out[f"{step.into}_variance_ratio"] = res.variance_ratio
+                    out[f"{step.into}_means"] = res.means
+                    if progress:
+                        progress(("pca", step.into))
+                else:
+                    emb = resident[step.embedding]
+                    g = _run_neighbors(dev, emb, step)
+                    key = "distances" if step.into is None else f"{step.into}_distances"
+                    out[key] = g
+                    if progress:
+                        progress(("neighbors", key))
+        finally:
+            for m in resident.values():
+                m.free()
+            dev.close()
+        return out
+
+
+def _fits(layer: _Layer, step: _Pca, limit):
+    if limit is None:
+        return
+    n, g = layer.n_cells, layer.n_genes
+    width = min(step.n_components + 10, g) + 1
+    need = layer.nnz * 14 + (n + 1) * 8 + n * width * 4 + n * step.n_components * 4
+    if need > limit:
+        raise KytoError(KY_E215, f"collect: the resident layer needs {need} bytes of device memory, above "
+                                 f"memory_limit={limit}; pass a larger memory_limit")
+
+
+def _run_pca(dev: Device, layer: _Layer, step: _Pca):
+    if step.genes is None:
+        csr = dev.upload_csr(layer.indptr, layer.indices, layer.values, layer.n_genes)
+        try:
+            res = dev.pca(csr, step.n_components, step.seed, PCA_POWER_ITERATIONS)
+        finally:
+            csr.free()
+        return res, res.device_embedding
+    # A gene set: the layer's columns of the set, in order (pca.rs: the sums run in
+    # the order a filter_genes before the node gives them), zero loadings elsewhere.
+    flags = step.genes
+    pos = np.cumsum(flags) - 1
+    keep = flags[layer.indices]
+    rows = np.repeat(np.arange(layer.n_cells), np.diff(layer.indptr.astype(np.int64)))[keep]
+    indptr = np.zeros(layer.n_cells + 1, np.uint64)
+    np.cumsum(np.bincount(rows, minlength=layer.n_cells), out=indptr[1:])
+    n_set = int(flags.sum())
+    if step.n_components > min(layer.n_cells, n_set):
+        raise _invalid("pca", "n_components", step.n_components,
+                       f"{layer.n_cells} cells and {n_set} genes (those of its gene set, `genes`) reach pca",
+                       f"an int from 1 to {min(layer.n_cells, n_set)}")
+    csr = dev.upload_csr(indptr, pos[layer.indices[keep]].astype(np.uint32), layer.values[keep], n_set)
+    try:
+        res = dev.pca(csr, step.n_components, step.seed, PCA_POWER_ITERATIONS)
+    finally:
+        csr.free()
+    full = np.zeros((layer.n_genes, step.n_components), np.float32)
+    full[flags] = res.loadings
+    res.loadings = full
+    return res, res.device_embedding
+
+
+def _run_neighbors(dev: Device, emb: DenseMatrix, step: _Neighbors) -> Graph:
+    if step.search == "exact":
+        return dev.neighbors(emb, step.n_neighbors, step.n_components, step.metric, step.precision)
+    if step.n_components is not None and step.n_components != emb.cols:
+        raise KytoError(KY_E501, "neighbors: the approximate search uses every component")
+    n_lists = max(1, min(4096, emb.rows // 256))
+    index = dev.ivf_build(emb, n_lists=n_lists, seed=step.seed, label_k=step.n_neighbors - 1)
+    try:
+        return index.search(emb, step.n_neighbors)
+    finally:
+        index.free()
+
+
+def scan(layers, default_layer: str = "X") -> Atlas:
+    """A plan over one CSR layer (named `default_layer`) or a dict of them."""
+    if isinstance(layers, dict):
+        parsed = {k: _as_layer(v) for k, v in layers.items()}
+    else:
+        parsed = {default_layer: _as_layer(layers)}
+    return Atlas(parsed, default_layer)

@@ -1,0 +1,352 @@
+// The extern "C" boundary of libkyto_gpu.so (include/kyto_gpu.h): status codes,
+// thread-local messages, no exception crossing the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "common.hpp"
+#include "dense.hpp"
+#include "ivf.hpp"
+#include "knn.hpp"
+#include "pca.hpp"
+
+namespace ky {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &message) { g_last_error = message; }
+
+template <typename F>
+static int guarded(F &&f) {
+    try {
+        f();
+        g_last_error.clear();
+        return KY_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host memory exhausted";
+        return KY_E505;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return KY_E505;
+    } catch (...) {
+        g_last_error = "unknown failure";
+        return KY_E505;
+    }
+}
+
+static void need(const void *p, const char *what) {
+    if (!p) throw Error(KY_E215, std::string(what) + " is null");
+}
+
+static void use_device(const Context &ctx) { KY_HIP(hipSetDevice(ctx.device)); }
+
+}  // namespace ky
+
+using namespace ky;
+
+extern "C" {
+
+const char *ky_last_error(void) { return g_last_error.c_str(); }
+
+int ky_device_count(int *count) {
+    return guarded([&] {
+        need(count, "count");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        *count = e == hipSuccess ? n : 0;
+    });
+}
+
+int ky_open(int device, ky_ctx **out) {
+    return guarded([&] {
+        need(out, "out");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+            throw Error(KY_E504, "device " + std::to_string(device) + " was not found (" + std::to_string(n) +
+                                     " visible)");
+        ky_ctx *ctx = new ky_ctx();
+        try {
+            ctx->device = device;
+            KY_HIP(hipSetDevice(device));
+            KY_HIP(hipGetDeviceProperties(&ctx->props, device));
+            ctx->n_cu = ctx->props.multiProcessorCount;
+            KY_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        } catch (...) {
+            delete ctx;
+            throw;
+        }
+        *out = ctx;
+    });
+}
+
+int ky_close(ky_ctx *ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        hipStream_t st = ctx->stream;
+        delete ctx;
+        if (st) (void)hipStreamDestroy(st);
+    });
+}
+
+int ky_device_info(ky_ctx *ctx, char *name, size_t name_len, uint64_t *total_bytes) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        if (name && name_len) {
+            std::strncpy(name, ctx->props.gcnArchName, name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (total_bytes) *total_bytes = ctx->props.totalGlobalMem;
+    });
+}
+
+int ky_csr_upload(ky_ctx *ctx, const uint64_t *indptr, const uint32_t *indices, const float *values, uint64_t n_rows,
+                  uint32_t n_cols, ky_csr **out) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(indptr, "indptr");
+        need(out, "out");
+        use_device(*ctx);
+        if (indptr[n_rows] > 0) {
+            need(indices, "indices");
+            need(values, "values");
+        }
+        ky_csr *csr = new ky_csr();
+        try {
+            csr->ctx = ctx;
+            csr->n_rows = n_rows;
+            csr->n_cols = n_cols;
+            csr_build(*csr, indptr, indices, values);
+        } catch (...) {
+            delete csr;
+            throw;
+        }
+        *out = csr;
+    });
+}
+
+int ky_csr_free(ky_csr *csr) {
+    return guarded([&] {
+        if (!csr) return;
+        (void)hipSetDevice(csr->ctx->device);
+        delete csr;
+    });
+}
+
+int ky_csr_shape(const ky_csr *csr, uint64_t *n_rows, uint32_t *n_cols, uint64_t *nnz) {
+    return guarded([&] {
+        need(csr, "csr");
+        if (n_rows) *n_rows = csr->n_rows;
+        if (n_cols) *n_cols = csr->n_cols;
+        if (nnz) *nnz = csr->nnz;
+    });
+}
+
+int ky_dense_upload(ky_ctx *ctx, const float *host, uint64_t rows, uint32_t cols, ky_dense **out) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(host, "host");
+        need(out, "out");
+        use_device(*ctx);
+        ky_dense *m = new ky_dense();
+        try {
+            m->ctx = ctx;
+            m->rows = rows;
+            m->cols = cols;
+            m->data.alloc(rows * cols);
+            KY_HIP(hipMemcpyAsync(m->data.ptr, host, sizeof(float) * rows * cols, hipMemcpyHostToDevice, ctx->stream));
+            KY_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int ky_dense_download(const ky_dense *m, float *host) {
+    return guarded([&] {
+        need(m, "matrix");
+        need(host, "host");
+        use_device(*m->ctx);
+        KY_HIP(hipMemcpyAsync(host, m->data.ptr, sizeof(float) * m->rows * m->cols, hipMemcpyDeviceToHost,
+                              m->ctx->stream));
+        KY_HIP(hipStreamSynchronize(m->ctx->stream));
+    });
+}
+
+int ky_dense_shape(const ky_dense *m, uint64_t *rows, uint32_t *cols) {
+    return guarded([&] {
+        need(m, "matrix");
+        if (rows) *rows = m->rows;
+        if (cols) *cols = m->cols;
+    });
+}
+
+int ky_dense_free(ky_dense *m) {
+    return guarded([&] {
+        if (!m) return;
+        (void)hipSetDevice(m->ctx->device);
+        delete m;
+    });
+}
+
+int ky_pca_sketch(ky_ctx *ctx, const ky_csr *csr, uint64_t seed, uint64_t cell_offset, uint32_t l, double *sketch,
+                  double *gene_sums, double *omega_sums) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(csr, "csr");
+        need(sketch, "sketch");
+        need(gene_sums, "gene_sums");
+        need(omega_sums, "omega_sums");
+        use_device(*ctx);
+        pca_sketch(const_cast<ky_csr &>(*csr), seed, cell_offset, l, sketch, gene_sums, omega_sums);
+    });
+}
+
+int ky_pca_product(ky_ctx *ctx, const ky_csr *csr, const double *q, uint32_t w, const double *block_means,
+                   double *product, double *sums) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(csr, "csr");
+        need(q, "q");
+        need(block_means, "block_means");
+        need(product, "product");
+        need(sums, "sums");
+        use_device(*ctx);
+        pca_product(const_cast<ky_csr &>(*csr), q, w, block_means, product, sums);
+    });
+}
+
+int ky_pca_project(ky_ctx *ctx, const ky_csr *csr, const double *v, uint32_t k, const double *mean_projection,
+                   const double *means, float *host_embedding, ky_dense **embedding, double *squares,
+                   uint64_t *stored) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(csr, "csr");
+        need(v, "v");
+        need(mean_projection, "mean_projection");
+        need(means, "means");
+        need(squares, "squares");
+        need(stored, "stored");
+        use_device(*ctx);
+        Dense *emb = nullptr;
+        pca_project(const_cast<ky_csr &>(*csr), v, k, mean_projection, means, host_embedding,
+                    embedding ? &emb : nullptr, squares, stored);
+        if (embedding) *embedding = static_cast<ky_dense *>(emb);
+    });
+}
+
+int ky_pca(ky_ctx *ctx, const ky_csr *csr, uint32_t n_components, uint64_t seed, uint32_t power_iterations,
+           float *host_embedding, ky_dense **embedding, float *loadings, double *variance, double *variance_ratio,
+           double *means, uint32_t *passes_out, ky_pca_timing *timing) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(csr, "csr");
+        need(variance, "variance");
+        need(variance_ratio, "variance_ratio");
+        use_device(*ctx);
+        Dense *emb = nullptr;
+        pca_run(const_cast<ky_csr &>(*csr), n_components, seed, power_iterations, host_embedding,
+                embedding ? &emb : nullptr, loadings, variance, variance_ratio, means, passes_out, timing);
+        if (embedding) *embedding = static_cast<ky_dense *>(emb);
+    });
+}
+
+int ky_neighbors(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uint32_t n_neighbors, int metric,
+                 int precision, uint32_t *indices, float *distances, ky_knn_info *info) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(points, "points");
+        need(indices, "indices");
+        need(distances, "distances");
+        use_device(*ctx);
+        neighbors_run(*ctx, *points, n_components, n_neighbors, metric, precision, indices, distances, info);
+    });
+}
+
+int ky_nearest_keys(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uint64_t first, uint64_t count,
+                    uint32_t m, int precision, float *keys, uint32_t *rows) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(points, "points");
+        need(keys, "keys");
+        need(rows, "rows");
+        use_device(*ctx);
+        nearest_keys_run(*ctx, *points, n_components, first, count, m, precision, keys, rows);
+    });
+}
+
+int ky_ivf_build(ky_ctx *ctx, const ky_dense *points, const ky_ivf_params *params, ky_ivf **out, ky_ivf_info *info) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(points, "points");
+        need(params, "params");
+        need(out, "out");
+        use_device(*ctx);
+        *out = ivf_build(*ctx, *points, *params, info);
+    });
+}
+
+int ky_ivf_free(ky_ivf *ivf) {
+    return guarded([&] { ivf_free(ivf); });
+}
+
+int ky_ivf_search(ky_ctx *ctx, const ky_ivf *ivf, const ky_dense *points, uint32_t n_neighbors, uint32_t n_probe,
+                  int use_router, float confidence, uint32_t *indices, float *distances, ky_ivf_search_info *info) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(ivf, "ivf");
+        need(points, "points");
+        need(indices, "indices");
+        need(distances, "distances");
+        use_device(*ctx);
+        ivf_search(*ctx, *ivf, *points, n_neighbors, n_probe, use_router, confidence, indices, distances, info);
+    });
+}
+
+int ky_dense_extend_basis(const double *basis, uint32_t rows, uint32_t bc, const double *block, uint32_t bw,
+                          double *out, uint32_t *kept) {
+    return guarded([&] {
+        need(block, "block");
+        need(out, "out");
+        need(kept, "kept");
+        Mat b(rows, bc), blk(rows, bw);
+        if (bc) {
+            need(basis, "basis");
+            std::memcpy(b.v.data(), basis, sizeof(double) * rows * bc);
+        }
+        std::memcpy(blk.v.data(), block, sizeof(double) * rows * bw);
+        Mat r = extend_basis(b, blk);
+        *kept = (uint32_t)r.cols;
+        for (uint32_t i = 0; i < rows; i++)
+            for (uint32_t j = 0; j < r.cols; j++) out[(size_t)i * bw + j] = r.at(i, j);
+    });
+}
+
+int ky_dense_rayleigh_ritz(const double *basis, const double *products, uint32_t rows, uint32_t cols, uint32_t k,
+                           double *eigenvalues, double *loadings) {
+    return guarded([&] {
+        need(eigenvalues, "eigenvalues");
+        need(loadings, "loadings");
+        Mat b(rows, cols), p(rows, cols);
+        if (cols) {
+            need(basis, "basis");
+            need(products, "products");
+            std::memcpy(b.v.data(), basis, sizeof(double) * rows * cols);
+            std::memcpy(p.v.data(), products, sizeof(double) * rows * cols);
+        }
+        std::vector<double> ev;
+        Mat l;
+        rayleigh_ritz(b, p, k, ev, l);
+        for (uint32_t j = 0; j < k; j++) eigenvalues[j] = ev[j];
+        std::memcpy(loadings, l.v.data(), sizeof(double) * rows * k);
+    });
+}
+
+}  // extern "C"

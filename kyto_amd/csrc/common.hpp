@@ -1,0 +1,123 @@
+// Shared plumbing of libkyto_gpu.so: the context, device buffers, error reporting.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kyto_gpu.h"
+
+namespace ky {
+
+// A failure carrying a kyto error code; converted to a status at the C ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &message);
+
+#define KY_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::ky::Error(KY_E505, std::string("device failure in ") + #call + ": " + \
+                                           hipGetErrorString(e_));                        \
+    } while (0)
+
+#define KY_CHECK(cond, code, msg)                   \
+    do {                                            \
+        if (!(cond)) throw ::ky::Error(code, msg);  \
+    } while (0)
+
+// A device allocation owned by one object.
+template <typename T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), n(o.n) { o.ptr = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; n = o.n; o.ptr = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) KY_HIP(hipMalloc(reinterpret_cast<void **>(&ptr), sizeof(T) * count));
+    }
+    // Grow only (keeps the allocation when it is already large enough).
+    void reserve(size_t count) {
+        if (count > n) alloc(count);
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return sizeof(T) * n; }
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t props{};
+    int n_cu = 0;
+    // Scratch reused between calls (grown on demand).
+    DevBuf<double> partials;
+    DevBuf<double> reduced;
+    DevBuf<float> dense_f32;
+    DevBuf<double> dense_f64;
+    DevBuf<float> wbuf;
+    DevBuf<double> sums;
+};
+
+struct Csr {
+    Context *ctx = nullptr;
+    uint64_t n_rows = 0;
+    uint32_t n_cols = 0;
+    uint64_t nnz = 0;
+    DevBuf<uint64_t> indptr;   // n_rows + 1
+    DevBuf<uint32_t> indices;  // nnz
+    DevBuf<float> values;      // nnz
+    // Chunked column-major copy: for chunk c (rows [c*CH, (c+1)*CH)), the chunk's
+    // entries (which occupy [indptr[c*CH], indptr[min((c+1)*CH, n)]) in the rows)
+    // reordered by (gene, row). col_ptr[c*(n_cols+1) + g] is the global offset of
+    // gene g's run in chunk c.
+    uint32_t chunk_rows = 0;
+    uint32_t n_chunks = 0;
+    DevBuf<uint32_t> col_ptr;
+    DevBuf<uint16_t> col_row;  // row within the chunk
+    DevBuf<float> col_val;
+};
+
+struct Dense {
+    Context *ctx = nullptr;
+    uint64_t rows = 0;
+    uint32_t cols = 0;
+    DevBuf<float> data;
+};
+
+// Host wall clock in milliseconds.
+inline double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+inline unsigned div_up(uint64_t a, uint64_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+}  // namespace ky
+
+struct ky_ctx : ky::Context {};
+struct ky_csr : ky::Csr {};
+struct ky_dense : ky::Dense {};

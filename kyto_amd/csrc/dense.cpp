@@ -1,0 +1,477 @@
+// The FP64 dense steps of the streamed PCA on the host: kyto's extend_basis and
+// rayleigh_ritz (crates/kyto-exec/src/pca_dense.rs:105-273), restated over this
+// library's own FP64 routines (kyto uses faer 0.24.4 for these; its results are
+// pinned by tolerances, pca_dense.rs:285-355):
+// - thin SVD: Householder QR, then one-sided Jacobi (Hestenes) on the small R;
+// - symmetric eigenpairs: Householder tridiagonalisation and the implicit QL method
+//   with Wilkinson-style shifts (the EISPACK tred2/tql2 pair).
+// Products are cut over a fixed number of row groups run on std::threads; each
+// output element is summed in one fixed order, so the bits do not depend on the
+// thread count.
+#include "dense.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <thread>
+
+namespace ky {
+
+static unsigned worker_count() {
+    unsigned n = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(n, 32u));
+}
+
+template <typename F>
+static void parallel_for(size_t n, F &&f) {
+    unsigned t = worker_count();
+    if (n < 64 || t == 1) {
+        f(size_t(0), n);
+        return;
+    }
+    t = static_cast<unsigned>(std::min<size_t>(t, n / 16 + 1));
+    std::vector<std::thread> threads;
+    size_t per = (n + t - 1) / t;
+    for (unsigned i = 0; i < t; i++) {
+        size_t a = i * per, b = std::min(n, a + per);
+        if (a >= b) break;
+        threads.emplace_back([&, a, b] { f(a, b); });
+    }
+    for (auto &th : threads) th.join();
+}
+
+Mat Mat::beside(const Mat &right) const {
+    Mat out(rows, cols + right.cols);
+    for (size_t r = 0; r < rows; r++) {
+        std::copy(row(r), row(r) + cols, out.row(r));
+        std::copy(right.row(r), right.row(r) + right.cols, out.row(r) + cols);
+    }
+    return out;
+}
+
+// out (p x q) = A^T B, A m x p, B m x q.
+Mat mul_tn(const Mat &a, const Mat &b) {
+    Mat out(a.cols, b.cols);
+    parallel_for(a.cols, [&](size_t i0, size_t i1) {
+        for (size_t r = 0; r < a.rows; r++) {
+            const double *ar = a.row(r);
+            const double *br = b.row(r);
+            for (size_t i = i0; i < i1; i++) {
+                double f = ar[i];
+                double *o = out.row(i);
+                for (size_t j = 0; j < b.cols; j++) o[j] += f * br[j];
+            }
+        }
+    });
+    return out;
+}
+
+// out (m x q) = A B, A m x p, B p x q.
+Mat mul_nn(const Mat &a, const Mat &b) {
+    Mat out(a.rows, b.cols);
+    parallel_for(a.rows, [&](size_t r0, size_t r1) {
+        for (size_t r = r0; r < r1; r++) {
+            const double *ar = a.row(r);
+            double *o = out.row(r);
+            for (size_t k = 0; k < a.cols; k++) {
+                double f = ar[k];
+                const double *br = b.row(k);
+                for (size_t j = 0; j < b.cols; j++) o[j] += f * br[j];
+            }
+        }
+    });
+    return out;
+}
+
+static double frobenius(const Mat &m) {
+    // Scaled sum of squares: no overflow for large entries.
+    double scale = 0.0, ssq = 1.0;
+    for (double v : m.v) {
+        if (v != 0.0) {
+            double a = std::fabs(v);
+            if (scale < a) {
+                ssq = 1.0 + ssq * (scale / a) * (scale / a);
+                scale = a;
+            } else {
+                ssq += (a / scale) * (a / scale);
+            }
+        }
+    }
+    return scale * std::sqrt(ssq);
+}
+
+// Thin Householder QR of a (m x n, m >= n): returns Q (m x n, orthonormal columns)
+// and R (n x n, upper triangular) with a = Q R.
+static void householder_qr(const Mat &a, Mat &q, Mat &r) {
+    size_t m = a.rows, n = a.cols;
+    Mat w = a;                      // reflected in place
+    std::vector<std::vector<double>> vs(n);
+    std::vector<double> betas(n, 0.0);
+    r = Mat(n, n);
+    for (size_t j = 0; j < n; j++) {
+        double norm2 = 0.0;
+        for (size_t i = j; i < m; i++) norm2 += w.at(i, j) * w.at(i, j);
+        double norm = std::sqrt(norm2);
+        std::vector<double> v(m - j);
+        for (size_t i = j; i < m; i++) v[i - j] = w.at(i, j);
+        if (norm == 0.0) {
+            vs[j] = std::move(v);
+            betas[j] = 0.0;
+            continue;
+        }
+        double alpha = v[0] > 0 ? -norm : norm;
+        v[0] -= alpha;
+        double vtv = 0.0;
+        for (double x : v) vtv += x * x;
+        double beta = vtv > 0 ? 2.0 / vtv : 0.0;
+        // Apply (I - beta v v^T) to columns j..n-1 of w (rows j..m-1).
+        for (size_t c = j; c < n; c++) {
+            double s = 0.0;
+            for (size_t i = j; i < m; i++) s += v[i - j] * w.at(i, c);
+            s *= beta;
+            for (size_t i = j; i < m; i++) w.at(i, c) -= s * v[i - j];
+        }
+        vs[j] = std::move(v);
+        betas[j] = beta;
+    }
+    for (size_t i = 0; i < n; i++)
+        for (size_t c = i; c < n; c++) r.at(i, c) = w.at(i, c);
+    // Q = H_0 H_1 ... H_{n-1} [I; 0], applied from the last reflection back.
+    q = Mat(m, n);
+    for (size_t i = 0; i < n; i++) q.at(i, i) = 1.0;
+    for (size_t jj = n; jj-- > 0;) {
+        const std::vector<double> &v = vs[jj];
+        double beta = betas[jj];
+        if (beta == 0.0) continue;
+        for (size_t c = 0; c < n; c++) {
+            double s = 0.0;
+            for (size_t i = jj; i < m; i++) s += v[i - jj] * q.at(i, c);
+            s *= beta;
+            for (size_t i = jj; i < m; i++) q.at(i, c) -= s * v[i - jj];
+        }
+    }
+}
+
+// One-sided Jacobi SVD of a square a (n x n): a = U diag(s) V^T with s descending.
+// Columns of U whose singular value is 0 are left as zeros.
+static void jacobi_svd(const Mat &a, Mat &u, std::vector<double> &s) {
+    size_t n = a.rows;
+    // Work on columns: store the transpose so each column is a contiguous row.
+    std::vector<std::vector<double>> col(n, std::vector<double>(n));
+    for (size_t i = 0; i < n; i++)
+        for (size_t j = 0; j < n; j++) col[j][i] = a.at(i, j);
+    const double eps = 2.220446049250313e-16;
+    for (int sweep = 0; sweep < 80; sweep++) {
+        bool rotated = false;
+        for (size_t p = 0; p + 1 < n; p++) {
+            for (size_t qq = p + 1; qq < n; qq++) {
+                double alpha = 0, beta = 0, gamma = 0;
+                const std::vector<double> &cp = col[p], &cq = col[qq];
+                for (size_t i = 0; i < n; i++) {
+                    alpha += cp[i] * cp[i];
+                    beta += cq[i] * cq[i];
+                    gamma += cp[i] * cq[i];
+                }
+                if (gamma == 0.0 || std::fabs(gamma) <= eps * std::sqrt(alpha * beta)) continue;
+                rotated = true;
+                double zeta = (beta - alpha) / (2.0 * gamma);
+                double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                double c = 1.0 / std::sqrt(1.0 + t * t), sn = c * t;
+                std::vector<double> &xp = col[p], &xq = col[qq];
+                for (size_t i = 0; i < n; i++) {
+                    double x = xp[i], y = xq[i];
+                    xp[i] = c * x - sn * y;
+                    xq[i] = sn * x + c * y;
+                }
+            }
+        }
+        if (!rotated) break;
+    }
+    std::vector<double> norms(n);
+    for (size_t j = 0; j < n; j++) {
+        double ss = 0;
+        for (double x : col[j]) ss += x * x;
+        norms[j] = std::sqrt(ss);
+    }
+    std::vector<size_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return norms[x] > norms[y]; });
+    u = Mat(n, n);
+    s.assign(n, 0.0);
+    for (size_t k = 0; k < n; k++) {
+        size_t j = order[k];
+        s[k] = norms[j];
+        if (norms[j] > 0)
+            for (size_t i = 0; i < n; i++) u.at(i, k) = col[j][i] / norms[j];
+    }
+}
+
+Mat extend_basis(const Mat &basis, const Mat &block) {
+    size_t rows = block.rows;
+    double scale = frobenius(block);
+    if (scale == 0.0 || block.cols == 0) return Mat(rows, 0);
+    if (!std::isfinite(scale))
+        throw Error(KY_E215, "pca: orthonormalising a Krylov block did not converge (a value that is not finite)");
+    Mat rest = block;
+    if (basis.cols > 0) {
+        for (int pass = 0; pass < 2; pass++) {
+            Mat proj = mul_tn(basis, rest);   // bc x bw
+            Mat back = mul_nn(basis, proj);   // rows x bw
+            for (size_t i = 0; i < rest.v.size(); i++) rest.v[i] -= back.v[i];
+        }
+    }
+    size_t n = rest.cols;
+    Mat q, r, ur;
+    std::vector<double> s;
+    KY_CHECK(rows >= n, KY_E505, "pca: a Krylov block wider than the genes (internal)");
+    householder_qr(rest, q, r);
+    jacobi_svd(r, ur, s);
+    size_t keep = 0;
+    while (keep < s.size() && s[keep] > DROP_TOLERANCE * scale) keep++;
+    keep = std::min(keep, rows);
+    Mat ukeep(n, keep);
+    for (size_t i = 0; i < n; i++)
+        for (size_t j = 0; j < keep; j++) ukeep.at(i, j) = ur.at(i, j);
+    return mul_nn(q, ukeep);
+}
+
+// Symmetric eigendecomposition: a (n x n) = V diag(d) V^T, d ascending.
+static void symmetric_eigen(const Mat &a, std::vector<double> &d, Mat &vout) {
+    size_t n = a.rows;
+    std::vector<std::vector<double>> V(n, std::vector<double>(n));
+    for (size_t i = 0; i < n; i++)
+        for (size_t j = 0; j < n; j++) V[i][j] = a.at(i, j);
+    d.assign(n, 0.0);
+    std::vector<double> e(n, 0.0);
+    if (n == 0) { vout = Mat(0, 0); return; }
+    // Householder reduction to tridiagonal form.
+    for (size_t j = 0; j < n; j++) d[j] = V[n - 1][j];
+    for (size_t i = n - 1; i > 0; i--) {
+        double scale = 0.0, h = 0.0;
+        for (size_t k = 0; k < i; k++) scale += std::fabs(d[k]);
+        if (scale == 0.0) {
+            e[i] = d[i - 1];
+            for (size_t j = 0; j < i; j++) {
+                d[j] = V[i - 1][j];
+                V[i][j] = 0.0;
+                V[j][i] = 0.0;
+            }
+        } else {
+            for (size_t k = 0; k < i; k++) {
+                d[k] /= scale;
+                h += d[k] * d[k];
+            }
+            double f = d[i - 1];
+            double g = std::sqrt(h);
+            if (f > 0) g = -g;
+            e[i] = scale * g;
+            h = h - f * g;
+            d[i - 1] = f - g;
+            for (size_t j = 0; j < i; j++) e[j] = 0.0;
+            for (size_t j = 0; j < i; j++) {
+                f = d[j];
+                V[j][i] = f;
+                g = e[j] + V[j][j] * f;
+                for (size_t k = j + 1; k <= i - 1; k++) {
+                    g += V[k][j] * d[k];
+                    e[k] += V[k][j] * f;
+                }
+                e[j] = g;
+            }
+            f = 0.0;
+            for (size_t j = 0; j < i; j++) {
+                e[j] /= h;
+                f += e[j] * d[j];
+            }
+            double hh = f / (h + h);
+            for (size_t j = 0; j < i; j++) e[j] -= hh * d[j];
+            for (size_t j = 0; j < i; j++) {
+                f = d[j];
+                g = e[j];
+                for (size_t k = j; k <= i - 1; k++) V[k][j] -= (f * e[k] + g * d[k]);
+                d[j] = V[i - 1][j];
+                V[i][j] = 0.0;
+            }
+        }
+        d[i] = h;
+    }
+    // Accumulate the transformations.
+    for (size_t i = 0; i + 1 < n; i++) {
+        V[n - 1][i] = V[i][i];
+        V[i][i] = 1.0;
+        double h = d[i + 1];
+        if (h != 0.0) {
+            for (size_t k = 0; k <= i; k++) d[k] = V[k][i + 1] / h;
+            for (size_t j = 0; j <= i; j++) {
+                double g = 0.0;
+                for (size_t k = 0; k <= i; k++) g += V[k][i + 1] * V[k][j];
+                for (size_t k = 0; k <= i; k++) V[k][j] -= g * d[k];
+            }
+        }
+        for (size_t k = 0; k <= i; k++) V[k][i + 1] = 0.0;
+    }
+    for (size_t j = 0; j < n; j++) {
+        d[j] = V[n - 1][j];
+        V[n - 1][j] = 0.0;
+    }
+    V[n - 1][n - 1] = 1.0;
+    e[0] = 0.0;
+    // Implicit QL on the tridiagonal matrix.
+    for (size_t i = 1; i < n; i++) e[i - 1] = e[i];
+    e[n - 1] = 0.0;
+    double f = 0.0, tst1 = 0.0;
+    const double eps = 2.220446049250313e-16;
+    for (size_t l = 0; l < n; l++) {
+        tst1 = std::max(tst1, std::fabs(d[l]) + std::fabs(e[l]));
+        size_t m = l;
+        while (m < n) {
+            if (std::fabs(e[m]) <= eps * tst1) break;
+            m++;
+        }
+        if (m > l) {
+            int iter = 0;
+            do {
+                iter++;
+                if (iter > 200)
+                    throw Error(KY_E215, "pca: the Rayleigh-Ritz eigendecomposition did not converge");
+                double g = d[l];
+                double p = (d[l + 1] - g) / (2.0 * e[l]);
+                double r = std::hypot(p, 1.0);
+                if (p < 0) r = -r;
+                d[l] = e[l] / (p + r);
+                d[l + 1] = e[l] * (p + r);
+                double dl1 = d[l + 1];
+                double h = g - d[l];
+                for (size_t i = l + 2; i < n; i++) d[i] -= h;
+                f += h;
+                p = d[m];
+                double c = 1.0, c2 = c, c3 = c;
+                double el1 = e[l + 1];
+                double s = 0.0, s2 = 0.0;
+                for (size_t ii = m; ii-- > l;) {
+                    c3 = c2;
+                    c2 = c;
+                    s2 = s;
+                    g = c * e[ii];
+                    h = c * p;
+                    r = std::hypot(p, e[ii]);
+                    e[ii + 1] = s * r;
+                    s = e[ii] / r;
+                    c = p / r;
+                    p = c * d[ii] - s * g;
+                    d[ii + 1] = h + s * (c * g + s * d[ii]);
+                    for (size_t k = 0; k < n; k++) {
+                        h = V[k][ii + 1];
+                        V[k][ii + 1] = s * V[k][ii] + c * h;
+                        V[k][ii] = c * V[k][ii] - s * h;
+                    }
+                }
+                p = -s * s2 * c3 * el1 * e[l] / dl1;
+                e[l] = s * p;
+                d[l] = c * p;
+            } while (std::fabs(e[l]) > eps * tst1);
+        }
+        d[l] = d[l] + f;
+        e[l] = 0.0;
+    }
+    std::vector<size_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return d[x] < d[y]; });
+    std::vector<double> ds(n);
+    vout = Mat(n, n);
+    for (size_t k = 0; k < n; k++) {
+        ds[k] = d[order[k]];
+        for (size_t i = 0; i < n; i++) vout.at(i, k) = V[i][order[k]];
+    }
+    d = ds;
+}
+
+// pca_dense.rs:215-252.
+static Mat complement(const Mat &columns, size_t count) {
+    Mat all = columns;
+    Mat added(columns.rows, 0);
+    for (size_t c = 0; c < count; c++) {
+        size_t gene = 0;
+        double least = INFINITY;
+        for (size_t r = 0; r < all.rows; r++) {
+            double cov = 0.0;
+            for (size_t j = 0; j < all.cols; j++) cov += all.at(r, j) * all.at(r, j);
+            if (cov < least) {
+                least = cov;
+                gene = r;
+            }
+        }
+        std::vector<double> v(all.rows, 0.0);
+        v[gene] = 1.0;
+        for (int pass = 0; pass < 2; pass++) {
+            std::vector<double> coef(all.cols, 0.0);
+            for (size_t j = 0; j < all.cols; j++)
+                for (size_t r = 0; r < all.rows; r++) coef[j] += all.at(r, j) * v[r];
+            for (size_t r = 0; r < all.rows; r++) {
+                double s = 0.0;
+                for (size_t j = 0; j < all.cols; j++) s += all.at(r, j) * coef[j];
+                v[r] -= s;
+            }
+        }
+        double norm = 0.0;
+        for (double x : v) norm += x * x;
+        norm = std::sqrt(norm);
+        Mat col(all.rows, 1);
+        for (size_t r = 0; r < all.rows; r++) col.at(r, 0) = v[r] / norm;
+        all = all.beside(col);
+        added = added.beside(col);
+    }
+    return added;
+}
+
+// pca_dense.rs:256-273: each column's largest absolute value (the first on ties)
+// made positive.
+static void flip_signs(Mat &m) {
+    for (size_t c = 0; c < m.cols; c++) {
+        double largest = 0.0, sign = 1.0;
+        for (size_t r = 0; r < m.rows; r++) {
+            double v = m.at(r, c);
+            if (std::fabs(v) > largest) {
+                largest = std::fabs(v);
+                sign = v < 0 ? -1.0 : 1.0;
+            }
+        }
+        if (sign < 0)
+            for (size_t r = 0; r < m.rows; r++) m.at(r, c) = -m.at(r, c);
+    }
+}
+
+void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<double> &eigenvalues,
+                   Mat &loadings) {
+    loadings = Mat(basis.rows, 0);
+    eigenvalues.clear();
+    if (basis.cols > 0) {
+        Mat t = mul_tn(basis, products);
+        size_t p = t.rows;
+        Mat sym(p, p);
+        for (size_t i = 0; i < p; i++)
+            for (size_t j = 0; j < p; j++) sym.at(i, j) = 0.5 * (t.at(i, j) + t.at(j, i));
+        for (double v : sym.v)
+            if (!std::isfinite(v))
+                throw Error(KY_E215, "pca: the Rayleigh-Ritz eigendecomposition did not converge (a value that is not finite)");
+        std::vector<double> d;
+        Mat vecs;
+        symmetric_eigen(sym, d, vecs);
+        size_t taken = std::min(k, p);
+        Mat chosen(p, taken);
+        for (size_t c = 0; c < taken; c++) {
+            size_t idx = p - 1 - c;
+            eigenvalues.push_back(std::max(d[idx], 0.0));
+            for (size_t i = 0; i < p; i++) chosen.at(i, c) = vecs.at(i, idx);
+        }
+        loadings = mul_nn(basis, chosen);
+    }
+    size_t missing = k - eigenvalues.size();
+    if (missing > 0) {
+        loadings = loadings.beside(complement(loadings, missing));
+        eigenvalues.resize(k, 0.0);
+    }
+    flip_signs(loadings);
+}
+
+}  // namespace ky

@@ -1,0 +1,660 @@
+// pca() on the device: the resident CSR layer, its chunked column-major copy, and the
+// three kinds of pass kyto's GPU fold runs (crates/kyto-exec/src/pca/gpu.rs:149-219),
+// with the host FP64 steps of PcaAccumulator (pca.rs:302-862) around them.
+//
+// Kernels (gfx950, wave64):
+// - build_columns: per chunk of CHUNK_ROWS rows, the chunk's entries reordered by
+//   (gene, row) in place of kyto's host CSC (launch_matrix.rs:405-424). Deterministic:
+//   per-wave integer counts in LDS, a scan, then each wave places its rows in order.
+// - omega_rows: kyto's Omega rows (pca.rs:936-950) drawn per (cell, column) from the
+//   integer SplitMix64 stream, rounded to f32, plus a column of ones (gene sums).
+// - row_project: one wavefront per row, lane j = column j: w_j = -offset_j +
+//   sum_e value_e * M[gene_e, j], exact f32 x f32 products accumulated in f64, the
+//   row's entries broadcast from lanes with readlane, M's rows read from L2.
+//   Serves X Q - mu^T Q (product passes) and X V - mu^T V (the embedding).
+// - column_product: X^T W per chunk, one wavefront per (chunk, gene), lane = column,
+//   over the gene's run of the chunk (rows ascending) with W's rows of the chunk read
+//   from L2; blocks of one chunk are mapped onto one XCD (blockIdx % 8) so they share
+//   its W rows. f64 partials per chunk, then reduce_chunks adds the chunks in order.
+// No float atomics anywhere: the same bits on every run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "dense.hpp"
+#include "device.hpp"
+#include "pca.hpp"
+
+namespace ky {
+
+// The stream's SplitMix64 output function (pca.rs:946-950).
+__device__ __forceinline__ uint64_t mix64(uint64_t v) {
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
+    return v ^ (v >> 31);
+}
+
+template <typename T>
+__device__ __forceinline__ T bcast(T v, int lane) {
+    if constexpr (sizeof(T) == 4) {
+        int x = __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane);
+        return __builtin_bit_cast(T, x);
+    } else {
+        static_assert(sizeof(T) == 2, "4- or 2-byte values");
+        int x = __builtin_amdgcn_readlane(static_cast<int>(__builtin_bit_cast(uint16_t, v)), lane);
+        return __builtin_bit_cast(T, static_cast<uint16_t>(x));
+    }
+}
+
+// ----------------------------------------------------------------- validation --
+
+// One thread per row: indices strictly increasing below n_cols, values finite and 0 or
+// within GPU_RANGE in magnitude (pca/gpu.rs:120). flags: bit 0 index, bit 1 value.
+__global__ void check_rows(const uint64_t *__restrict__ indptr, const uint32_t *__restrict__ indices,
+                           const float *__restrict__ values, uint64_t n_rows, uint32_t n_cols,
+                           unsigned *__restrict__ flags) {
+    uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    uint64_t s = indptr[r], e = indptr[r + 1];
+    unsigned bad = 0;
+    uint32_t prev = 0;
+    for (uint64_t i = s; i < e; i++) {
+        uint32_t g = indices[i];
+        if (g >= n_cols || (i > s && g <= prev)) bad |= 1u;
+        prev = g;
+        float v = values[i];
+        float a = fabsf(v);
+        if (!(a == 0.0f || (a >= 1e-10f && a <= 1e10f))) bad |= 2u;
+    }
+    if (bad) atomicOr(flags, bad);
+}
+
+// --------------------------------------------------------- chunked columns ----
+
+// One block per chunk; `waves` waves count and place rows. LDS: waves * n_cols
+// counters then blockDim partial sums.
+__global__ void build_columns(const uint64_t *__restrict__ indptr, const uint32_t *__restrict__ indices,
+                              const float *__restrict__ values, uint64_t n_rows, uint32_t n_cols,
+                              uint32_t chunk_rows, uint32_t *__restrict__ col_ptr,
+                              uint16_t *__restrict__ col_row, float *__restrict__ col_val) {
+    extern __shared__ uint32_t lds[];
+    const unsigned waves = blockDim.x / 64;
+    uint32_t *cnt = lds;                         // [waves][n_cols]
+    uint32_t *part = lds + (size_t)waves * n_cols;  // [blockDim]
+    const unsigned c = blockIdx.x;
+    const uint64_t r0 = (uint64_t)c * chunk_rows;
+    const uint64_t r1 = min(r0 + chunk_rows, n_rows);
+    const unsigned wave = threadIdx.x / 64, lane = lane_id();
+    for (size_t i = threadIdx.x; i < (size_t)waves * n_cols; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+    const uint64_t span = (r1 - r0 + waves - 1) / waves;
+    const uint64_t wr0 = min(r0 + wave * span, r1), wr1 = min(wr0 + span, r1);
+    uint32_t *mine = cnt + (size_t)wave * n_cols;
+    // Genes are distinct within a row, so no two lanes of one step touch one counter.
+    for (uint64_t r = wr0; r < wr1; r++)
+        for (uint64_t e = indptr[r] + lane; e < indptr[r + 1]; e += 64) mine[indices[e]] += 1;
+    __syncthreads();
+    // Exclusive scan over (gene, wave) in gene-major order.
+    const unsigned per = (n_cols + blockDim.x - 1) / blockDim.x;
+    const unsigned g0 = min(threadIdx.x * per, n_cols), g1 = min(g0 + per, n_cols);
+    uint32_t local = 0;
+    for (unsigned g = g0; g < g1; g++)
+        for (unsigned w = 0; w < waves; w++) local += cnt[(size_t)w * n_cols + g];
+    part[threadIdx.x] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (unsigned t = 0; t < blockDim.x; t++) {
+            uint32_t x = part[t];
+            part[t] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    const uint32_t base = (uint32_t)indptr[r0];
+    uint32_t run = base + part[threadIdx.x];
+    uint32_t *cp = col_ptr + (size_t)c * (n_cols + 1);
+    for (unsigned g = g0; g < g1; g++) {
+        cp[g] = run;
+        for (unsigned w = 0; w < waves; w++) {
+            uint32_t x = cnt[(size_t)w * n_cols + g];
+            cnt[(size_t)w * n_cols + g] = run;
+            run += x;
+        }
+    }
+    if (threadIdx.x == 0) cp[n_cols] = (uint32_t)indptr[r1];
+    __syncthreads();
+    for (uint64_t r = wr0; r < wr1; r++) {
+        for (uint64_t e = indptr[r] + lane; e < indptr[r + 1]; e += 64) {
+            uint32_t g = indices[e];
+            uint32_t pos = mine[g];
+            mine[g] = pos + 1;
+            col_row[pos] = (uint16_t)(r - r0);
+            col_val[pos] = values[e];
+        }
+    }
+}
+
+// ------------------------------------------------------------------- Omega ----
+
+// W[r, j] = f32(Omega[cell_offset + r, j]) for j < l, and W[r, l] = 1.
+__global__ void omega_rows(uint64_t n_rows, uint64_t seed, uint64_t cell_offset, uint32_t l,
+                           float *__restrict__ W, uint32_t ldw) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    uint64_t total = n_rows * (uint64_t)(l + 1);
+    if (i >= total) return;
+    uint64_t r = i / (l + 1);
+    uint32_t j = (uint32_t)(i % (l + 1));
+    float out = 1.0f;
+    if (j < l) {
+        const uint64_t phi = 0x9E3779B97F4A7C15ULL;
+        uint64_t s0 = mix64(seed ^ mix64(cell_offset + r + phi));
+        uint64_t state = s0 + (uint64_t)(j + 1) * phi;
+        double v = (double)(mix64(state) >> 11) * 0x1p-52 - 1.0;
+        out = (float)v;
+    }
+    W[r * ldw + j] = out;
+}
+
+// --------------------------------------------------------------- row project ----
+
+// One wavefront per row; lane j holds columns j, j+64, ... (CPL of them, w <= 64*CPL).
+// out[r, j] = f32(-offsets[j] + sum_e f64(value_e) * f64(M[gene_e, j])).
+template <int CPL>
+__global__ __launch_bounds__(256) void row_project(const uint64_t *__restrict__ indptr,
+                                                   const uint32_t *__restrict__ indices,
+                                                   const float *__restrict__ values, uint64_t n_rows,
+                                                   const float *__restrict__ M, uint32_t w,
+                                                   const double *__restrict__ offsets,
+                                                   float *__restrict__ out, uint32_t ldo) {
+    const uint64_t r = blockIdx.x * (uint64_t)(blockDim.x / 64) + threadIdx.x / 64;
+    if (r >= n_rows) return;
+    const int lane = lane_id();
+    double acc[CPL];
+#pragma unroll
+    for (int i = 0; i < CPL; i++) {
+        unsigned j = lane + 64 * i;
+        acc[i] = j < w ? -offsets[j] : 0.0;
+    }
+    const uint64_t s = indptr[r], e = indptr[r + 1];
+    for (uint64_t e0 = s; e0 < e; e0 += 64) {
+        const unsigned cnt = (unsigned)min<uint64_t>(64, e - e0);
+        uint32_t my_g = 0;
+        float my_v = 0.0f;
+        if ((unsigned)lane < cnt) {
+            my_g = indices[e0 + lane];
+            my_v = values[e0 + lane];
+        }
+        unsigned t = 0;
+        for (; t + 4 <= cnt; t += 4) {
+            uint32_t g[4];
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                g[u] = bcast(my_g, t + u);
+                v[u] = bcast(my_v, t + u);
+            }
+            float m[4][CPL];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int i = 0; i < CPL; i++) {
+                    unsigned j = lane + 64 * i;
+                    m[u][i] = j < w ? M[(size_t)g[u] * w + j] : 0.0f;
+                }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int i = 0; i < CPL; i++) acc[i] += (double)v[u] * (double)m[u][i];
+        }
+        for (; t < cnt; t++) {
+            uint32_t g = bcast(my_g, t);
+            float v = bcast(my_v, t);
+#pragma unroll
+            for (int i = 0; i < CPL; i++) {
+                unsigned j = lane + 64 * i;
+                float m = j < w ? M[(size_t)g * w + j] : 0.0f;
+                acc[i] += (double)v * (double)m;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < CPL; i++) {
+        unsigned j = lane + 64 * i;
+        if (j < w) out[r * ldo + j] = (float)acc[i];
+    }
+}
+
+// ------------------------------------------------------------ column product ----
+
+// partial[(chunk * n_cols + g) * w + j] = sum over gene g's entries of the chunk (rows
+// ascending) of f64(value) * f64(W[row, j]). One wavefront per gene, 4 per block,
+// `genes_per_block` genes per block; blocks of one chunk land on one XCD.
+template <int CPL>
+__global__ __launch_bounds__(256) void column_product(const uint32_t *__restrict__ col_ptr,
+                                                      const uint16_t *__restrict__ col_row,
+                                                      const float *__restrict__ col_val, uint32_t n_cols,
+                                                      uint32_t n_chunks, uint32_t chunk_rows,
+                                                      const float *__restrict__ W, uint32_t w, uint32_t ldw,
+                                                      uint32_t genes_per_block, double *__restrict__ partial) {
+    const unsigned groups = (n_cols + genes_per_block - 1) / genes_per_block;
+    const unsigned xcd = blockIdx.x % 8, local = blockIdx.x / 8;
+    const unsigned slot = local / groups, group = local % groups;
+    const unsigned chunk = slot * 8 + xcd;
+    if (chunk >= n_chunks) return;
+    const int lane = lane_id();
+    const unsigned wave = threadIdx.x / 64, waves = blockDim.x / 64;
+    const unsigned gend = min((group + 1) * genes_per_block, n_cols);
+    const uint32_t *cp = col_ptr + (size_t)chunk * (n_cols + 1);
+    const float *Wc = W + (size_t)chunk * chunk_rows * ldw;
+    for (unsigned g = group * genes_per_block + wave; g < gend; g += waves) {
+        double acc[CPL];
+#pragma unroll
+        for (int i = 0; i < CPL; i++) acc[i] = 0.0;
+        const uint32_t s = cp[g], e = cp[g + 1];
+        for (uint32_t e0 = s; e0 < e; e0 += 64) {
+            const unsigned cnt = min<uint32_t>(64, e - e0);
+            uint16_t my_r = 0;
+            float my_v = 0.0f;
+            if ((unsigned)lane < cnt) {
+                my_r = col_row[e0 + lane];
+                my_v = col_val[e0 + lane];
+            }
+            unsigned t = 0;
+            for (; t + 4 <= cnt; t += 4) {
+                uint32_t rr[4];
+                float v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    rr[u] = bcast(my_r, t + u);
+                    v[u] = bcast(my_v, t + u);
+                }
+                float m[4][CPL];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int i = 0; i < CPL; i++) {
+                        unsigned j = lane + 64 * i;
+                        m[u][i] = j < w ? Wc[(size_t)rr[u] * ldw + j] : 0.0f;
+                    }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int i = 0; i < CPL; i++) acc[i] += (double)v[u] * (double)m[u][i];
+            }
+            for (; t < cnt; t++) {
+                uint32_t rr = bcast(my_r, t);
+                float v = bcast(my_v, t);
+#pragma unroll
+                for (int i = 0; i < CPL; i++) {
+                    unsigned j = lane + 64 * i;
+                    float m = j < w ? Wc[(size_t)rr * ldw + j] : 0.0f;
+                    acc[i] += (double)v * (double)m;
+                }
+            }
+        }
+        double *out = partial + ((size_t)chunk * n_cols + g) * w;
+#pragma unroll
+        for (int i = 0; i < CPL; i++) {
+            unsigned j = lane + 64 * i;
+            if (j < w) out[j] = acc[i];
+        }
+    }
+}
+
+// out[i] = sum over chunks c (in order) of partial[c * stride + i].
+__global__ void reduce_chunks(const double *__restrict__ partial, uint64_t stride, uint32_t n_chunks,
+                              double *__restrict__ out) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= stride) return;
+    double s = 0.0;
+    for (uint32_t c = 0; c < n_chunks; c++) s += partial[(size_t)c * stride + i];
+    out[i] = s;
+}
+
+// chunk_sums[c * w + j] = sum over the chunk's rows (in order) of f64(W[r, j]).
+__global__ void column_sums(const float *__restrict__ W, uint64_t n_rows, uint32_t w, uint32_t ldw,
+                            uint32_t chunk_rows, double *__restrict__ chunk_sums) {
+    const unsigned c = blockIdx.x;
+    const unsigned j = threadIdx.x;
+    if (j >= w) return;
+    const uint64_t r0 = (uint64_t)c * chunk_rows, r1 = min(r0 + chunk_rows, n_rows);
+    double s = 0.0;
+    for (uint64_t r = r0; r < r1; r++) s += (double)W[r * ldw + j];
+    chunk_sums[(size_t)c * w + j] = s;
+}
+
+// Per (chunk, gene): the centred squares sum_e (f64(v_e) - mu_g)^2 over the chunk's
+// stored values of the gene, and their count.
+__global__ void gene_squares(const uint32_t *__restrict__ col_ptr, const float *__restrict__ col_val,
+                             uint32_t n_cols, uint32_t n_chunks, const double *__restrict__ means,
+                             double *__restrict__ sq_partial, double *__restrict__ count_partial) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n_chunks * n_cols) return;
+    const unsigned c = (unsigned)(i / n_cols), g = (unsigned)(i % n_cols);
+    const uint32_t *cp = col_ptr + (size_t)c * (n_cols + 1);
+    const double mu = means[g];
+    double s = 0.0;
+    for (uint32_t e = cp[g]; e < cp[g + 1]; e++) {
+        double d = (double)col_val[e] - mu;
+        s += d * d;
+    }
+    sq_partial[i] = s;
+    count_partial[i] = (double)(cp[g + 1] - cp[g]);
+}
+
+// ================================================================== host side ====
+
+static unsigned lds_limit(const Context &ctx) {
+    return (unsigned)std::min<size_t>(ctx.props.sharedMemPerBlock, 64 * 1024);
+}
+
+void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const float *values) {
+    Context &ctx = *csr.ctx;
+    const uint64_t n = csr.n_rows;
+    KY_CHECK(n >= 1, KY_E215, "csr: a layer without cells");
+    KY_CHECK(indptr[0] == 0, KY_E215, "csr: indptr does not start at 0");
+    for (uint64_t r = 0; r < n; r++)
+        KY_CHECK(indptr[r + 1] >= indptr[r], KY_E215, "csr: indptr decreases at row " + std::to_string(r));
+    csr.nnz = indptr[n];
+    KY_CHECK(csr.nnz < (1ull << 32), KY_E505,
+             "csr: a layer holds 2^32 or more values on one device; pass a smaller memory_limit");
+    KY_CHECK(csr.n_cols >= 1, KY_E215, "csr: a layer without genes");
+    csr.indptr.alloc(n + 1);
+    csr.indices.alloc(std::max<uint64_t>(csr.nnz, 1));
+    csr.values.alloc(std::max<uint64_t>(csr.nnz, 1));
+    hipStream_t st = ctx.stream;
+    KY_HIP(hipMemcpyAsync(csr.indptr.ptr, indptr, sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, st));
+    if (csr.nnz) {
+        KY_HIP(hipMemcpyAsync(csr.indices.ptr, indices, sizeof(uint32_t) * csr.nnz, hipMemcpyHostToDevice, st));
+        KY_HIP(hipMemcpyAsync(csr.values.ptr, values, sizeof(float) * csr.nnz, hipMemcpyHostToDevice, st));
+    }
+    DevBuf<unsigned> flags(1);
+    KY_HIP(hipMemsetAsync(flags.ptr, 0, sizeof(unsigned), st));
+    check_rows<<<div_up(n, 256), 256, 0, st>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, n, csr.n_cols,
+                                              flags.ptr);
+    KY_HIP(hipGetLastError());
+    unsigned hflags = 0;
+    KY_HIP(hipMemcpyAsync(&hflags, flags.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    KY_HIP(hipStreamSynchronize(st));
+    KY_CHECK(!(hflags & 1u), KY_E215,
+             "csr: a row's gene indices are not strictly increasing below the number of genes");
+    KY_CHECK(!(hflags & 2u), KY_E215,
+             "pca: the layer holds a value that is not finite or is outside the magnitudes pca's "
+             "float32 products on a GPU keep exact (1e-10 to 1e10)");
+    // Chunked columns.
+    csr.chunk_rows = CHUNK_ROWS;
+    csr.n_chunks = div_up(n, csr.chunk_rows);
+    unsigned waves = std::min<unsigned>(16, std::max<unsigned>(1, lds_limit(ctx) / (4u * csr.n_cols + 256)));
+    size_t lds = sizeof(uint32_t) * ((size_t)waves * csr.n_cols + 64 * waves);
+    KY_CHECK(lds <= ctx.props.sharedMemPerBlock, KY_E215,
+             "csr: too many genes for the device's column copy (" + std::to_string(csr.n_cols) + ")");
+    csr.col_ptr.alloc((size_t)csr.n_chunks * (csr.n_cols + 1));
+    csr.col_row.alloc(std::max<uint64_t>(csr.nnz, 1));
+    csr.col_val.alloc(std::max<uint64_t>(csr.nnz, 1));
+    build_columns<<<csr.n_chunks, 64 * waves, lds, st>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, n,
+                                                         csr.n_cols, csr.chunk_rows, csr.col_ptr.ptr,
+                                                         csr.col_row.ptr, csr.col_val.ptr);
+    KY_HIP(hipGetLastError());
+    KY_HIP(hipStreamSynchronize(st));
+}
+
+// W (n x w floats, row stride ldw) is in ctx.wbuf. Computes X^T W into `product`
+// (host, genes x w) and optionally the column sums of W into `sums` (host, w).
+static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *product, double *sums) {
+    Context &ctx = *csr.ctx;
+    hipStream_t st = ctx.stream;
+    const size_t stride = (size_t)csr.n_cols * w;
+    ctx.partials.reserve(stride * csr.n_chunks);
+    ctx.reduced.reserve(stride + w);
+    const uint32_t gpb = 16;
+    const unsigned groups = div_up(csr.n_cols, gpb);
+    const unsigned slots = div_up(csr.n_chunks, 8);
+    const unsigned blocks = 8 * slots * groups;
+    if (w <= 64)
+        column_product<1><<<blocks, 256, 0, st>>>(csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols,
+                                                  csr.n_chunks, csr.chunk_rows, ctx.wbuf.ptr, w, ldw, gpb,
+                                                  ctx.partials.ptr);
+    else if (w <= 128)
+        column_product<2><<<blocks, 256, 0, st>>>(csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols,
+                                                  csr.n_chunks, csr.chunk_rows, ctx.wbuf.ptr, w, ldw, gpb,
+                                                  ctx.partials.ptr);
+    else
+        column_product<4><<<blocks, 256, 0, st>>>(csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols,
+                                                  csr.n_chunks, csr.chunk_rows, ctx.wbuf.ptr, w, ldw, gpb,
+                                                  ctx.partials.ptr);
+    KY_HIP(hipGetLastError());
+    reduce_chunks<<<div_up(stride, 256), 256, 0, st>>>(ctx.partials.ptr, stride, csr.n_chunks, ctx.reduced.ptr);
+    KY_HIP(hipGetLastError());
+    KY_HIP(hipMemcpyAsync(product, ctx.reduced.ptr, sizeof(double) * stride, hipMemcpyDeviceToHost, st));
+    if (sums) {
+        ctx.sums.reserve((size_t)csr.n_chunks * w + w);
+        column_sums<<<csr.n_chunks, 256, 0, st>>>(ctx.wbuf.ptr, csr.n_rows, w, ldw, csr.chunk_rows, ctx.sums.ptr);
+        KY_HIP(hipGetLastError());
+        double *tot = ctx.sums.ptr + (size_t)csr.n_chunks * w;
+        reduce_chunks<<<div_up(w, 256), 256, 0, st>>>(ctx.sums.ptr, w, csr.n_chunks, tot);
+        KY_HIP(hipGetLastError());
+        KY_HIP(hipMemcpyAsync(sums, tot, sizeof(double) * w, hipMemcpyDeviceToHost, st));
+    }
+    KY_HIP(hipStreamSynchronize(st));
+}
+
+static void launch_row_project(Csr &csr, const float *M, uint32_t w, const double *offsets, float *out,
+                               uint32_t ldo) {
+    Context &ctx = *csr.ctx;
+    const unsigned rows_per_block = 4;
+    const unsigned blocks = div_up(csr.n_rows, rows_per_block);
+    KY_CHECK(w <= 256, KY_E215, "pca: more than 256 columns in one pass");
+    if (w <= 64)
+        row_project<1><<<blocks, 256, 0, ctx.stream>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows,
+                                                       M, w, offsets, out, ldo);
+    else if (w <= 128)
+        row_project<2><<<blocks, 256, 0, ctx.stream>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows,
+                                                       M, w, offsets, out, ldo);
+    else
+        row_project<4><<<blocks, 256, 0, ctx.stream>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows,
+                                                       M, w, offsets, out, ldo);
+    KY_HIP(hipGetLastError());
+}
+
+void pca_sketch(Csr &csr, uint64_t seed, uint64_t cell_offset, uint32_t l, double *sketch, double *gene_sums,
+                double *omega_sums) {
+    Context &ctx = *csr.ctx;
+    KY_CHECK(l >= 1 && l + 1 <= 256, KY_E215, "pca: sketch width out of range");
+    const uint32_t w = l + 1, ldw = w;
+    ctx.wbuf.reserve((size_t)csr.n_rows * ldw);
+    uint64_t total = csr.n_rows * (uint64_t)w;
+    omega_rows<<<div_up(total, 256), 256, 0, ctx.stream>>>(csr.n_rows, seed, cell_offset, l, ctx.wbuf.ptr, ldw);
+    KY_HIP(hipGetLastError());
+    std::vector<double> prod((size_t)csr.n_cols * w), sums(w);
+    transposed_product(csr, w, ldw, prod.data(), sums.data());
+    for (uint32_t g = 0; g < csr.n_cols; g++) {
+        for (uint32_t j = 0; j < l; j++) sketch[(size_t)g * l + j] = prod[(size_t)g * w + j];
+        gene_sums[g] = prod[(size_t)g * w + l];
+    }
+    for (uint32_t j = 0; j < l; j++) omega_sums[j] = sums[j];
+}
+
+void pca_product(Csr &csr, const double *q, uint32_t w, const double *block_means, double *product, double *sums) {
+    Context &ctx = *csr.ctx;
+    KY_CHECK(w >= 1 && w <= 256, KY_E215, "pca: block width out of range");
+    std::vector<float> q32((size_t)csr.n_cols * w);
+    for (size_t i = 0; i < q32.size(); i++) q32[i] = (float)q[i];
+    ctx.dense_f32.reserve(q32.size());
+    ctx.dense_f64.reserve(w);
+    KY_HIP(hipMemcpyAsync(ctx.dense_f32.ptr, q32.data(), sizeof(float) * q32.size(), hipMemcpyHostToDevice,
+                          ctx.stream));
+    KY_HIP(hipMemcpyAsync(ctx.dense_f64.ptr, block_means, sizeof(double) * w, hipMemcpyHostToDevice, ctx.stream));
+    ctx.wbuf.reserve((size_t)csr.n_rows * w);
+    launch_row_project(csr, ctx.dense_f32.ptr, w, ctx.dense_f64.ptr, ctx.wbuf.ptr, w);
+    transposed_product(csr, w, w, product, sums);
+}
+
+void pca_project(Csr &csr, const double *v, uint32_t k, const double *mean_projection, const double *means,
+                 float *host_embedding, Dense **embedding, double *squares, uint64_t *stored) {
+    Context &ctx = *csr.ctx;
+    KY_CHECK(k >= 1 && k <= 256, KY_E215, "pca: n_components out of range");
+    std::vector<float> v32((size_t)csr.n_cols * k);
+    for (size_t i = 0; i < v32.size(); i++) v32[i] = (float)v[i];
+    ctx.dense_f32.reserve(v32.size());
+    ctx.dense_f64.reserve(k + csr.n_cols);
+    hipStream_t st = ctx.stream;
+    KY_HIP(hipMemcpyAsync(ctx.dense_f32.ptr, v32.data(), sizeof(float) * v32.size(), hipMemcpyHostToDevice, st));
+    KY_HIP(hipMemcpyAsync(ctx.dense_f64.ptr, mean_projection, sizeof(double) * k, hipMemcpyHostToDevice, st));
+    KY_HIP(hipMemcpyAsync(ctx.dense_f64.ptr + k, means, sizeof(double) * csr.n_cols, hipMemcpyHostToDevice, st));
+    Dense *emb = new ky_dense();
+    try {
+        emb->ctx = &ctx;
+        emb->rows = csr.n_rows;
+        emb->cols = k;
+        emb->data.alloc((size_t)csr.n_rows * k);
+        launch_row_project(csr, ctx.dense_f32.ptr, k, ctx.dense_f64.ptr, emb->data.ptr, k);
+        // Squares and counts per gene.
+        const size_t cells = (size_t)csr.n_chunks * csr.n_cols;
+        ctx.partials.reserve(2 * cells);
+        ctx.reduced.reserve(2 * (size_t)csr.n_cols);
+        gene_squares<<<div_up(cells, 256), 256, 0, st>>>(csr.col_ptr.ptr, csr.col_val.ptr, csr.n_cols, csr.n_chunks,
+                                                         ctx.dense_f64.ptr + k, ctx.partials.ptr,
+                                                         ctx.partials.ptr + cells);
+        KY_HIP(hipGetLastError());
+        reduce_chunks<<<div_up(csr.n_cols, 256), 256, 0, st>>>(ctx.partials.ptr, csr.n_cols, csr.n_chunks,
+                                                                ctx.reduced.ptr);
+        reduce_chunks<<<div_up(csr.n_cols, 256), 256, 0, st>>>(ctx.partials.ptr + cells, csr.n_cols, csr.n_chunks,
+                                                                ctx.reduced.ptr + csr.n_cols);
+        KY_HIP(hipGetLastError());
+        std::vector<double> counts(csr.n_cols);
+        KY_HIP(hipMemcpyAsync(squares, ctx.reduced.ptr, sizeof(double) * csr.n_cols, hipMemcpyDeviceToHost, st));
+        KY_HIP(hipMemcpyAsync(counts.data(), ctx.reduced.ptr + csr.n_cols, sizeof(double) * csr.n_cols,
+                              hipMemcpyDeviceToHost, st));
+        if (host_embedding)
+            KY_HIP(hipMemcpyAsync(host_embedding, emb->data.ptr, sizeof(float) * csr.n_rows * k,
+                                  hipMemcpyDeviceToHost, st));
+        KY_HIP(hipStreamSynchronize(st));
+        for (uint32_t g = 0; g < csr.n_cols; g++) stored[g] = (uint64_t)counts[g];
+    } catch (...) {
+        delete static_cast<ky_dense *>(emb);
+        throw;
+    }
+    if (embedding)
+        *embedding = emb;
+    else
+        delete static_cast<ky_dense *>(emb);
+}
+
+// The whole pca(): PcaAccumulator's passes (pca.rs:302-862) with GPU folds.
+void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *host_embedding, Dense **embedding,
+             float *loadings_out, double *variance, double *variance_ratio, double *means_out, uint32_t *passes_out,
+             ky_pca_timing *timing) {
+    const double t_start = now_ms();
+    double dense_ms = 0.0;
+    const uint64_t n = csr.n_rows;
+    const size_t genes = csr.n_cols;
+    // check_cells (pca.rs:876-920).
+    KY_CHECK(n >= 2, KY_E215,
+             std::to_string(n) + " cells reach pca (the filters before it keep fewer than 2)");
+    KY_CHECK(n_components >= 1 && n_components <= std::min<uint64_t>(n, genes), KY_E215,
+             "pca: n_components " + std::to_string(n_components) + " is above min(cells, genes) = " +
+                 std::to_string(std::min<uint64_t>(n, genes)));
+    const size_t k = n_components;
+    const size_t width = std::min<size_t>(k + PCA_OVERSAMPLES, genes);
+    uint32_t pass_idx = 0;
+    auto pass_time = [&](double t0) {
+        if (timing && pass_idx < 16) timing->pass_ms[pass_idx] = now_ms() - t0;
+        pass_idx++;
+    };
+    // Pass 0: the sketch.
+    double t0 = now_ms();
+    Mat sketch(genes, width);
+    std::vector<double> sums(genes), omega_sums(width);
+    pca_sketch(csr, seed, 0, (uint32_t)width, sketch.v.data(), sums.data(), omega_sums.data());
+    pass_time(t0);
+    double td = now_ms();
+    std::vector<double> means(genes);
+    for (size_t g = 0; g < genes; g++) {
+        means[g] = sums[g] / (double)n;
+        KY_CHECK(std::isfinite(means[g]), KY_E215,
+                 "pca: gene " + std::to_string(g) + " holds a value that is not finite (NaN or infinity)");
+    }
+    for (size_t g = 0; g < genes; g++)
+        for (size_t j = 0; j < width; j++) sketch.at(g, j) -= means[g] * omega_sums[j];
+    Mat basis(genes, 0), products(genes, 0);
+    Mat block = extend_basis(basis, sketch);
+    std::vector<double> eigenvalues;
+    Mat loadings;
+    bool finished = false;
+    auto block_means_of = [&](const Mat &b) {
+        std::vector<double> bm(b.cols, 0.0);
+        for (size_t g = 0; g < genes; g++)
+            for (size_t j = 0; j < b.cols; j++) bm[j] += means[g] * b.at(g, j);
+        return bm;
+    };
+    auto finish_basis = [&]() {
+        rayleigh_ritz(basis, products, k, eigenvalues, loadings);
+        finished = true;
+    };
+    if (block.cols == 0) finish_basis();
+    dense_ms += now_ms() - td;
+    uint32_t passes = 0;
+    for (uint32_t p = 1; p <= q && !finished; p++) {
+        const size_t w = block.cols;
+        std::vector<double> bm = block_means_of(block);
+        Mat product(genes, w);
+        std::vector<double> wsum(w);
+        t0 = now_ms();
+        pca_product(csr, block.v.data(), (uint32_t)w, bm.data(), product.v.data(), wsum.data());
+        pass_time(t0);
+        passes++;
+        td = now_ms();
+        for (size_t g = 0; g < genes; g++)
+            for (size_t j = 0; j < w; j++) product.at(g, j) -= means[g] * wsum[j];
+        basis = basis.beside(block);
+        products = products.beside(product);
+        if (p == q) {
+            finish_basis();
+        } else {
+            block = extend_basis(basis, product);
+            if (block.cols == 0) finish_basis();
+        }
+        dense_ms += now_ms() - td;
+    }
+    if (!finished) {
+        td = now_ms();
+        finish_basis();
+        dense_ms += now_ms() - td;
+    }
+    // mean_projection = mu^T V.
+    std::vector<double> mean_projection(k, 0.0);
+    for (size_t g = 0; g < genes; g++)
+        for (size_t j = 0; j < k; j++) mean_projection[j] += means[g] * loadings.at(g, j);
+    std::vector<double> squares(genes);
+    std::vector<uint64_t> stored(genes);
+    t0 = now_ms();
+    pca_project(csr, loadings.v.data(), (uint32_t)k, mean_projection.data(), means.data(), host_embedding, embedding,
+                squares.data(), stored.data());
+    pass_time(t0);
+    // Total variance (pca.rs:776-791) and finish (pca.rs:804-845).
+    double total = 0.0;
+    for (size_t g = 0; g < genes; g++)
+        total += squares[g] + (double)(n - stored[g]) * means[g] * means[g];
+    const double total_variance = total / ((double)n - 1.0);
+    const bool constant = total_variance == 0.0;
+    for (size_t j = 0; j < k; j++) {
+        variance[j] = constant ? 0.0 : eigenvalues[j] / ((double)n - 1.0);
+        variance_ratio[j] = variance[j] / total_variance;
+    }
+    if (loadings_out)
+        for (size_t i = 0; i < genes * k; i++) loadings_out[i] = (float)loadings.v[i];
+    if (means_out)
+        for (size_t g = 0; g < genes; g++) means_out[g] = means[g];
+    if (passes_out) *passes_out = passes;
+    if (timing) {
+        timing->dense_ms = dense_ms;
+        timing->total_ms = now_ms() - t_start;
+        timing->n_passes = pass_idx;
+    }
+}
+
+}  // namespace ky

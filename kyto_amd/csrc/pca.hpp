@@ -1,0 +1,23 @@
+// pca() on the device (pca.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace ky {
+
+// Rows of one chunk of the column-major copy (the transposed product's unit).
+constexpr uint32_t CHUNK_ROWS = 8192;
+// estimate.rs:153: the sketch's extra columns.
+constexpr uint32_t PCA_OVERSAMPLES = 10;
+
+void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const float *values);
+void pca_sketch(Csr &csr, uint64_t seed, uint64_t cell_offset, uint32_t l, double *sketch, double *gene_sums,
+                double *omega_sums);
+void pca_product(Csr &csr, const double *q, uint32_t w, const double *block_means, double *product, double *sums);
+void pca_project(Csr &csr, const double *v, uint32_t k, const double *mean_projection, const double *means,
+                 float *host_embedding, Dense **embedding, double *squares, uint64_t *stored);
+void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *host_embedding, Dense **embedding,
+             float *loadings_out, double *variance, double *variance_ratio, double *means_out, uint32_t *passes_out,
+             ky_pca_timing *timing);
+
+}  // namespace ky

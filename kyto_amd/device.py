@@ -1,0 +1,330 @@
+"""Device handles over libkyto_gpu.so: a context, resident CSR layers and dense
+matrices, and the pca / neighbors / IVF entry points as numpy-facing calls.
+
+Mirrors kyto-gpu's DeviceKernels (crates/kyto-gpu/src/launch.rs:42-93): one handle
+per device, results returned as owned arrays, failures raised as KytoError with
+kyto's codes (KY-E215 arguments/data, KY-E501 no kernel, KY-E504 no device,
+KY-E505 device failure).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_uint32, c_uint64, c_void_p
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    N.check(N.gpu_lib().ky_device_count(byref(n)))
+    return int(n.value)
+
+
+@dataclass
+class PcaOutput:
+    embedding: np.ndarray | None   # cells x k f32 (None when left on the device only)
+    loadings: np.ndarray           # genes x k f32
+    variance: np.ndarray           # k f64
+    variance_ratio: np.ndarray     # k f64
+    means: np.ndarray              # genes f64
+    passes: int
+    timing: dict = field(default_factory=dict)
+    device_embedding: "DenseMatrix | None" = None
+
+
+@dataclass
+class Graph:
+    """kyto's NeighborGraph (neighbors.rs:52-175): CSR, k = n_neighbors - 1 a row."""
+    n_neighbors: int
+    indptr: np.ndarray     # u64, n + 1
+    indices: np.ndarray    # u32, n * k, increasing within a row
+    distances: np.ndarray  # f32
+    info: dict = field(default_factory=dict)
+
+    @property
+    def k(self) -> int:
+        return self.n_neighbors - 1
+
+    def rows(self) -> np.ndarray:
+        return self.indices.reshape(-1, self.k)
+
+
+class Device:
+    """One GPU context (ky_open)."""
+
+    def __init__(self, ordinal: int = 0):
+        self._lib = N.gpu_lib()
+        h = c_void_p()
+        N.check(self._lib.ky_open(int(ordinal), byref(h)))
+        self._h = h
+        self.ordinal = ordinal
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            N.check(self._lib.ky_close(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        name = ctypes.create_string_buffer(128)
+        total = c_uint64(0)
+        N.check(self._lib.ky_device_info(self._h, name, 128, byref(total)))
+        return {"name": name.value.decode(), "memory_bytes": int(total.value)}
+
+    # ------------------------------------------------------------ residency --
+
+    def upload_csr(self, indptr, indices, values, n_genes: int) -> "CsrLayer":
+        return CsrLayer(self, indptr, indices, values, n_genes)
+
+    def upload_dense(self, x) -> "DenseMatrix":
+        return DenseMatrix.upload(self, x)
+
+    # -------------------------------------------------------------------- pca --
+
+    def pca(self, layer: "CsrLayer", n_components: int = 50, seed: int = 0, power_iterations: int = 9,
+            download_embedding: bool = True, keep_on_device: bool = True) -> PcaOutput:
+        genes = layer.n_genes
+        k = int(n_components)
+        emb = np.empty((layer.n_rows, k), np.float32) if download_embedding else None
+        loadings = np.empty((genes, k), np.float32)
+        variance = np.empty(k)
+        ratio = np.empty(k)
+        means = np.empty(genes)
+        passes = c_uint32(0)
+        timing = N.PcaTiming()
+        dh = c_void_p()
+        N.check(self._lib.ky_pca(self._h, layer.handle, k, int(seed), int(power_iterations),
+                                 N.ptr(emb, N.f32p) if emb is not None else None,
+                                 byref(dh) if keep_on_device else None,
+                                 N.ptr(loadings, N.f32p), N.ptr(variance, N.f64p), N.ptr(ratio, N.f64p),
+                                 N.ptr(means, N.f64p), byref(passes), byref(timing)))
+        dev = DenseMatrix._adopt(self, dh, layer.n_rows, k) if keep_on_device else None
+        t = {"pass_ms": [timing.pass_ms[i] for i in range(min(16, timing.n_passes))],
+             "dense_ms": timing.dense_ms, "total_ms": timing.total_ms}
+        return PcaOutput(emb, loadings, variance, ratio, means, int(passes.value), t, dev)
+
+    def pca_sketch(self, layer, seed, l, cell_offset=0):
+        g = layer.n_genes
+        sketch = np.empty((g, l))
+        sums = np.empty(g)
+        osum = np.empty(l)
+        N.check(self._lib.ky_pca_sketch(self._h, layer.handle, int(seed), int(cell_offset), int(l),
+                                        N.ptr(sketch, N.f64p), N.ptr(sums, N.f64p), N.ptr(osum, N.f64p)))
+        return sketch, sums, osum
+
+    def pca_product(self, layer, q, block_means):
+        q = np.ascontiguousarray(q, np.float64)
+        bm = np.ascontiguousarray(block_means, np.float64)
+        g, w = q.shape
+        prod = np.empty((g, w))
+        sums = np.empty(w)
+        N.check(self._lib.ky_pca_product(self._h, layer.handle, N.ptr(q, N.f64p), w, N.ptr(bm, N.f64p),
+                                         N.ptr(prod, N.f64p), N.ptr(sums, N.f64p)))
+        return prod, sums
+
+    def pca_project(self, layer, v, mean_projection, means):
+        v = np.ascontiguousarray(v, np.float64)
+        mp = np.ascontiguousarray(mean_projection, np.float64)
+        mu = np.ascontiguousarray(means, np.float64)
+        g, k = v.shape
+        emb = np.empty((layer.n_rows, k), np.float32)
+        sq = np.empty(g)
+        st = np.empty(g, np.uint64)
+        N.check(self._lib.ky_pca_project(self._h, layer.handle, N.ptr(v, N.f64p), k, N.ptr(mp, N.f64p),
+                                         N.ptr(mu, N.f64p), N.ptr(emb, N.f32p), None, N.ptr(sq, N.f64p),
+                                         N.ptr(st, N.u64p)))
+        return emb, sq, st
+
+    # -------------------------------------------------------------- neighbours --
+
+    def neighbors(self, points: "DenseMatrix", n_neighbors: int = 15, n_components: int | None = None,
+                  metric: str = "euclidean", precision: str = "fp32") -> Graph:
+        n = points.rows
+        k = int(n_neighbors) - 1
+        if metric not in N.METRIC:
+            raise N.KytoError(N.KY_E215, f"neighbors: metric {metric!r} is not 'euclidean' or 'cosine'")
+        if precision not in N.PRECISION:
+            raise N.KytoError(N.KY_E215, f"neighbors: precision {precision!r} is not 'fp32' or 'bf16'")
+        idx = np.empty(max(n * k, 1), np.uint32)
+        dist = np.empty(max(n * k, 1), np.float32)
+        info = N.KnnInfo()
+        N.check(self._lib.ky_neighbors(self._h, points.handle, int(n_components or 0), int(n_neighbors),
+                                       N.METRIC[metric], N.PRECISION[precision], N.ptr(idx, N.u32p),
+                                       N.ptr(dist, N.f32p), byref(info)))
+        indptr = np.arange(0, n * k + 1, k, dtype=np.uint64)
+        return Graph(int(n_neighbors), indptr, idx[: n * k], dist[: n * k], _info(info))
+
+    def nearest_keys(self, points: "DenseMatrix", first: int, count: int, m: int,
+                     n_components: int | None = None):
+        keys = np.empty((count, m), np.float32)
+        rows = np.empty((count, m), np.uint32)
+        N.check(self._lib.ky_nearest_keys(self._h, points.handle, int(n_components or 0), int(first), int(count),
+                                          int(m), N.PRECISION["fp32"], N.ptr(keys, N.f32p), N.ptr(rows, N.u32p)))
+        return keys, rows
+
+    # ---------------------------------------------------------------------- IVF --
+
+    def ivf_build(self, points: "DenseMatrix", n_lists=4096, kmeans_iters=10, seed=0, hidden=256,
+                  train_queries=200_000, batch=4096, epochs=4, lr=1e-3, label_k=14) -> "IvfIndex":
+        p = N.IvfParams(n_lists, kmeans_iters, seed, hidden, train_queries, batch, epochs, lr, label_k)
+        h = c_void_p()
+        info = N.IvfInfo()
+        N.check(self._lib.ky_ivf_build(self._h, points.handle, byref(p), byref(h), byref(info)))
+        return IvfIndex(self, h, {f: getattr(info, f) for f, _ in N.IvfInfo._fields_})
+
+
+def _info(info) -> dict:
+    return {f: getattr(info, f) for f, _ in type(info)._fields_}
+
+
+class CsrLayer:
+    """A CSR layer resident on the device (ky_csr_upload)."""
+
+    def __init__(self, dev: Device, indptr, indices, values, n_genes: int):
+        self.dev = dev
+        indptr = np.ascontiguousarray(indptr, np.uint64)
+        indices = np.ascontiguousarray(indices, np.uint32)
+        values = np.ascontiguousarray(values, np.float32)
+        self.n_rows = len(indptr) - 1
+        self.n_genes = int(n_genes)
+        self.nnz = int(indptr[-1]) if len(indptr) else 0
+        h = c_void_p()
+        N.check(dev._lib.ky_csr_upload(dev.handle, N.ptr(indptr, N.u64p), N.ptr(indices, N.u32p),
+                                       N.ptr(values, N.f32p), self.n_rows, self.n_genes, byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def free(self):
+        if self._h:
+            N.check(self.dev._lib.ky_csr_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DenseMatrix:
+    """A row-major f32 matrix resident on the device."""
+
+    def __init__(self):
+        self.dev = None
+        self._h = None
+        self.rows = 0
+        self.cols = 0
+
+    @classmethod
+    def upload(cls, dev: Device, x) -> "DenseMatrix":
+        x = np.ascontiguousarray(x, np.float32)
+        if x.ndim != 2:
+            raise N.KytoError(N.KY_E215, "a dense matrix is 2-D")
+        h = c_void_p()
+        N.check(dev._lib.ky_dense_upload(dev.handle, N.ptr(x, N.f32p), x.shape[0], x.shape[1], byref(h)))
+        return cls._adopt(dev, h, x.shape[0], x.shape[1])
+
+    @classmethod
+    def _adopt(cls, dev, h, rows, cols) -> "DenseMatrix":
+        m = cls()
+        m.dev, m._h, m.rows, m.cols = dev, h, int(rows), int(cols)
+        return m
+
+    @property
+    def handle(self):
+        return self._h
+
+    def download(self) -> np.ndarray:
+        out = np.empty((self.rows, self.cols), np.float32)
+        N.check(self.dev._lib.ky_dense_download(self._h, N.ptr(out, N.f32p)))
+        return out
+
+    def free(self):
+        if self._h:
+            N.check(self.dev._lib.ky_dense_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class IvfIndex:
+    """A learned IVF index on the device (ky_ivf_build)."""
+
+    def __init__(self, dev: Device, h, info: dict):
+        self.dev, self._h, self.info = dev, h, info
+
+    def search(self, points: DenseMatrix, n_neighbors=15, n_probe=8, use_router=True, confidence=0.0) -> Graph:
+        n = points.rows
+        k = int(n_neighbors) - 1
+        idx = np.empty(n * k, np.uint32)
+        dist = np.empty(n * k, np.float32)
+        info = N.IvfSearchInfo()
+        N.check(self.dev._lib.ky_ivf_search(self.dev.handle, self._h, points.handle, int(n_neighbors), int(n_probe),
+                                            1 if use_router else 0, float(confidence), N.ptr(idx, N.u32p),
+                                            N.ptr(dist, N.f32p), byref(info)))
+        indptr = np.arange(0, n * k + 1, k, dtype=np.uint64)
+        return Graph(int(n_neighbors), indptr, idx, dist, _info(info))
+
+    def free(self):
+        if self._h:
+            N.check(self.dev._lib.ky_ivf_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# Host-only FP64 dense steps (no device needed).
+
+def dense_extend_basis(basis: np.ndarray, block: np.ndarray) -> np.ndarray:
+    lib = N.gpu_lib()
+    basis = np.ascontiguousarray(basis, np.float64)
+    block = np.ascontiguousarray(block, np.float64)
+    rows, bw = block.shape
+    bc = basis.shape[1] if basis.size else 0
+    out = np.zeros((rows, bw))
+    kept = c_uint32(0)
+    N.check(lib.ky_dense_extend_basis(N.ptr(basis, N.f64p) if bc else None, rows, bc, N.ptr(block, N.f64p), bw,
+                                      N.ptr(out, N.f64p), byref(kept)))
+    return out[:, : kept.value].copy()
+
+
+def dense_rayleigh_ritz(basis: np.ndarray, products: np.ndarray, k: int):
+    lib = N.gpu_lib()
+    basis = np.ascontiguousarray(basis, np.float64)
+    products = np.ascontiguousarray(products, np.float64)
+    rows, cols = basis.shape
+    ev = np.zeros(k)
+    lo = np.zeros((rows, k))
+    N.check(lib.ky_dense_rayleigh_ritz(N.ptr(basis, N.f64p) if cols else None,
+                                       N.ptr(products, N.f64p) if cols else None, rows, cols, k,
+                                       N.ptr(ev, N.f64p), N.ptr(lo, N.f64p)))
+    return ev, lo

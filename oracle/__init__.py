@@ -73,6 +73,9 @@ def lib():
         L.ko_knn_exact.restype = ctypes.c_int
         L.ko_knn_exact.argtypes = [_f32p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int,
                                    ctypes.c_size_t, _f64p, _u32p, _u64p]
+        L.ko_knn_exact_range.restype = ctypes.c_int
+        L.ko_knn_exact_range.argtypes = [_f32p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t,
+                                         ctypes.c_size_t, ctypes.c_size_t, _f64p, _u32p, _u64p]
         L.ko_round_up.restype = ctypes.c_float
         L.ko_round_up.argtypes = [ctypes.c_double]
         L.ko_max_threads.restype = ctypes.c_int
@@ -345,6 +348,22 @@ def knn_exact(points, k, metric="euclidean", threads=None):
     fb = np.zeros(1, np.uint64)
     L.ko_knn_exact(_p(pts, _f32p), n, w, 1 if metric == "cosine" else 0, k, _p(keys, _f64p),
                    _p(rows, _u32p), _p(fb, _u64p))
+    return keys, rows, int(fb[0])
+
+
+def knn_exact_range(points, k, q_first, q_count, threads=None):
+    """The filtered exact search for queries [q_first, q_first + q_count) only (the
+    CPU baseline's bounded sample). Returns (keys, rows, fallback count)."""
+    L = lib()
+    if threads is not None:
+        L.ko_set_threads(int(threads))
+    pts = np.ascontiguousarray(points, np.float32)
+    n, w = pts.shape
+    keys = np.zeros((q_count, k))
+    rows = np.zeros((q_count, k), np.uint32)
+    fb = np.zeros(1, np.uint64)
+    L.ko_knn_exact_range(_p(pts, _f32p), n, w, 0, k, q_first, q_count, _p(keys, _f64p), _p(rows, _u32p),
+                         _p(fb, _u64p))
     return keys, rows, int(fb[0])
 
 

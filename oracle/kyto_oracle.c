@@ -328,8 +328,8 @@ static void f32_products(const float *qs, size_t nq, const float *ps, size_t np,
  * `Prepared::new` :92-180 (Euclidean and cosine), `search_block` :257-307, `scan`
  * :313-334, `rank` :338-355; the fallback is `search_row`. Returns 1 when the
  * filter ran, 0 when norms forced brute force (`Prepared::new` -> None). */
-int ko_knn_exact(const float *pts, size_t n, size_t w, int cosine, size_t k, double *keys_out,
-                 uint32_t *rows_out, uint64_t *fallback_rows) {
+int ko_knn_exact_range(const float *pts, size_t n, size_t w, int cosine, size_t k, size_t q_first,
+                       size_t q_count, double *keys_out, uint32_t *rows_out, uint64_t *fallback_rows) {
     double factor = 2.0 * ((double)w + 8.0) * UNIT_ROUNDOFF;
     double underflow = 4.0 * ((double)w + 8.0) * (double)FLT_MIN;
     float *rows = (float *)malloc(sizeof(float) * n * w);
@@ -357,11 +357,11 @@ int ko_knn_exact(const float *pts, size_t n, size_t w, int cosine, size_t k, dou
         largest = sqrt(largest);
         if (!isfinite(largest) || largest > LARGEST_NORM) {
             free(squares); free(rows); free(slack);
-            uint32_t *all = (uint32_t *)malloc(sizeof(uint32_t) * n);
-            for (size_t r = 0; r < n; r++) all[r] = (uint32_t)r;
-            ko_search_rows(pts, n, w, NULL, k, all, n, keys_out, rows_out);
+            uint32_t *all = (uint32_t *)malloc(sizeof(uint32_t) * q_count);
+            for (size_t r = 0; r < q_count; r++) all[r] = (uint32_t)(q_first + r);
+            ko_search_rows(pts, n, w, NULL, k, all, q_count, keys_out, rows_out);
             free(all);
-            if (fallback_rows) *fallback_rows = n;
+            if (fallback_rows) *fallback_rows = q_count;
             return 0;
         }
         offsets = (float *)malloc(sizeof(float) * n);
@@ -381,14 +381,15 @@ int ko_knn_exact(const float *pts, size_t n, size_t w, int cosine, size_t k, dou
         }
     }
     uint64_t fallbacks = 0;
-    size_t n_blocks = (n + QUERY_ROWS - 1) / QUERY_ROWS;
+    size_t q_end = q_first + q_count;
+    size_t n_blocks = (q_count + QUERY_ROWS - 1) / QUERY_ROWS;
     #pragma omp parallel reduction(+ : fallbacks)
     {
         float *products = (float *)malloc(sizeof(float) * QUERY_ROWS * POINT_ROWS);
         query_t states[QUERY_ROWS];
         #pragma omp for schedule(dynamic, 1)
         for (size_t b = 0; b < n_blocks; b++) {
-            size_t first = b * QUERY_ROWS, last = first + QUERY_ROWS < n ? first + QUERY_ROWS : n;
+            size_t first = q_first + b * QUERY_ROWS, last = first + QUERY_ROWS < q_end ? first + QUERY_ROWS : q_end;
             size_t nq = last - first;
             for (size_t i = 0; i < nq; i++) {
                 memset(&states[i], 0, sizeof(query_t));
@@ -423,8 +424,8 @@ int ko_knn_exact(const float *pts, size_t n, size_t w, int cosine, size_t k, dou
             for (size_t i = 0; i < nq; i++) {
                 size_t query = first + i;
                 query_t *s = &states[i];
-                double *ok = keys_out + query * k;
-                uint32_t *orow = rows_out + query * k;
+                double *ok = keys_out + (query - q_first) * k;
+                uint32_t *orow = rows_out + (query - q_first) * k;
                 nearest_t nn = {k, 0, ok, orow};
                 int good = !s->overflowed;
                 for (size_t c = 0; good && c < s->ncand; c++) {
@@ -447,6 +448,11 @@ int ko_knn_exact(const float *pts, size_t n, size_t w, int cosine, size_t k, dou
     if (fallback_rows) *fallback_rows = fallbacks;
     free(rows); free(offsets); free(slack); free(norms);
     return 1;
+}
+
+int ko_knn_exact(const float *pts, size_t n, size_t w, int cosine, size_t k, double *keys_out,
+                 uint32_t *rows_out, uint64_t *fallback_rows) {
+    return ko_knn_exact_range(pts, n, w, cosine, k, 0, n, keys_out, rows_out, fallback_rows);
 }
 
 int ko_max_threads(void) {

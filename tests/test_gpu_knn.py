@@ -1,0 +1,117 @@
+"""neighbors() on the device against the oracle: the graph must be kyto's exact
+graph bit for bit (indices and f32 distances), whatever the candidate precision,
+on the golden embeddings, clustered / far / tied / duplicated / tiny inputs, and
+the argument errors."""
+import numpy as np
+import pytest
+
+import oracle
+from kyto_amd import _native as N
+from tests.helpers import clustered, load, scanpy_rows
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = ["fp32"]  # bf16 joins when its kernel lands
+
+
+def _exact(device, pts, n_neighbors, precision, n_components=None):
+    m = device.upload_dense(pts)
+    try:
+        return device.neighbors(m, n_neighbors, n_components=n_components, precision=precision)
+    finally:
+        m.free()
+
+
+def _assert_oracle_graph(g, pts, n_neighbors):
+    _, idx, dist = oracle.neighbors(pts, n_neighbors)
+    got = g.rows()
+    assert np.array_equal(got, idx)
+    assert np.array_equal(g.distances.reshape(idx.shape).view(np.uint32), dist.view(np.uint32))
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_pbmc3k_embedding_equals_oracle_and_scanpy(device, precision):
+    emb = load("pbmc3k_pca.npz")["embedding"][:, :50].copy()
+    g = _exact(device, emb, 15, precision)
+    _assert_oracle_graph(g, emb, 15)
+    ri, rd = scanpy_rows("pbmc3k", 2700)
+    assert np.array_equal(g.rows(), ri)
+    d = g.distances.reshape(2700, 14).astype(np.float64)
+    assert (np.abs(d - rd) / np.maximum(rd, 1e-12)).max() <= 2e-7
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_synthetic_embedding_equals_oracle(device, precision):
+    emb = load("synthetic_pca.npz")["embedding"].copy()
+    g = _exact(device, emb, 15, precision)
+    _assert_oracle_graph(g, emb, 15)
+    # n_components: the leading 10 of the 20 columns.
+    g10 = _exact(device, emb, 15, precision, n_components=10)
+    _assert_oracle_graph(g10, np.ascontiguousarray(emb[:, :10]), 15)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_clusters_far_ties_and_duplicates(device, precision):
+    cases = [
+        (clustered(1, 3000, 50, 8, 0.0), 15),
+        (clustered(2, 1500, 7, 8, 0.0), 6),
+        (clustered(4, 2000, 20, 5, 1000.0), 15),
+        (np.full((300, 3), 0.25, np.float32), 8),
+    ]
+    rng = np.random.default_rng(5)
+    cases.append((np.floor(rng.random((6000, 2)) * 4).astype(np.float32), 15))  # 16 distinct rows
+    cases.append((np.floor(rng.random((2000, 3)) * 6).astype(np.float32), 11))
+    for pts, nn in cases:
+        g = _exact(device, pts, nn, precision)
+        _assert_oracle_graph(g, pts, nn)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_small_inputs(device, precision):
+    for pts, nn in [
+        (np.array([[0.0], [1.0]], np.float32), 2),
+        (np.array([[0.0], [1.0], [3.0], [7.0]], np.float32), 3),
+        (clustered(6, 16, 4, 2, 0.0), 16),
+        (clustered(7, 1024 + 128 + 3, 9, 3, 0.0), 5),
+        (clustered(8, 65, 64, 3, 0.0), 65),
+    ]:
+        g = _exact(device, pts, nn, precision)
+        _assert_oracle_graph(g, pts, nn)
+    g = _exact(device, np.array([[0.0], [1.0], [3.0], [7.0]], np.float32), 3, precision)
+    assert list(g.indices) == [1, 2, 0, 2, 0, 1, 1, 2]
+    assert list(g.distances) == [1.0, 3.0, 1.0, 2.0, 3.0, 2.0, 6.0, 4.0]
+
+
+def test_huge_norms_searched_exactly(device):
+    pts = clustered(9, 400, 3, 2, 0.0) * np.float32(1e20)
+    g = _exact(device, pts, 6, "fp32")
+    _assert_oracle_graph(g, pts, 6)
+
+
+def test_refusals(device):
+    pts = np.zeros((3, 2), np.float32)
+    m = device.upload_dense(pts)
+    with pytest.raises(N.KytoError, match="KY-E215"):
+        device.neighbors(m, 1)
+    with pytest.raises(N.KytoError, match="KY-E215"):
+        device.neighbors(m, 4)
+    with pytest.raises(N.KytoError, match="KY-E215"):
+        device.neighbors(m, 2, n_components=3)
+    with pytest.raises(N.KytoError, match="KY-E501"):
+        device.neighbors(m, 2, metric="cosine")
+    bad = device.upload_dense(np.array([[0.0, np.nan], [1.0, 2.0]], np.float32))
+    with pytest.raises(N.KytoError, match="KY-E215"):
+        device.neighbors(bad, 2)
+
+
+def test_nearest_keys_are_the_smallest_approximate_keys(device):
+    pts = clustered(3, 2100, 16, 9, 0.0)
+    m = device.upload_dense(pts)
+    keys, rows = device.nearest_keys(m, 100, 50, 20)
+    assert np.all(np.diff(keys, axis=1) >= 0)
+    for i in range(50):
+        q = 100 + i
+        d = ((pts.astype(np.float64) - pts[q].astype(np.float64)) ** 2).sum(axis=1)
+        d[q] = np.inf
+        true = np.sort(d)[:20]
+        assert np.abs(keys[i].astype(np.float64) - true).max() < 1e-3 * (1 + true.max())
