@@ -16,6 +16,14 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// As wave_sync, for device-memory writes of one lane read by another lane of the
+// same wavefront (waits for the writes to complete).
+__device__ __forceinline__ void wave_sync_memory() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 // Lanes below this one whose bit is set in mask.

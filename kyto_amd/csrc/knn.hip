@@ -309,11 +309,11 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
             if (floor_mode == 0) {
                 floor = far - slack[q];
             } else {
-                // BF16: the m-th candidate's key is the squared distance of the rounded
-                // rows up to a relative accumulation error; every other row's rounded
-                // distance is at least sqrt(far/(1+rel)) - err_q - err_max.
-                double dr = sqrt(fmax(far / (1.0 + bf_acc_rel) - 0x1p-100, 0.0)) - bf_err[q] - bf_err_max;
-                floor = dr > 0.0 ? dr * dr : -1.0;
+                // BF16 (knn_bf16.hip): keys >= far give |b_q - b_p| >= sqrt(far - delta_q),
+                // so the exact distance is at least that less err_q + max err.
+                const double s = far - bf_err[n + q];
+                const double r = s > 0.0 ? sqrt(s) - bf_err[q] - bf_err_max : -1.0;
+                floor = r > 0.0 ? r * r * (1.0 - 1e-12) : -1.0;
             }
             cert = isfinite(far) && floor > kth;
         }
@@ -560,6 +560,9 @@ void neighbors_run(Context &ctx, const Dense &points, uint32_t n_components, uin
     uint32_t m = 0;
     double search_ms = 0, rerank_ms = 0, fallback_ms = 0;
     std::vector<uint32_t> list;
+    // The BF16 kernel's augmented rows hold d + 6 <= 64 components; wider embeddings
+    // take their candidates from the FP32 kernel (the graph is the same).
+    if (precision == KY_PRECISION_BF16 && d > 58) precision = KY_PRECISION_FP32;
     if (prep.filter_ok) {
         m = candidates_for(k, n, precision);
         DevBuf<uint32_t> c_rows(n * m), c_count(n);

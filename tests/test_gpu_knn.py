@@ -11,7 +11,7 @@ from tests.helpers import clustered, load, scanpy_rows
 
 pytestmark = pytest.mark.gpu
 
-PRECISIONS = ["fp32"]  # bf16 joins when its kernel lands
+PRECISIONS = ["fp32", "bf16"]
 
 
 def _exact(device, pts, n_neighbors, precision, n_components=None):
