@@ -195,6 +195,20 @@ int ky_ivf_search(ky_ctx *ctx, const ky_ivf *ivf, const ky_dense *points, uint32
                   uint32_t n_probe, int use_router, float confidence, uint32_t *indices,
                   float *distances, ky_ivf_search_info *info);
 
+/* One training step's pieces on the device, exposed so the router's kernels can be
+ * checked against an FP64 CPU restatement (no counterpart in the reference).
+ * params: [W1 (d_in x hidden), b1, W2 (hidden x hidden), b2, W3 (hidden x n_out), b3]
+ * (f32, row-major); x: batch x d_in; targets as tk (list, weight) pairs a row.
+ * Writes the gradients (same layout) of the mean cross-entropy and the mean loss. */
+int ky_router_grad(ky_ctx *ctx, uint32_t d_in, uint32_t hidden, uint32_t n_out, const float *params,
+                   const float *x, uint32_t batch, const uint32_t *t_idx, const float *t_w, uint32_t tk,
+                   float *grads, float *loss);
+
+/* One Adam step (lr, beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected at `step`)
+ * on host arrays w, m, v (updated in place) with gradient g. */
+int ky_adam_step(ky_ctx *ctx, float *w, const float *g, float *m, float *v, uint64_t count, float lr,
+                 uint32_t step);
+
 /* -------------------------------------------- host-only FP64 dense steps ---- */
 /* The FP64 steps of the PCA on the host, exported so they can be checked without a
  * device. Row-major matrices. */

@@ -89,6 +89,9 @@ SIGNATURES = {
     "ky_ivf_free": [c_void_p],
     "ky_ivf_search": [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_float, u32p, f32p,
                       POINTER(IvfSearchInfo)],
+    "ky_router_grad": [c_void_p, c_uint32, c_uint32, c_uint32, f32p, f32p, c_uint32, u32p, f32p, c_uint32, f32p,
+                       POINTER(c_float)],
+    "ky_adam_step": [c_void_p, f32p, f32p, f32p, f32p, c_uint64, c_float, c_uint32],
     "ky_dense_extend_basis": [f64p, c_uint32, c_uint32, f64p, c_uint32, f64p, u32p],
     "ky_dense_rayleigh_ritz": [f64p, f64p, c_uint32, c_uint32, c_uint32, f64p, f64p],
 }

@@ -28,7 +28,10 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-GPU_SOURCES = ["pca.hip", "knn.hip", "knn_bf16.hip", "ivf.hip", "capi.cpp", "dense.cpp"]
+# MFMA kernels keep their accumulators in ordinary VGPRs (no accumulator-register
+# copies before the epilogue reads them).
+MFMA_FLAGS = {"knn_bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"], "router.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+GPU_SOURCES = ["pca.hip", "knn.hip", "knn_bf16.hip", "router.hip", "ivf.hip", "sort.hip", "capi.cpp", "dense.cpp"]
 
 
 def _run(cmd):
@@ -42,7 +45,7 @@ def _compile(src: str) -> str:
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     path = os.path.join(CSRC, src)
     # .cpp sources hold host code only; hipcc compiles them with the HIP headers.
-    cmd = [HIPCC, *HIP_FLAGS, "-c", path, "-o", out]
+    cmd = [HIPCC, *HIP_FLAGS, *MFMA_FLAGS.get(src, []), "-c", path, "-o", out]
     _run(cmd)
     return out
 

@@ -4,12 +4,14 @@
 
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 #include "dense.hpp"
 #include "ivf.hpp"
 #include "knn.hpp"
 #include "pca.hpp"
+#include "router.hpp"
 
 namespace ky {
 
@@ -307,6 +309,66 @@ int ky_ivf_search(ky_ctx *ctx, const ky_ivf *ivf, const ky_dense *points, uint32
         need(distances, "distances");
         use_device(*ctx);
         ivf_search(*ctx, *ivf, *points, n_neighbors, n_probe, use_router, confidence, indices, distances, info);
+    });
+}
+
+int ky_router_grad(ky_ctx *ctx, uint32_t d_in, uint32_t hidden, uint32_t n_out, const float *params, const float *x,
+                   uint32_t batch, const uint32_t *t_idx, const float *t_w, uint32_t tk, float *grads, float *loss) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(params, "params");
+        need(x, "x");
+        need(t_idx, "t_idx");
+        need(t_w, "t_w");
+        need(grads, "grads");
+        KY_CHECK(batch >= 1 && tk >= 1 && tk <= 64 && d_in >= 1 && hidden >= 1 && n_out >= 1, KY_E215,
+                 "router: sizes out of range");
+        use_device(*ctx);
+        hipStream_t st = ctx->stream;
+        Router R;
+        R.init(*ctx, d_in, hidden, n_out, 0);
+        KY_HIP(hipMemcpyAsync(R.params.ptr, params, sizeof(float) * R.n_params, hipMemcpyHostToDevice, st));
+        DevBuf<float> xb((size_t)batch * d_in), h1((size_t)batch * hidden), h2((size_t)batch * hidden),
+            lg((size_t)batch * n_out), dh1((size_t)batch * hidden), dh2((size_t)batch * hidden), ls(batch), tw((size_t)batch * tk);
+        DevBuf<uint32_t> ti((size_t)batch * tk);
+        KY_HIP(hipMemcpyAsync(xb.ptr, x, sizeof(float) * batch * d_in, hipMemcpyHostToDevice, st));
+        KY_HIP(hipMemcpyAsync(ti.ptr, t_idx, sizeof(uint32_t) * batch * tk, hipMemcpyHostToDevice, st));
+        KY_HIP(hipMemcpyAsync(tw.ptr, t_w, sizeof(float) * batch * tk, hipMemcpyHostToDevice, st));
+        R.forward(st, xb.ptr, batch, h1.ptr, h2.ptr, lg.ptr);
+        softmax_xent(st, lg.ptr, batch, (int)n_out, ti.ptr, tw.ptr, (int)tk, 1.0f / (float)batch, ls.ptr);
+        R.backward(st, xb.ptr, batch, h1.ptr, h2.ptr, lg.ptr, dh2.ptr, dh1.ptr);
+        KY_HIP(hipMemcpyAsync(grads, R.grads.ptr, sizeof(float) * R.n_params, hipMemcpyDeviceToHost, st));
+        std::vector<float> hl(batch);
+        KY_HIP(hipMemcpyAsync(hl.data(), ls.ptr, sizeof(float) * batch, hipMemcpyDeviceToHost, st));
+        KY_HIP(hipStreamSynchronize(st));
+        if (loss) {
+            double s = 0.0;
+            for (float v : hl) s += v;
+            *loss = (float)(s / batch);
+        }
+    });
+}
+
+int ky_adam_step(ky_ctx *ctx, float *w, const float *g, float *m, float *v, uint64_t count, float lr, uint32_t step) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(w, "w");
+        need(g, "g");
+        need(m, "m");
+        need(v, "v");
+        KY_CHECK(step >= 1, KY_E215, "adam: step counts from 1");
+        use_device(*ctx);
+        hipStream_t st = ctx->stream;
+        DevBuf<float> dw(count), dg(count), dm(count), dv(count);
+        KY_HIP(hipMemcpyAsync(dw.ptr, w, sizeof(float) * count, hipMemcpyHostToDevice, st));
+        KY_HIP(hipMemcpyAsync(dg.ptr, g, sizeof(float) * count, hipMemcpyHostToDevice, st));
+        KY_HIP(hipMemcpyAsync(dm.ptr, m, sizeof(float) * count, hipMemcpyHostToDevice, st));
+        KY_HIP(hipMemcpyAsync(dv.ptr, v, sizeof(float) * count, hipMemcpyHostToDevice, st));
+        adam(st, dw.ptr, dg.ptr, dm.ptr, dv.ptr, (int64_t)count, lr, (int)step);
+        KY_HIP(hipMemcpyAsync(w, dw.ptr, sizeof(float) * count, hipMemcpyDeviceToHost, st));
+        KY_HIP(hipMemcpyAsync(m, dm.ptr, sizeof(float) * count, hipMemcpyDeviceToHost, st));
+        KY_HIP(hipMemcpyAsync(v, dv.ptr, sizeof(float) * count, hipMemcpyDeviceToHost, st));
+        KY_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -40,6 +40,21 @@ static void parallel_for(size_t n, F &&f) {
     for (auto &th : threads) th.join();
 }
 
+// Dot product in four interleaved partial sums (a fixed order: the same bits on every
+// run), which keeps four additions in flight.
+static inline double dot4(const double *a, const double *b, size_t n) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    size_t i = 0;
+    for (; i + 4 <= n; i += 4) {
+        s0 += a[i] * b[i];
+        s1 += a[i + 1] * b[i + 1];
+        s2 += a[i + 2] * b[i + 2];
+        s3 += a[i + 3] * b[i + 3];
+    }
+    for (; i < n; i++) s0 += a[i] * b[i];
+    return (s0 + s2) + (s1 + s3);
+}
+
 Mat Mat::beside(const Mat &right) const {
     Mat out(rows, cols + right.cols);
     for (size_t r = 0; r < rows; r++) {
@@ -101,55 +116,53 @@ static double frobenius(const Mat &m) {
 }
 
 // Thin Householder QR of a (m x n, m >= n): returns Q (m x n, orthonormal columns)
-// and R (n x n, upper triangular) with a = Q R.
+// and R (n x n, upper triangular) with a = Q R. Works on the columns stored
+// contiguously (the transpose), so every reflection streams memory.
 static void householder_qr(const Mat &a, Mat &q, Mat &r) {
-    size_t m = a.rows, n = a.cols;
-    Mat w = a;                      // reflected in place
-    std::vector<std::vector<double>> vs(n);
+    const size_t m = a.rows, n = a.cols;
+    std::vector<double> col(n * m);  // col[c * m + i] = a(i, c)
+    for (size_t i = 0; i < m; i++)
+        for (size_t c = 0; c < n; c++) col[c * m + i] = a.at(i, c);
     std::vector<double> betas(n, 0.0);
+    std::vector<double> vs(n * m, 0.0);  // reflection vectors, v_j in vs[j * m + j ..]
     r = Mat(n, n);
     for (size_t j = 0; j < n; j++) {
-        double norm2 = 0.0;
-        for (size_t i = j; i < m; i++) norm2 += w.at(i, j) * w.at(i, j);
-        double norm = std::sqrt(norm2);
-        std::vector<double> v(m - j);
-        for (size_t i = j; i < m; i++) v[i - j] = w.at(i, j);
-        if (norm == 0.0) {
-            vs[j] = std::move(v);
-            betas[j] = 0.0;
-            continue;
-        }
-        double alpha = v[0] > 0 ? -norm : norm;
-        v[0] -= alpha;
-        double vtv = 0.0;
-        for (double x : v) vtv += x * x;
-        double beta = vtv > 0 ? 2.0 / vtv : 0.0;
-        // Apply (I - beta v v^T) to columns j..n-1 of w (rows j..m-1).
-        for (size_t c = j; c < n; c++) {
-            double s = 0.0;
-            for (size_t i = j; i < m; i++) s += v[i - j] * w.at(i, c);
-            s *= beta;
-            for (size_t i = j; i < m; i++) w.at(i, c) -= s * v[i - j];
-        }
-        vs[j] = std::move(v);
+        double *cj = &col[j * m];
+        const double norm = std::sqrt(dot4(cj + j, cj + j, m - j));
+        double *v = &vs[j * m];
+        for (size_t i = j; i < m; i++) v[i] = cj[i];
+        if (norm == 0.0) continue;
+        const double alpha = v[j] > 0 ? -norm : norm;
+        v[j] -= alpha;
+        const double vtv = dot4(v + j, v + j, m - j);
+        const double beta = vtv > 0 ? 2.0 / vtv : 0.0;
         betas[j] = beta;
+        // Apply (I - beta v v^T) to columns j..n-1 (rows j..m-1).
+        for (size_t c = j; c < n; c++) {
+            double *cc = &col[c * m];
+            const double sdot = beta * dot4(v + j, cc + j, m - j);
+            for (size_t i = j; i < m; i++) cc[i] -= sdot * v[i];
+        }
     }
     for (size_t i = 0; i < n; i++)
-        for (size_t c = i; c < n; c++) r.at(i, c) = w.at(i, c);
-    // Q = H_0 H_1 ... H_{n-1} [I; 0], applied from the last reflection back.
-    q = Mat(m, n);
-    for (size_t i = 0; i < n; i++) q.at(i, i) = 1.0;
+        for (size_t c = i; c < n; c++) r.at(i, c) = col[c * m + i];
+    // Q = H_0 H_1 ... H_{n-1} [I; 0], applied from the last reflection back, on
+    // contiguous columns.
+    std::vector<double> qc(n * m, 0.0);
+    for (size_t i = 0; i < n; i++) qc[i * m + i] = 1.0;
     for (size_t jj = n; jj-- > 0;) {
-        const std::vector<double> &v = vs[jj];
-        double beta = betas[jj];
+        const double beta = betas[jj];
         if (beta == 0.0) continue;
+        const double *v = &vs[jj * m];
         for (size_t c = 0; c < n; c++) {
-            double s = 0.0;
-            for (size_t i = jj; i < m; i++) s += v[i - jj] * q.at(i, c);
-            s *= beta;
-            for (size_t i = jj; i < m; i++) q.at(i, c) -= s * v[i - jj];
+            double *cc = &qc[c * m];
+            const double sdot = beta * dot4(v + jj, cc + jj, m - jj);
+            for (size_t i = jj; i < m; i++) cc[i] -= sdot * v[i];
         }
     }
+    q = Mat(m, n);
+    for (size_t i = 0; i < m; i++)
+        for (size_t c = 0; c < n; c++) q.at(i, c) = qc[c * m + i];
 }
 
 // One-sided Jacobi SVD of a square a (n x n): a = U diag(s) V^T with s descending.
@@ -160,19 +173,17 @@ static void jacobi_svd(const Mat &a, Mat &u, std::vector<double> &s) {
     std::vector<std::vector<double>> col(n, std::vector<double>(n));
     for (size_t i = 0; i < n; i++)
         for (size_t j = 0; j < n; j++) col[j][i] = a.at(i, j);
-    const double eps = 2.220446049250313e-16;
+    // Rotate while a pair's cosine exceeds n eps (dgesvj's sqrt(m) eps criterion, with
+    // n for m): below it the columns are orthogonal to working precision.
+    const double tol = (double)n * 2.220446049250313e-16;
     for (int sweep = 0; sweep < 80; sweep++) {
         bool rotated = false;
         for (size_t p = 0; p + 1 < n; p++) {
             for (size_t qq = p + 1; qq < n; qq++) {
-                double alpha = 0, beta = 0, gamma = 0;
                 const std::vector<double> &cp = col[p], &cq = col[qq];
-                for (size_t i = 0; i < n; i++) {
-                    alpha += cp[i] * cp[i];
-                    beta += cq[i] * cq[i];
-                    gamma += cp[i] * cq[i];
-                }
-                if (gamma == 0.0 || std::fabs(gamma) <= eps * std::sqrt(alpha * beta)) continue;
+                const double alpha = dot4(cp.data(), cp.data(), n), beta = dot4(cq.data(), cq.data(), n),
+                             gamma = dot4(cp.data(), cq.data(), n);
+                if (gamma == 0.0 || std::fabs(gamma) <= tol * std::sqrt(alpha * beta)) continue;
                 rotated = true;
                 double zeta = (beta - alpha) / (2.0 * gamma);
                 double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
@@ -235,90 +246,64 @@ Mat extend_basis(const Mat &basis, const Mat &block) {
     return mul_nn(q, ukeep);
 }
 
-// Symmetric eigendecomposition: a (n x n) = V diag(d) V^T, d ascending.
+// Symmetric eigendecomposition: a (n x n) = V diag(d) V^T, d ascending. Householder
+// reduction to tridiagonal form on full row-major storage (a symmetric matrix-vector
+// product and a rank-2 update per column, both along contiguous rows), the
+// reflections accumulated into the eigenvector columns, then the implicit QL method.
 static void symmetric_eigen(const Mat &a, std::vector<double> &d, Mat &vout) {
-    size_t n = a.rows;
-    std::vector<std::vector<double>> V(n, std::vector<double>(n));
-    for (size_t i = 0; i < n; i++)
-        for (size_t j = 0; j < n; j++) V[i][j] = a.at(i, j);
+    const size_t n = a.rows;
     d.assign(n, 0.0);
     std::vector<double> e(n, 0.0);
     if (n == 0) { vout = Mat(0, 0); return; }
-    // Householder reduction to tridiagonal form.
-    for (size_t j = 0; j < n; j++) d[j] = V[n - 1][j];
-    for (size_t i = n - 1; i > 0; i--) {
-        double scale = 0.0, h = 0.0;
-        for (size_t k = 0; k < i; k++) scale += std::fabs(d[k]);
-        if (scale == 0.0) {
-            e[i] = d[i - 1];
-            for (size_t j = 0; j < i; j++) {
-                d[j] = V[i - 1][j];
-                V[i][j] = 0.0;
-                V[j][i] = 0.0;
-            }
-        } else {
-            for (size_t k = 0; k < i; k++) {
-                d[k] /= scale;
-                h += d[k] * d[k];
-            }
-            double f = d[i - 1];
-            double g = std::sqrt(h);
-            if (f > 0) g = -g;
-            e[i] = scale * g;
-            h = h - f * g;
-            d[i - 1] = f - g;
-            for (size_t j = 0; j < i; j++) e[j] = 0.0;
-            for (size_t j = 0; j < i; j++) {
-                f = d[j];
-                V[j][i] = f;
-                g = e[j] + V[j][j] * f;
-                for (size_t k = j + 1; k <= i - 1; k++) {
-                    g += V[k][j] * d[k];
-                    e[k] += V[k][j] * f;
-                }
-                e[j] = g;
-            }
-            f = 0.0;
-            for (size_t j = 0; j < i; j++) {
-                e[j] /= h;
-                f += e[j] * d[j];
-            }
-            double hh = f / (h + h);
-            for (size_t j = 0; j < i; j++) e[j] -= hh * d[j];
-            for (size_t j = 0; j < i; j++) {
-                f = d[j];
-                g = e[j];
-                for (size_t k = j; k <= i - 1; k++) V[k][j] -= (f * e[k] + g * d[k]);
-                d[j] = V[i - 1][j];
-                V[i][j] = 0.0;
-            }
+    Mat A = a;
+    std::vector<std::vector<double>> vs(n);
+    std::vector<double> betas(n, 0.0);
+    std::vector<double> p(n), w(n);
+    for (size_t j = 0; j + 2 < n; j++) {
+        const size_t m = n - j - 1;  // trailing size
+        std::vector<double> &v = vs[j];
+        v.assign(A.row(j) + j + 1, A.row(j) + n);  // column j below the diagonal (= row j by symmetry)
+        const double norm = std::sqrt(dot4(v.data(), v.data(), m));
+        if (norm == 0.0) {
+            e[j] = 0.0;
+            continue;
         }
-        d[i] = h;
-    }
-    // Accumulate the transformations.
-    for (size_t i = 0; i + 1 < n; i++) {
-        V[n - 1][i] = V[i][i];
-        V[i][i] = 1.0;
-        double h = d[i + 1];
-        if (h != 0.0) {
-            for (size_t k = 0; k <= i; k++) d[k] = V[k][i + 1] / h;
-            for (size_t j = 0; j <= i; j++) {
-                double g = 0.0;
-                for (size_t k = 0; k <= i; k++) g += V[k][i + 1] * V[k][j];
-                for (size_t k = 0; k <= i; k++) V[k][j] -= g * d[k];
-            }
+        const double alpha = v[0] > 0 ? -norm : norm;
+        v[0] -= alpha;
+        const double beta = 2.0 / dot4(v.data(), v.data(), m);
+        betas[j] = beta;
+        // p = beta A' v, K = beta/2 v.p, w = p - K v, A' -= v w^T + w v^T.
+        for (size_t r = 0; r < m; r++) p[r] = beta * dot4(A.row(j + 1 + r) + j + 1, v.data(), m);
+        const double K = 0.5 * beta * dot4(v.data(), p.data(), m);
+        for (size_t r = 0; r < m; r++) w[r] = p[r] - K * v[r];
+        for (size_t r = 0; r < m; r++) {
+            double *ar = A.row(j + 1 + r) + j + 1;
+            const double vr = v[r], wr = w[r];
+            for (size_t c = 0; c < m; c++) ar[c] -= vr * w[c] + wr * v[c];
         }
-        for (size_t k = 0; k <= i; k++) V[k][i + 1] = 0.0;
+        A.at(j + 1, j) = alpha;
+        A.at(j, j + 1) = alpha;
     }
-    for (size_t j = 0; j < n; j++) {
-        d[j] = V[n - 1][j];
-        V[n - 1][j] = 0.0;
-    }
-    V[n - 1][n - 1] = 1.0;
-    e[0] = 0.0;
-    // Implicit QL on the tridiagonal matrix.
-    for (size_t i = 1; i < n; i++) e[i - 1] = e[i];
+    for (size_t j = 0; j < n; j++) d[j] = A.at(j, j);
+    for (size_t j = 0; j + 1 < n; j++) e[j] = A.at(j + 1, j);
     e[n - 1] = 0.0;
+    // Eigenvector columns Z[c] (contiguous): start from Q = H_0 ... H_{n-3}, whose
+    // column c is Q e_c, applied from the last reflection back.
+    std::vector<std::vector<double>> Z(n, std::vector<double>(n, 0.0));
+    for (size_t c = 0; c < n; c++) Z[c][c] = 1.0;
+    for (size_t jj = n >= 2 ? n - 2 : 0; jj-- > 0;) {
+        if (betas[jj] == 0.0) continue;
+        const std::vector<double> &v = vs[jj];
+        const size_t m = n - jj - 1;
+        for (size_t c = 0; c < n; c++) {
+            double *z = Z[c].data() + jj + 1;
+            const double s = betas[jj] * dot4(v.data(), z, m);
+            if (s != 0.0)
+                for (size_t r = 0; r < m; r++) z[r] -= s * v[r];
+        }
+    }
+    // Implicit QL on the tridiagonal matrix (e[i] couples i and i + 1); the
+    // rotations act on the contiguous eigenvector columns Z.
     double f = 0.0, tst1 = 0.0;
     const double eps = 2.220446049250313e-16;
     for (size_t l = 0; l < n; l++) {
@@ -360,10 +345,11 @@ static void symmetric_eigen(const Mat &a, std::vector<double> &d, Mat &vout) {
                     c = p / r;
                     p = c * d[ii] - s * g;
                     d[ii + 1] = h + s * (c * g + s * d[ii]);
+                    double *z1 = Z[ii + 1].data(), *z0 = Z[ii].data();
                     for (size_t k = 0; k < n; k++) {
-                        h = V[k][ii + 1];
-                        V[k][ii + 1] = s * V[k][ii] + c * h;
-                        V[k][ii] = c * V[k][ii] - s * h;
+                        h = z1[k];
+                        z1[k] = s * z0[k] + c * h;
+                        z0[k] = c * z0[k] - s * h;
                     }
                 }
                 p = -s * s2 * c3 * el1 * e[l] / dl1;
@@ -381,7 +367,7 @@ static void symmetric_eigen(const Mat &a, std::vector<double> &d, Mat &vout) {
     vout = Mat(n, n);
     for (size_t k = 0; k < n; k++) {
         ds[k] = d[order[k]];
-        for (size_t i = 0; i < n; i++) vout.at(i, k) = V[i][order[k]];
+        for (size_t i = 0; i < n; i++) vout.at(i, k) = Z[order[k]][i];
     }
     d = ds;
 }

@@ -89,4 +89,28 @@ __device__ __forceinline__ void wave_bitonic(K *keys, uint32_t *rows, unsigned N
     }
 }
 
+// Compact query slot `ql`'s buffer (count entries) to its `keep` smallest (key, row)
+// pairs, sorted ascending; when it held at least `keep`, the keep-th pair becomes the
+// threshold. Called by the whole wavefront.
+template <typename K, int CAP>
+__device__ void compact(K *bkey, uint32_t *brow, uint32_t *bcnt, K *tkey, uint32_t *trow, unsigned keep) {
+    const unsigned lane = lane_id();
+    const unsigned cnt = *bcnt;
+    for (unsigned i = lane; i < CAP; i += 64)
+        if (i >= cnt) {
+            bkey[i] = ~K(0);
+            brow[i] = ~0u;
+        }
+    wave_sync();
+    wave_bitonic<K>(bkey, brow, CAP);
+    if (lane == 0) {
+        if (cnt >= keep) {
+            *tkey = bkey[keep - 1];
+            *trow = brow[keep - 1];
+        }
+        *bcnt = min(cnt, keep);
+    }
+    wave_sync();
+}
+
 }  // namespace ky

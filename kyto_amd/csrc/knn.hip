@@ -28,6 +28,7 @@
 #include "common.hpp"
 #include "device.hpp"
 #include "knn.hpp"
+#include "pca.hpp"
 
 namespace ky {
 
@@ -102,30 +103,6 @@ __global__ void pack_columns(const float *__restrict__ x, uint64_t n, uint32_t d
 }
 
 // ------------------------------------------------------------- the point scan ----
-
-// Compact query slot `ql`'s buffer (count entries) to its `keep` smallest (key, row)
-// pairs, sorted ascending; when it held at least `keep`, the keep-th pair becomes the
-// threshold. Called by the whole wavefront.
-template <typename K, int CAP>
-__device__ void compact(K *bkey, uint32_t *brow, uint32_t *bcnt, K *tkey, uint32_t *trow, unsigned keep) {
-    const unsigned lane = lane_id();
-    const unsigned cnt = *bcnt;
-    for (unsigned i = lane; i < CAP; i += 64)
-        if (i >= cnt) {
-            bkey[i] = ~K(0);
-            brow[i] = ~0u;
-        }
-    wave_sync();
-    wave_bitonic<K>(bkey, brow, CAP);
-    if (lane == 0) {
-        if (cnt >= keep) {
-            *tkey = bkey[keep - 1];
-            *trow = brow[keep - 1];
-        }
-        *bcnt = min(cnt, keep);
-    }
-    wave_sync();
-}
 
 // EXACT = false: approximate f32 keys on centred rows `a` (dp floats a row) with
 // offsets; keys kept as order_f32 bits. EXACT = true: kyto's exact f64 keys on the
@@ -268,13 +245,17 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
 // ------------------------------------------------------------------- re-rank ----
 
 // One wavefront per query: exact keys of its candidates, the k smallest by (key, row),
-// and the certificate. floor_mode 0: f32 keys of the centred expanded form, floor =
-// key - slack[q] (neighbors_exact.rs:76-89); 1: BF16 keys, floor from the per-row
-// bound (knn_bf16.hip). Lanes hold one candidate each (m <= 64).
+// and the certificate. Eight lanes share a candidate: lane t of the group sums kyto's
+// lane t (components t, t + 8, ...) in order, and the fixed tree combines the eight
+// through shuffles, so the key has the bits of Kernels::squared_distances while the
+// group reads the candidate's row in contiguous pieces. floor_mode 0: f32 keys of
+// the centred expanded form, floor = key - slack[q] (neighbors_exact.rs:76-89);
+// 1: BF16 keys (knn_bf16.hip). m <= 64.
 __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uint64_t n, uint32_t d,
                                               const uint32_t *__restrict__ cand_rows,
                                               const float *__restrict__ cand_keys,
-                                              const uint32_t *__restrict__ cand_count, uint64_t q_first,
+                                              const uint32_t *__restrict__ cand_count,
+                                              const uint32_t *__restrict__ q_rows, uint64_t q_first,
                                               uint64_t q_count, uint32_t m, uint32_t k,
                                               const double *__restrict__ slack, const double *__restrict__ bf_err,
                                               double bf_err_max, double bf_acc_rel, int floor_mode,
@@ -285,17 +266,31 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
     const unsigned wave = threadIdx.x / 64, lane = lane_id();
     const uint64_t i = (uint64_t)blockIdx.x * 4 + wave;
     if (i >= q_count) return;
-    const uint64_t q = q_first + i;
+    const uint64_t q = q_rows ? (uint64_t)q_rows[i] : q_first + i;
     const unsigned cnt = cand_count[i];
-    unsigned long long kb = ~0ull;
-    uint32_t row = 0xFFFFFFFFu;
-    if (lane < cnt) {
-        row = cand_rows[i * m + lane];
-        double key = exact_sq_distance(pts + q * d, pts + (uint64_t)row * d, d);
-        kb = (unsigned long long)__double_as_longlong(key);
+    const unsigned t = lane & 7, g = lane >> 3;  // kyto lane t, candidate group g
+    const float *qrow = pts + q * d;
+    for (unsigned c0 = 0; c0 < 64; c0 += 8) {
+        const unsigned c = c0 + g;
+        uint32_t row = 0xFFFFFFFFu;
+        double part = 0.0;
+        if (c < cnt) {
+            row = cand_rows[i * m + c];
+            const float *prow = pts + (uint64_t)row * d;
+            for (unsigned j = t; j < d; j += 8) {
+                double diff = (double)qrow[j] - (double)prow[j];
+                part += diff * diff;
+            }
+        }
+        // ((l0 + l4) + (l2 + l6)) + ((l1 + l5) + (l3 + l7)) over the group's lanes.
+        double h = part + __shfl_xor(part, 4, 8);       // t < 4: l_t + l_{t+4}
+        double qv = h + __shfl_xor(h, 2, 8);            // t in {0,1}: h_t + h_{t+2}
+        double key = qv + __shfl_xor(qv, 1, 8);         // t = 0: q0 + q1
+        if (t == 0) {
+            skey[wave][c] = c < cnt ? (unsigned long long)__double_as_longlong(key) : ~0ull;
+            srow[wave][c] = row;
+        }
     }
-    skey[wave][lane] = kb;
-    srow[wave][lane] = row;
     wave_sync();
     wave_bitonic<unsigned long long>(skey[wave], srow[wave], 64);
     bool cert = false;
@@ -412,12 +407,14 @@ void knn_prepare(Context &ctx, const float *x, uint64_t n, uint32_t d, KnnPrep &
     prep.n = n;
     prep.d = d;
     prep.dp = padded_width(d);
-    const uint32_t chunk = 8192;
+    const uint32_t chunk = 256;
     const uint32_t n_chunks = div_up(n, chunk);
-    DevBuf<double> partial((size_t)n_chunks * d);
+    DevBuf<double> partial((size_t)n_chunks * d * 2 + 2 * d);
     prep.mean.alloc(d);
     embed_col_sums<<<n_chunks, 64, 0, st>>>(x, n, d, d, chunk, partial.ptr);
-    embed_reduce<<<1, 64, 0, st>>>(partial.ptr, d, n_chunks, 1.0 / (double)n, prep.mean.ptr);
+    double *total = partial.ptr + (size_t)n_chunks * d * 2 + d;
+    ordered_sum(st, partial.ptr, n_chunks, d, partial.ptr + (size_t)n_chunks * d, total);
+    embed_reduce<<<1, 64, 0, st>>>(total, d, 1, 1.0 / (double)n, prep.mean.ptr);
     prep.cent.alloc(n * prep.dp);
     prep.offs.alloc(n);
     prep.sqn.alloc(n);
@@ -467,13 +464,13 @@ void knn_candidates(Context &ctx, const KnnPrep &prep, uint64_t q_first, uint64_
                             q_first, q_count, m, d_rows, d_keys, d_count);
 }
 
-void knn_rerank(Context &ctx, const float *pts, const KnnPrep &prep, uint64_t q_first, uint64_t q_count, uint32_t m,
-                uint32_t k, const uint32_t *cand_rows, const float *cand_keys, const uint32_t *cand_count,
-                int floor_mode, const double *bf_err, double bf_err_max, double bf_acc_rel, double *d_keys,
-                uint32_t *d_rows, uint8_t *d_cert) {
-    rerank<<<div_up(q_count, 4), 256, 0, ctx.stream>>>(pts, prep.n, prep.d, cand_rows, cand_keys, cand_count, q_first,
-                                                       q_count, m, k, prep.slack.ptr, bf_err, bf_err_max, bf_acc_rel,
-                                                       floor_mode, d_keys, d_rows, d_cert);
+void knn_rerank(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t q_first,
+                uint64_t q_count, uint32_t m, uint32_t k, const uint32_t *cand_rows, const float *cand_keys,
+                const uint32_t *cand_count, int floor_mode, const double *bf_err, double bf_err_max,
+                double bf_acc_rel, double *d_keys, uint32_t *d_rows, uint8_t *d_cert) {
+    rerank<<<div_up(q_count, 4), 256, 0, ctx.stream>>>(pts, prep.n, prep.d, cand_rows, cand_keys, cand_count, q_rows,
+                                                       q_first, q_count, m, k, prep.slack.ptr, bf_err, bf_err_max,
+                                                       bf_acc_rel, floor_mode, d_keys, d_rows, d_cert);
     KY_HIP(hipGetLastError());
 }
 
@@ -512,11 +509,12 @@ struct EventTimer {
     }
 };
 
-void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, uint32_t m, uint32_t *d_rows,
-                     float *d_keys, uint32_t *d_count, DevBuf<double> &err, double &err_max, double &acc_rel);
+void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t nq,
+                     uint32_t m, uint32_t *d_rows, float *d_keys, uint32_t *d_count, DevBuf<double> &err,
+                     double &err_max, double &acc_rel);
 
 // The checks of neighbors::search (neighbors.rs:226-312) that precede the search.
-static uint32_t check_search(const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric) {
+uint32_t check_search(const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric) {
     KY_CHECK(points.cols >= 1 && points.rows >= 1, KY_E215, "neighbors: an embedding without rows or components");
     uint32_t used = n_components == 0 ? points.cols : n_components;
     KY_CHECK(used >= 1 && used <= points.cols, KY_E215,
@@ -534,6 +532,75 @@ static uint32_t check_search(const Dense &points, uint32_t n_components, uint32_
     return used;
 }
 
+// Exact neighbours (kyto's (key, row) order) of the rows q_rows[0..nq) (every row
+// when q_rows is null, nq = n) of the packed n x d embedding: candidates at
+// `precision`, exact re-rank and certificate, exact search again of the rows the
+// certificate does not cover. keys / rows are n x k, filled at the queried rows.
+void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32_t k, int precision,
+                  const uint32_t *q_rows, uint64_t nq, double *keys, uint32_t *rows, ky_knn_info *info) {
+    hipStream_t st = ctx.stream;
+    EventTimer timer(st);
+    KnnPrep prep;
+    timer.start();
+    knn_prepare(ctx, pts, n, d, prep);
+    KY_CHECK(prep.finite, KY_E215, "neighbors: the embedding holds a value that is not finite");
+    DevBuf<uint8_t> cert(n);
+    uint32_t m = 0;
+    double search_ms = 0, rerank_ms = 0, fallback_ms = 0;
+    std::vector<uint32_t> list;
+    std::vector<uint32_t> hq;
+    if (q_rows) {
+        hq.resize(nq);
+        KY_HIP(hipMemcpyAsync(hq.data(), q_rows, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, st));
+    }
+    // The BF16 kernel's augmented rows hold d + 6 <= 64 components; wider embeddings
+    // take their candidates from the FP32 kernel (the graph is the same).
+    if (precision == KY_PRECISION_BF16 && d > 58) precision = KY_PRECISION_FP32;
+    if (prep.filter_ok) {
+        m = candidates_for(k, n, precision);
+        DevBuf<uint32_t> c_rows(nq * m), c_count(nq);
+        DevBuf<float> c_keys(nq * m);
+        DevBuf<double> err;
+        double err_max = 0.0, acc_rel = 0.0;
+        if (precision == KY_PRECISION_FP32) {
+            KY_CHECK(q_rows == nullptr, KY_E501, "neighbors: FP32 candidates for a row subset");
+            knn_candidates(ctx, prep, 0, n, m, c_rows.ptr, c_keys.ptr, c_count.ptr);
+        } else {
+            bf16_candidates(ctx, pts, prep, q_rows, nq, m, c_rows.ptr, c_keys.ptr, c_count.ptr, err, err_max, acc_rel);
+        }
+        search_ms = timer.stop();
+        timer.start();
+        knn_rerank(ctx, pts, prep, q_rows, 0, nq, m, k, c_rows.ptr, c_keys.ptr, c_count.ptr,
+                   precision == KY_PRECISION_FP32 ? 0 : 1, err.ptr, err_max, acc_rel, keys, rows, cert.ptr);
+        std::vector<uint8_t> hcert(n);
+        KY_HIP(hipMemcpyAsync(hcert.data(), cert.ptr, n, hipMemcpyDeviceToHost, st));
+        rerank_ms = timer.stop();
+        for (uint64_t i = 0; i < nq; i++) {
+            uint32_t q = q_rows ? hq[i] : (uint32_t)i;
+            if (!hcert[q]) list.push_back(q);
+        }
+    } else {
+        search_ms = timer.stop();
+        for (uint64_t i = 0; i < nq; i++) list.push_back(q_rows ? hq[i] : (uint32_t)i);
+    }
+    timer.start();
+    if (!list.empty()) {
+        DevBuf<uint32_t> d_list(list.size());
+        KY_HIP(hipMemcpyAsync(d_list.ptr, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, st));
+        knn_exact_rows(ctx, pts, n, d, k, d_list.ptr, list.size(), keys, rows);
+        KY_HIP(hipStreamSynchronize(st));
+    }
+    fallback_ms = timer.stop();
+    if (info) {
+        info->candidates = m;
+        info->fallback = list.size();
+        info->certified = nq - list.size();
+        info->search_ms = search_ms;
+        info->rerank_ms = rerank_ms;
+        info->fallback_ms = fallback_ms;
+    }
+}
+
 void neighbors_run(Context &ctx, const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric,
                    int precision, uint32_t *h_idx, float *h_dist, ky_knn_info *info) {
     const double t_start = now_ms();
@@ -549,67 +616,16 @@ void neighbors_run(Context &ctx, const Dense &points, uint32_t n_components, uin
         knn_pack(ctx, points.data.ptr, n, d, points.cols, packed.ptr);
         pts = packed.ptr;
     }
-    EventTimer timer(st);
-    KnnPrep prep;
-    timer.start();
-    knn_prepare(ctx, pts, n, d, prep);
-    KY_CHECK(prep.finite, KY_E215, "neighbors: the embedding holds a value that is not finite");
     DevBuf<double> keys(n * k);
     DevBuf<uint32_t> rows(n * k);
-    DevBuf<uint8_t> cert(n);
-    uint32_t m = 0;
-    double search_ms = 0, rerank_ms = 0, fallback_ms = 0;
-    std::vector<uint32_t> list;
-    // The BF16 kernel's augmented rows hold d + 6 <= 64 components; wider embeddings
-    // take their candidates from the FP32 kernel (the graph is the same).
-    if (precision == KY_PRECISION_BF16 && d > 58) precision = KY_PRECISION_FP32;
-    if (prep.filter_ok) {
-        m = candidates_for(k, n, precision);
-        DevBuf<uint32_t> c_rows(n * m), c_count(n);
-        DevBuf<float> c_keys(n * m);
-        DevBuf<double> err;
-        double err_max = 0.0, acc_rel = 0.0;
-        if (precision == KY_PRECISION_FP32)
-            knn_candidates(ctx, prep, 0, n, m, c_rows.ptr, c_keys.ptr, c_count.ptr);
-        else
-            bf16_candidates(ctx, pts, prep, m, c_rows.ptr, c_keys.ptr, c_count.ptr, err, err_max, acc_rel);
-        search_ms = timer.stop();
-        timer.start();
-        knn_rerank(ctx, pts, prep, 0, n, m, k, c_rows.ptr, c_keys.ptr, c_count.ptr,
-                   precision == KY_PRECISION_FP32 ? 0 : 1, err.ptr, err_max, acc_rel, keys.ptr, rows.ptr, cert.ptr);
-        std::vector<uint8_t> hcert(n);
-        KY_HIP(hipMemcpyAsync(hcert.data(), cert.ptr, n, hipMemcpyDeviceToHost, st));
-        rerank_ms = timer.stop();
-        for (uint64_t q = 0; q < n; q++)
-            if (!hcert[q]) list.push_back((uint32_t)q);
-    } else {
-        search_ms = timer.stop();
-        list.resize(n);
-        for (uint64_t q = 0; q < n; q++) list[q] = (uint32_t)q;
-    }
-    timer.start();
-    if (!list.empty()) {
-        DevBuf<uint32_t> d_list(list.size());
-        KY_HIP(hipMemcpyAsync(d_list.ptr, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, st));
-        knn_exact_rows(ctx, pts, n, d, k, d_list.ptr, list.size(), keys.ptr, rows.ptr);
-        KY_HIP(hipStreamSynchronize(st));
-    }
-    fallback_ms = timer.stop();
+    exact_search(ctx, pts, n, d, k, precision, nullptr, n, keys.ptr, rows.ptr, info);
     DevBuf<uint32_t> g_idx(n * k);
     DevBuf<float> g_dist(n * k);
     knn_graph(ctx, keys.ptr, rows.ptr, n, k, g_idx.ptr, g_dist.ptr);
     KY_HIP(hipMemcpyAsync(h_idx, g_idx.ptr, sizeof(uint32_t) * n * k, hipMemcpyDeviceToHost, st));
     KY_HIP(hipMemcpyAsync(h_dist, g_dist.ptr, sizeof(float) * n * k, hipMemcpyDeviceToHost, st));
     KY_HIP(hipStreamSynchronize(st));
-    if (info) {
-        info->candidates = m;
-        info->fallback = list.size();
-        info->certified = n - list.size();
-        info->search_ms = search_ms;
-        info->rerank_ms = rerank_ms;
-        info->fallback_ms = fallback_ms;
-        info->total_ms = now_ms() - t_start;
-    }
+    if (info) info->total_ms = now_ms() - t_start;
 }
 
 void nearest_keys_run(Context &ctx, const Dense &points, uint32_t n_components, uint64_t first, uint64_t count,

@@ -315,16 +315,32 @@ __global__ void reduce_chunks(const double *__restrict__ partial, uint64_t strid
     out[i] = s;
 }
 
-// chunk_sums[c * w + j] = sum over the chunk's rows (in order) of f64(W[r, j]).
+// out[g * stride + i] = sum over parts c in [g * group, (g + 1) * group) (in order)
+// of partial[c * stride + i]: one level of a fixed reduction tree.
+__global__ void reduce_groups(const double *__restrict__ partial, uint64_t stride, uint32_t n_parts, uint32_t group,
+                              double *__restrict__ out) {
+    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint32_t n_groups = (n_parts + group - 1) / group;
+    if (t >= stride * n_groups) return;
+    const uint64_t g = t / stride, i = t % stride;
+    const uint32_t c1 = min<uint32_t>((uint32_t)(g + 1) * group, n_parts);
+    double s = 0.0;
+    for (uint32_t c = (uint32_t)g * group; c < c1; c++) s += partial[(size_t)c * stride + i];
+    out[g * stride + i] = s;
+}
+
+// seg_sums[s * w + j] = sum over the rows of segment s (SEG rows, in order) of
+// f64(W[r, j]); the segments are then added in order (reduce_chunks).
+constexpr uint32_t SEG = 256;
 __global__ void column_sums(const float *__restrict__ W, uint64_t n_rows, uint32_t w, uint32_t ldw,
-                            uint32_t chunk_rows, double *__restrict__ chunk_sums) {
-    const unsigned c = blockIdx.x;
+                            double *__restrict__ seg_sums) {
+    const uint64_t s = blockIdx.x;
     const unsigned j = threadIdx.x;
     if (j >= w) return;
-    const uint64_t r0 = (uint64_t)c * chunk_rows, r1 = min(r0 + chunk_rows, n_rows);
-    double s = 0.0;
-    for (uint64_t r = r0; r < r1; r++) s += (double)W[r * ldw + j];
-    chunk_sums[(size_t)c * w + j] = s;
+    const uint64_t r0 = s * SEG, r1 = min(r0 + SEG, n_rows);
+    double acc = 0.0;
+    for (uint64_t r = r0; r < r1; r++) acc += (double)W[r * ldw + j];
+    seg_sums[s * w + j] = acc;
 }
 
 // Per (chunk, gene): the centred squares sum_e (f64(v_e) - mu_g)^2 over the chunk's
@@ -402,13 +418,30 @@ void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const 
     KY_HIP(hipStreamSynchronize(st));
 }
 
+// The sum of n_parts arrays of `stride` values (part c at partial + c * stride) into
+// out, by a fixed tree of groups of 64 parts (the same bits on every run). `tmp`
+// holds at least stride * ceil(n_parts / 64) values.
+void ordered_sum(hipStream_t st, const double *partial, uint32_t n_parts, uint64_t stride, double *tmp, double *out) {
+    const uint32_t group = 64;
+    while (n_parts > group) {
+        const uint32_t n_groups = (n_parts + group - 1) / group;
+        reduce_groups<<<div_up(stride * n_groups, 256), 256, 0, st>>>(partial, stride, n_parts, group, tmp);
+        KY_HIP(hipGetLastError());
+        partial = tmp;
+        tmp = const_cast<double *>(partial) + stride * n_groups;  // the next level's output follows
+        n_parts = n_groups;
+    }
+    reduce_chunks<<<div_up(stride, 256), 256, 0, st>>>(partial, stride, n_parts, out);
+    KY_HIP(hipGetLastError());
+}
+
 // W (n x w floats, row stride ldw) is in ctx.wbuf. Computes X^T W into `product`
 // (host, genes x w) and optionally the column sums of W into `sums` (host, w).
 static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *product, double *sums) {
     Context &ctx = *csr.ctx;
     hipStream_t st = ctx.stream;
     const size_t stride = (size_t)csr.n_cols * w;
-    ctx.partials.reserve(stride * csr.n_chunks);
+    ctx.partials.reserve(stride * csr.n_chunks * 2);
     ctx.reduced.reserve(stride + w);
     const uint32_t gpb = 16;
     const unsigned groups = div_up(csr.n_cols, gpb);
@@ -427,15 +460,18 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
                                                   csr.n_chunks, csr.chunk_rows, ctx.wbuf.ptr, w, ldw, gpb,
                                                   ctx.partials.ptr);
     KY_HIP(hipGetLastError());
-    reduce_chunks<<<div_up(stride, 256), 256, 0, st>>>(ctx.partials.ptr, stride, csr.n_chunks, ctx.reduced.ptr);
-    KY_HIP(hipGetLastError());
+    ordered_sum(st, ctx.partials.ptr, csr.n_chunks, stride, ctx.partials.ptr + stride * csr.n_chunks,
+                ctx.reduced.ptr);
     KY_HIP(hipMemcpyAsync(product, ctx.reduced.ptr, sizeof(double) * stride, hipMemcpyDeviceToHost, st));
     if (sums) {
-        ctx.sums.reserve((size_t)csr.n_chunks * w + w);
-        column_sums<<<csr.n_chunks, 256, 0, st>>>(ctx.wbuf.ptr, csr.n_rows, w, ldw, csr.chunk_rows, ctx.sums.ptr);
+        const uint32_t n_seg = div_up(csr.n_rows, SEG);
+        // segment sums, then the tree's levels (each at most 1/64 of the one before), then the total
+        const size_t need = (size_t)n_seg * w * 2 + 2 * w;
+        ctx.sums.reserve(need);
+        column_sums<<<n_seg, 64 * div_up(w, 64), 0, st>>>(ctx.wbuf.ptr, csr.n_rows, w, ldw, ctx.sums.ptr);
         KY_HIP(hipGetLastError());
-        double *tot = ctx.sums.ptr + (size_t)csr.n_chunks * w;
-        reduce_chunks<<<div_up(w, 256), 256, 0, st>>>(ctx.sums.ptr, w, csr.n_chunks, tot);
+        double *tot = ctx.sums.ptr + need - w;
+        ordered_sum(st, ctx.sums.ptr, n_seg, w, ctx.sums.ptr + (size_t)n_seg * w, tot);
         KY_HIP(hipGetLastError());
         KY_HIP(hipMemcpyAsync(sums, tot, sizeof(double) * w, hipMemcpyDeviceToHost, st));
     }

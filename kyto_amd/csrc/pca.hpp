@@ -10,6 +10,7 @@ constexpr uint32_t CHUNK_ROWS = 8192;
 // estimate.rs:153: the sketch's extra columns.
 constexpr uint32_t PCA_OVERSAMPLES = 10;
 
+void ordered_sum(hipStream_t st, const double *partial, uint32_t n_parts, uint64_t stride, double *tmp, double *out);
 void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const float *values);
 void pca_sketch(Csr &csr, uint64_t seed, uint64_t cell_offset, uint32_t l, double *sketch, double *gene_sums,
                 double *omega_sums);

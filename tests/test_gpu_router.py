@@ -1,0 +1,114 @@
+"""The router's hand-written kernels against the FP64 CPU restatement (oracle/router.py):
+one step's gradients, Adam's closed form, and the learned IVF index end to end (its
+recall against the exact graph, and what the trained router gains over probing the
+nearest centroids)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import router as R
+from kyto_amd import _native as N
+from tests.helpers import clustered
+
+pytestmark = pytest.mark.gpu
+
+
+def _grad(device, params, x, t_idx, t_w, d_in, hidden, n_out):
+    g = np.empty_like(params)
+    loss = ctypes.c_float(0)
+    N.check(N.gpu_lib().ky_router_grad(device.handle, d_in, hidden, n_out, N.ptr(params, N.f32p), N.ptr(x, N.f32p),
+                                       x.shape[0], N.ptr(t_idx, N.u32p), N.ptr(t_w, N.f32p), t_idx.shape[1],
+                                       N.ptr(g, N.f32p), ctypes.byref(loss)))
+    return g, loss.value
+
+
+def test_router_gradients_match_fp64_oracle(device):
+    rng = np.random.default_rng(0)
+    d_in, hidden, n_out, B, tk = 20, 48, 70, 96, 5
+    n_params = d_in * hidden + hidden + hidden * hidden + hidden + hidden * n_out + n_out
+    params = (rng.standard_normal(n_params) * 0.2).astype(np.float32)
+    x = rng.standard_normal((B, d_in)).astype(np.float32)
+    t_idx = rng.integers(0, n_out, (B, tk)).astype(np.uint32)
+    t_w = np.full((B, tk), 1.0 / tk, np.float32)
+    g, loss = _grad(device, params, x, t_idx, t_w, d_in, hidden, n_out)
+    rg, rloss = R.grad(params, x, t_idx, t_w, d_in, hidden, n_out)
+    assert abs(loss - rloss) / abs(rloss) < 1e-4
+    # Each parameter block within 1e-3 of its largest gradient.
+    sizes = [d_in * hidden, hidden, hidden * hidden, hidden, hidden * n_out, n_out]
+    at = 0
+    for s in sizes:
+        a, b = g[at:at + s], rg[at:at + s]
+        assert np.abs(a - b).max() <= 1e-3 * np.abs(b).max() + 1e-7, (at, s)
+        at += s
+    # A planted mistake in the oracle (targets ignored) is caught by the same bar.
+    bad, _ = R.grad(params, x, t_idx, t_w * 0, d_in, hidden, n_out)
+    assert np.abs(g[-n_out:] - bad[-n_out:]).max() > 1e-2 * np.abs(rg[-n_out:]).max()
+    # Finite differences of the oracle's loss confirm its gradient direction.
+    e = np.zeros(n_params)
+    j = n_params - n_out - 3
+    e[j] = 1e-3
+    lp = R.grad(params.astype(np.float64) + e, x, t_idx, t_w, d_in, hidden, n_out)[1]
+    lm = R.grad(params.astype(np.float64) - e, x, t_idx, t_w, d_in, hidden, n_out)[1]
+    assert abs((lp - lm) / 2e-3 - rg[j]) < 1e-2 * max(abs(rg[j]), 1e-3)
+
+
+def test_adam_closed_form(device):
+    rng = np.random.default_rng(1)
+    n = 1000
+    w = rng.standard_normal(n).astype(np.float32)
+    g = rng.standard_normal(n).astype(np.float32)
+    m = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    w0 = w.copy()
+    N.check(N.gpu_lib().ky_adam_step(device.handle, N.ptr(w, N.f32p), N.ptr(g, N.f32p), N.ptr(m, N.f32p),
+                                     N.ptr(v, N.f32p), n, ctypes.c_float(1e-3), 1))
+    # Step 1 from zero moments: m_hat = g, v_hat = g^2, so w moves by -lr sign(g).
+    assert np.abs((w - w0) + 1e-3 * np.sign(g)).max() < 1e-6
+    # Two more steps against the FP64 restatement.
+    rw, rm, rv = w0.astype(np.float64), np.zeros(n), np.zeros(n)
+    rw, rm, rv = R.adam(rw, g.astype(np.float64), rm, rv, 1e-3, 1)
+    for step in (2, 3):
+        g2 = rng.standard_normal(n).astype(np.float32)
+        N.check(N.gpu_lib().ky_adam_step(device.handle, N.ptr(w, N.f32p), N.ptr(g2, N.f32p), N.ptr(m, N.f32p),
+                                         N.ptr(v, N.f32p), n, ctypes.c_float(1e-3), step))
+        rw, rm, rv = R.adam(rw, g2.astype(np.float64), rm, rv, 1e-3, step)
+    assert np.abs(w - rw).max() < 1e-6
+
+
+def _recall(graph_rows, exact_rows):
+    hits = sum(len(set(a.tolist()) & set(b.tolist())) for a, b in zip(graph_rows, exact_rows))
+    return hits / exact_rows.size
+
+
+def test_learned_ivf_recall_and_gain(device):
+    pts = clustered(11, 24000, 50, 30, 0.0)
+    m = device.upload_dense(pts)
+    try:
+        exact = device.neighbors(m, 15, precision="bf16").rows()
+        index = device.ivf_build(m, n_lists=128, kmeans_iters=10, seed=0, hidden=128, train_queries=12000,
+                                 batch=1024, epochs=20, lr=1e-3, label_k=14)
+        try:
+            assert index.info["final_loss"] > 0
+            g_router = index.search(m, 15, n_probe=8, use_router=True, confidence=0.0)
+            g_centroid = index.search(m, 15, n_probe=8, use_router=False)
+            r_router = _recall(g_router.rows(), exact)
+            r_centroid = _recall(g_centroid.rows(), exact)
+            print(f"recall@15 router {r_router:.4f} centroid probing {r_centroid:.4f} "
+                  f"label mass in top 8 {index.info['label_recall_top']:.4f}")
+            assert r_router >= 0.95
+            # Every returned neighbour carries its exact distance (kyto's key).
+            q = 123
+            row = g_router.rows()[q]
+            d = g_router.distances.reshape(-1, 14)[q]
+            for r_, dd in zip(row, d):
+                assert dd == np.float32(np.sqrt(oracle.squared_distance(pts[q], pts[r_])))
+            # Low-confidence rows go to the exact search: a threshold of 1.01 sends all.
+            g_all = index.search(m, 15, n_probe=8, use_router=True, confidence=1.01)
+            assert g_all.info["fallback"] == pts.shape[0]
+            assert np.array_equal(g_all.rows(), exact)
+        finally:
+            index.free()
+    finally:
+        m.free()
