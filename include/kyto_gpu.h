@@ -183,7 +183,11 @@ int ky_ivf_free(ky_ivf *ivf);
 
 typedef struct ky_ivf_search_info {
     uint64_t fallback;      /* queries below the confidence threshold searched exactly */
-    double route_ms, search_ms, total_ms;
+    double route_ms;        /* router (or centroid) scores and the top lists */
+    double search_ms;       /* exact keys over the probed lists */
+    double fallback_ms;     /* the plain method on the fallback queries */
+    double graph_ms;        /* graph assembly and download */
+    double total_ms;
 } ky_ivf_search_info;
 
 /* Approximate neighbours: each row's top `n_probe` lists (from the router when

@@ -62,7 +62,7 @@ class IvfInfo(ctypes.Structure):
 
 class IvfSearchInfo(ctypes.Structure):
     _fields_ = [("fallback", c_uint64), ("route_ms", c_double), ("search_ms", c_double),
-                ("total_ms", c_double)]
+                ("fallback_ms", c_double), ("graph_ms", c_double), ("total_ms", c_double)]
 
 
 # name -> (restype, argtypes); every function returns an int status.

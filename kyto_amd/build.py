@@ -30,7 +30,11 @@ HIP_FLAGS = [
 ]
 # MFMA kernels keep their accumulators in ordinary VGPRs (no accumulator-register
 # copies before the epilogue reads them).
-MFMA_FLAGS = {"knn_bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"], "router.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+# knn_bf16.hip compares keys that are never NaN: no NaN quieting before v_min_f32.
+MFMA_FLAGS = {
+    "knn_bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1", "-fno-honor-nans", "-mno-amdgpu-ieee"],
+    "router.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
 GPU_SOURCES = ["pca.hip", "knn.hip", "knn_bf16.hip", "router.hip", "ivf.hip", "sort.hip", "capi.cpp", "dense.cpp"]
 
 

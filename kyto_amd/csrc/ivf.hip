@@ -521,24 +521,31 @@ void ivf_search(Context &ctx, const ky_ivf &ivf, const Dense &points, uint32_t n
     KY_HIP(hipMemcpyAsync(hconf.data(), conf.ptr, sizeof(float) * n, hipMemcpyDeviceToHost, st));
     KY_HIP(hipMemcpyAsync(hcount.data(), count.ptr, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
     KY_HIP(hipStreamSynchronize(st));
+    const double scan_ms = timer.stop();
+    timer.start();
     std::vector<uint32_t> list;
     for (uint64_t q = 0; q < n; q++)
         if (hcount[q] < k || (use_router && hconf[q] < confidence)) list.push_back((uint32_t)q);
     if (!list.empty()) {
         DevBuf<uint32_t> d_list(list.size());
         KY_HIP(hipMemcpyAsync(d_list.ptr, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, st));
-        knn_exact_rows(ctx, x, n, d, k, d_list.ptr, list.size(), keys.ptr, rows.ptr);
+        // The plain method: BF16 candidates, exact re-rank and certificate.
+        exact_search(ctx, x, n, d, k, KY_PRECISION_BF16, d_list.ptr, list.size(), keys.ptr, rows.ptr, nullptr);
     }
+    const double fallback_ms = timer.stop();
+    timer.start();
     DevBuf<uint32_t> g_idx(n * k);
     DevBuf<float> g_dist(n * k);
     knn_graph(ctx, keys.ptr, rows.ptr, n, k, g_idx.ptr, g_dist.ptr);
     KY_HIP(hipMemcpyAsync(h_idx, g_idx.ptr, sizeof(uint32_t) * n * k, hipMemcpyDeviceToHost, st));
     KY_HIP(hipMemcpyAsync(h_dist, g_dist.ptr, sizeof(float) * n * k, hipMemcpyDeviceToHost, st));
-    const double search_ms = timer.stop();
+    const double graph_ms = timer.stop();
     if (info) {
         info->fallback = list.size();
         info->route_ms = route_ms;
-        info->search_ms = search_ms;
+        info->search_ms = scan_ms;
+        info->fallback_ms = fallback_ms;
+        info->graph_ms = graph_ms;
         info->total_ms = now_ms() - t_start;
     }
 }

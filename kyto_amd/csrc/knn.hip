@@ -255,6 +255,7 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
                                               const uint32_t *__restrict__ cand_rows,
                                               const float *__restrict__ cand_keys,
                                               const uint32_t *__restrict__ cand_count,
+                                              const float *__restrict__ cand_thr,
                                               const uint32_t *__restrict__ q_rows, uint64_t q_first,
                                               uint64_t q_count, uint32_t m, uint32_t k,
                                               const double *__restrict__ slack, const double *__restrict__ bf_err,
@@ -299,7 +300,10 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
         if ((uint64_t)cnt >= n - 1) {
             cert = true;  // every other row is a candidate
         } else {
-            const double far = (double)cand_keys[i * m + cnt - 1];
+            // Every row whose key is at most the scan's threshold is a candidate: with
+            // fewer than m candidates that threshold bounds the rest (cand_thr), else
+            // the m-th candidate's key does.
+            const double far = cand_thr && cnt < m ? (double)cand_thr[i] : (double)cand_keys[i * m + cnt - 1];
             double floor;
             if (floor_mode == 0) {
                 floor = far - slack[q];
@@ -466,9 +470,9 @@ void knn_candidates(Context &ctx, const KnnPrep &prep, uint64_t q_first, uint64_
 
 void knn_rerank(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t q_first,
                 uint64_t q_count, uint32_t m, uint32_t k, const uint32_t *cand_rows, const float *cand_keys,
-                const uint32_t *cand_count, int floor_mode, const double *bf_err, double bf_err_max,
-                double bf_acc_rel, double *d_keys, uint32_t *d_rows, uint8_t *d_cert) {
-    rerank<<<div_up(q_count, 4), 256, 0, ctx.stream>>>(pts, prep.n, prep.d, cand_rows, cand_keys, cand_count, q_rows,
+                const uint32_t *cand_count, const float *cand_thr, int floor_mode, const double *bf_err,
+                double bf_err_max, double bf_acc_rel, double *d_keys, uint32_t *d_rows, uint8_t *d_cert) {
+    rerank<<<div_up(q_count, 4), 256, 0, ctx.stream>>>(pts, prep.n, prep.d, cand_rows, cand_keys, cand_count, cand_thr, q_rows,
                                                        q_first, q_count, m, k, prep.slack.ptr, bf_err, bf_err_max,
                                                        bf_acc_rel, floor_mode, d_keys, d_rows, d_cert);
     KY_HIP(hipGetLastError());
@@ -510,8 +514,8 @@ struct EventTimer {
 };
 
 void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t nq,
-                     uint32_t m, uint32_t *d_rows, float *d_keys, uint32_t *d_count, DevBuf<double> &err,
-                     double &err_max, double &acc_rel);
+                     uint32_t m, uint32_t k, uint32_t *d_rows, float *d_keys, uint32_t *d_count, float *d_thr,
+                     DevBuf<double> &err, double &err_max, double &acc_rel);
 
 // The checks of neighbors::search (neighbors.rs:226-312) that precede the search.
 uint32_t check_search(const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric) {
@@ -559,18 +563,20 @@ void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32
     if (prep.filter_ok) {
         m = candidates_for(k, n, precision);
         DevBuf<uint32_t> c_rows(nq * m), c_count(nq);
-        DevBuf<float> c_keys(nq * m);
+        DevBuf<float> c_keys(nq * m), c_thr;
         DevBuf<double> err;
         double err_max = 0.0, acc_rel = 0.0;
         if (precision == KY_PRECISION_FP32) {
             KY_CHECK(q_rows == nullptr, KY_E501, "neighbors: FP32 candidates for a row subset");
             knn_candidates(ctx, prep, 0, n, m, c_rows.ptr, c_keys.ptr, c_count.ptr);
         } else {
-            bf16_candidates(ctx, pts, prep, q_rows, nq, m, c_rows.ptr, c_keys.ptr, c_count.ptr, err, err_max, acc_rel);
+            c_thr.alloc(nq);
+            bf16_candidates(ctx, pts, prep, q_rows, nq, m, k, c_rows.ptr, c_keys.ptr, c_count.ptr, c_thr.ptr, err,
+                            err_max, acc_rel);
         }
         search_ms = timer.stop();
         timer.start();
-        knn_rerank(ctx, pts, prep, q_rows, 0, nq, m, k, c_rows.ptr, c_keys.ptr, c_count.ptr,
+        knn_rerank(ctx, pts, prep, q_rows, 0, nq, m, k, c_rows.ptr, c_keys.ptr, c_count.ptr, c_thr.ptr,
                    precision == KY_PRECISION_FP32 ? 0 : 1, err.ptr, err_max, acc_rel, keys, rows, cert.ptr);
         std::vector<uint8_t> hcert(n);
         KY_HIP(hipMemcpyAsync(hcert.data(), cert.ptr, n, hipMemcpyDeviceToHost, st));

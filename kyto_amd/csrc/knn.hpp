@@ -25,8 +25,8 @@ void knn_candidates(Context &ctx, const KnnPrep &prep, uint64_t q_first, uint64_
                     uint32_t *d_rows, float *d_keys, uint32_t *d_count);
 void knn_rerank(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t q_first,
                 uint64_t q_count, uint32_t m, uint32_t k, const uint32_t *cand_rows, const float *cand_keys,
-                const uint32_t *cand_count, int floor_mode, const double *bf_err, double bf_err_max,
-                double bf_acc_rel, double *d_keys, uint32_t *d_rows, uint8_t *d_cert);
+                const uint32_t *cand_count, const float *cand_thr, int floor_mode, const double *bf_err,
+                double bf_err_max, double bf_acc_rel, double *d_keys, uint32_t *d_rows, uint8_t *d_cert);
 void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32_t k, int precision,
                   const uint32_t *q_rows, uint64_t nq, double *keys, uint32_t *rows, ky_knn_info *info);
 uint32_t check_search(const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric);

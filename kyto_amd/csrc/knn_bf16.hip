@@ -48,12 +48,15 @@ namespace ky {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int KDIM = 64;        // augmented components
 constexpr int TILE = 128;       // points per tile
-constexpr int ROW_BYTES = 144;  // LDS row stride (128 data bytes + 16 padding)
+constexpr int ROW_BYTES = 128;  // LDS row stride (chunks XOR-swizzled by row)
 constexpr int BCAP = 512;       // buffer entries per query (power of two)
 constexpr int NB = 4;           // column blocks (16 queries each) per wavefront
+constexpr int MB = 2;           // row blocks (16 points each) per super-block
+constexpr int SPB = 16 * MB;    // points per super-block
 constexpr int QPW = 16 * NB;    // queries per wavefront
 constexpr int QPB = 4 * QPW;    // queries per block
 
@@ -102,6 +105,14 @@ __global__ void bf16_rows(const float *__restrict__ x, const double *__restrict_
     atomicMax(nb_max, (unsigned long long)__double_as_longlong(nrm));
 }
 
+// Padding rows of P' (after the last row): key 1e30 against every query.
+__global__ void pad_rows(__bf16 *__restrict__ P, uint64_t first, uint64_t count, uint32_t d) {
+    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= count * KDIM) return;
+    const uint32_t k = (uint32_t)(t % KDIM);
+    P[first * KDIM + t] = (__bf16)(k == d ? 1e30f : 0.0f);
+}
+
 // Largest |value| of each of the first `c` components (bits of non-negative floats
 // order as the floats).
 __global__ void col_absmax(const float *__restrict__ cent, uint64_t n, uint32_t dp, uint32_t c,
@@ -147,6 +158,21 @@ __global__ void query_positions(const uint32_t *__restrict__ q_rows, uint64_t nq
     vals[i] = (uint32_t)i;
 }
 
+// The queries (scan-order row, output slot) whose count is below `need`, listed in
+// any order.
+__global__ void short_queries(const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq,
+                              const uint32_t *__restrict__ count, uint32_t need, uint32_t *__restrict__ pos,
+                              uint32_t *__restrict__ out, uint32_t *__restrict__ n_out) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const uint32_t o = qout[i];
+    if (count[o] < need) {
+        const uint32_t e = atomicAdd(n_out, 1u);
+        pos[e] = qpos ? qpos[i] : (uint32_t)i;
+        out[e] = o;
+    }
+}
+
 __global__ void bf16_delta(const double *__restrict__ nb, uint64_t n, double nb_max, double c,
                            double *__restrict__ delta) {
     uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -157,8 +183,10 @@ __global__ void bf16_delta(const double *__restrict__ nb, uint64_t n, double nb_
 
 // ------------------------------------------------------------------- search ----
 
-constexpr int INIT = 4 * TILE;   // the first points, kept without a threshold
-constexpr int EPL = BCAP / 64;   // buffer entries per lane during a compaction
+constexpr int EPL = BCAP / 64;  // buffer entries per lane during a compaction
+
+// A buffer entry: the key's f32 bits and the point's scan-order row.
+typedef uint2 Entry;
 
 // Number of entries (over the wave's registers) with key <= c.
 __device__ __forceinline__ unsigned count_le(const uint32_t (&k)[EPL], unsigned E, uint32_t c) {
@@ -174,8 +202,8 @@ __device__ __forceinline__ unsigned count_le(const uint32_t (&k)[EPL], unsigned 
 // (T, R) is found bit by bit, then the entries at or below it are written back in
 // lane order. When cnt >= keep, *thr becomes T's key. With `sorted_out`, the kept
 // entries are also sorted into the wave's LDS scratch (keep <= 64). Whole wavefront.
-__device__ unsigned compact_buffer(float *__restrict__ bk, uint32_t *__restrict__ br, unsigned cnt, unsigned keep,
-                                   float *thr, bool sorted_out, uint32_t *skey, uint32_t *srow) {
+__device__ unsigned compact_buffer(Entry *__restrict__ buf, unsigned cnt, unsigned keep, float *thr, bool sorted_out,
+                                   uint32_t *skey, uint32_t *srow) {
     const unsigned lane = lane_id();
     wave_sync_memory();
     const unsigned E = (cnt + 63) / 64;
@@ -184,8 +212,9 @@ __device__ unsigned compact_buffer(float *__restrict__ bk, uint32_t *__restrict_
     for (int e = 0; e < EPL; e++) {
         unsigned i = lane + 64 * e;
         bool valid = (unsigned)e < E && i < cnt;
-        k[e] = valid ? order_f32(bk[i]) : 0xFFFFFFFFu;
-        r[e] = valid ? br[i] : 0xFFFFFFFFu;
+        Entry v = valid ? buf[i] : make_uint2(0, 0);
+        k[e] = valid ? order_f32(__uint_as_float(v.x)) : 0xFFFFFFFFu;
+        r[e] = valid ? v.y : 0xFFFFFFFFu;
     }
     uint32_t T = 0xFFFFFFFFu, R = 0xFFFFFFFFu;
     unsigned kept = cnt;
@@ -233,8 +262,7 @@ __device__ unsigned compact_buffer(float *__restrict__ bk, uint32_t *__restrict_
                 skey[pos] = k[e];
                 srow[pos] = r[e];
             } else {
-                bk[pos] = unorder_f32(k[e]);
-                br[pos] = r[e];
+                buf[pos] = make_uint2(__float_as_uint(unorder_f32(k[e])), r[e]);
             }
         }
         base += (unsigned)__popcll(msk);
@@ -251,195 +279,296 @@ __device__ unsigned compact_buffer(float *__restrict__ bk, uint32_t *__restrict_
     return kept;
 }
 
-constexpr int INIT_TILES = INIT / TILE;
-constexpr int LEAD_TILES = 16;  // tiles scanned before a block's own position
+constexpr int LEAD_TILES = 16;        // tiles scanned before a block's own position
+constexpr uint64_t EST_MIN_ROWS = 1 << 17;  // threshold estimates from a sample above this size
+constexpr uint32_t EST_STRIDE = 16;         // the sample: every 16th row of the scan order
+constexpr uint32_t EST_RANK = 10;           // its 10th smallest key: ~160 rows at or below
 
-// Query i (< nq) is row qpos[i] of the scan order (ascending in i; row i when qpos is
-// null); its outputs go to slot qout[i]; point rows are reported as the original rows
-// perm[p].
-__global__ __launch_bounds__(256) void search_bf16(const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q,
-                                                   uint64_t n, const uint32_t *__restrict__ qpos,
-                                                   const uint32_t *__restrict__ qout, uint64_t nq,
-                                                   const uint32_t *__restrict__ perm, uint32_t m,
-                                                   float *__restrict__ buf_key, uint32_t *__restrict__ buf_row,
-                                                   uint32_t *__restrict__ out_rows, float *__restrict__ out_keys,
-                                                   uint32_t *__restrict__ out_count) {
-    __shared__ __attribute__((aligned(16))) unsigned char tile[2][TILE * ROW_BYTES];
-    __shared__ uint32_t scratch_key[4][64];
-    __shared__ uint32_t scratch_row[4][64];
-    __shared__ unsigned qcnt[QPB];
-    __shared__ float qthr[QPB];
+constexpr unsigned SCAP = 64;  // staged appends per wavefront
+
+// The candidate scan. Points: np rows, point p being row p * pstride of the scan
+// order (P'). Queries: i < nq, query i being row qpos[i] of the scan order (row i
+// when qpos is null; ascending in i), outputs at slot o = qout[i].
+//
+// Each query keeps in its buffer (BCAP entries in device memory) every point whose
+// key is at most its threshold: thr_init[o] at the start (+inf when null), then,
+// after a compaction, the m-th smallest key so far. Outputs per slot: the m smallest
+// (key, row) sorted (rows as original rows perm[.], when out_rows is not null), their
+// count, and out_thr[o]: the m-th key when m were kept, else the threshold every
+// point above which was left out.
+//
+// A query's count and threshold live in registers of the 4 lanes that hold its keys
+// (lane l % 16 of each column block); an append takes its slot from a ballot, and is
+// staged in LDS (SCAP entries a wave) and written out when the stage fills, before a
+// compaction and at the end. Global stores are then rare, so the wait for the next
+// tile at the end of each tile seldom waits for stores as well.
+//
+// Blocks are mapped to queries XCD by XCD (block b runs on XCD b % 8): the blocks
+// resident on one XCD hold consecutive queries, start their scans at nearby tiles
+// and read the same tiles through that XCD's L2 at about the same time.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void search_bf16(
+    const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q, uint64_t np, uint32_t pstride,
+    const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq,
+    const uint32_t *__restrict__ perm, uint32_t m, const float *__restrict__ thr_init, Entry *__restrict__ buf,
+    uint32_t *__restrict__ out_rows, float *__restrict__ out_keys, uint32_t *__restrict__ out_count,
+    float *__restrict__ out_thr) {
+    __shared__ __attribute__((aligned(16))) unsigned char tile[3][TILE * ROW_BYTES];
+    // Per wave: the append stage, which a compaction's sort scratch overlays.
+    __shared__ __attribute__((aligned(16))) uint32_t stage_key[4][SCAP];
+    __shared__ __attribute__((aligned(16))) uint32_t stage_row[4][SCAP];
+    __shared__ uint16_t stage_dst[4][SCAP];
+    __shared__ uint32_t qrow_lds[QPB];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
-    const uint64_t qbase = (uint64_t)blockIdx.x * QPB + (uint64_t)wave * QPW;  // this wave's first query slot
-    unsigned *cnt = qcnt + wave * QPW;
-    float *thr_lds = qthr + wave * QPW;
-    for (unsigned i = lane; i < QPW; i += 64) {
-        cnt[i] = 0;
-        thr_lds[i] = INFINITY;
-    }
-    // B fragments: the wave's queries, loaded once.
+    // XCD-aware block order.
+    const unsigned nblk = gridDim.x, xcd = blockIdx.x % 8, per = nblk / 8, extra = nblk % 8;
+    const unsigned lblk = xcd * per + min(xcd, extra) + blockIdx.x / 8;
+    const uint64_t qbase = (uint64_t)lblk * QPB + (uint64_t)wave * QPW;  // this wave's first query
+    uint32_t *qrow = qrow_lds + wave * QPW;  // scan-order row of each of the wave's queries
+    // B fragments (the wave's queries, loaded once), and per column block b the
+    // threshold and buffer count of this lane's query 16 b + l % 16.
     bf16x8 bq[NB][2];
     float thr[NB];
-    uint32_t qrow[NB];  // scan-order row of each of this lane's query slots (n for none)
+    unsigned cb[NB];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         const uint64_t i = qbase + 16 * b + (lane % 16);
-        qrow[b] = i < nq ? (qpos ? qpos[i] : (uint32_t)i) : (uint32_t)n;
-        const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Q + (uint64_t)min<uint32_t>(qrow[b], (uint32_t)(n - 1)) * KDIM);
+        const uint64_t ic = min<uint64_t>(i, nq - 1);
+        const uint32_t r = qpos ? qpos[ic] : (uint32_t)ic;
+        if (lane < 16) qrow[16 * b + lane] = r;
+        const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Q + (uint64_t)r * KDIM);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) bq[b][ks] = src[4 * ks + lane / 16];
-        thr[b] = INFINITY;
+        // Slots past the last query take no point.
+        thr[b] = i < nq ? (thr_init ? thr_init[qout[i]] : INFINITY) : -INFINITY;
+        cb[b] = 0;
     }
+    unsigned staged = 0;  // wave-uniform
     const bool wave_live = qbase < nq;
-    const uint64_t n_tiles = (n + TILE - 1) / TILE;
-    const uint64_t q0 = min<uint64_t>((uint64_t)blockIdx.x * QPB, nq - 1);
-    const uint64_t own = (qpos ? (uint64_t)qpos[q0] : q0) / TILE;
+    // The staged appends to the buffers (one store instruction).
+    auto flush = [&](const uint32_t *sk, const uint32_t *sr, const uint16_t *sd) {
+        wave_sync();
+        if (lane < staged) {
+            const unsigned dst = sd[lane];
+            buf[(qbase + (dst >> 9)) * BCAP + (dst & (BCAP - 1))] = make_uint2(sk[lane], sr[lane]);
+        }
+        staged = 0;
+        wave_sync();
+    };
+    // Compact query 16 b + l of the wave (whole wavefront; uniform b, l).
+    auto compact_query = [&](int b, unsigned l, bool sorted, uint32_t *sk, uint32_t *sr) {
+        const unsigned c = __builtin_amdgcn_readlane(cb[b], l);
+        float t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr[b]), l));
+        const unsigned kept = compact_buffer(buf + (qbase + 16 * b + l) * BCAP, c, m, &t, sorted, sk, sr);
+        if (lane % 16 == l) {
+            cb[b] = kept;
+            thr[b] = t;
+        }
+        return kept;
+    };
+    const uint64_t n_tiles = (np + TILE - 1) / TILE;
+    const uint64_t q0 = min<uint64_t>((uint64_t)lblk * QPB, nq - 1);
+    const uint64_t own = ((qpos ? (uint64_t)qpos[q0] : q0) / pstride) / TILE;
     const uint64_t lead = min<uint64_t>(LEAD_TILES, n_tiles - 1);
-    const uint64_t first_tile = (own + n_tiles - lead) % n_tiles;
-    // Tile staging through registers, one tile ahead: 128 rows x 128 bytes, 4 x 16 bytes a thread.
-    uint4 pre[TILE * 8 / 256];
-    auto fetch = [&](uint64_t t0) {
+    const uint64_t first_tile = (min<uint64_t>(own, n_tiles - 1) + n_tiles - lead) % n_tiles;
+    // Tiles go straight from global memory to LDS (global_load_lds_dwordx4) into three
+    // buffers: tile t + 2 is requested once tile t's keys are done, so it has the
+    // whole of tile t + 1 to arrive. A tile is 128 rows of 128 bytes with the 16-byte
+    // chunk c of row r stored at chunk c ^ (r % 8) of its row, so the 16 rows an MFMA
+    // fragment read touches fall on different banks. Each wave instruction fills 8 rows
+    // (lane l: row l / 8, stored chunk l % 8). Rows past the last point read the padding
+    // rows after P' (keys 1e30, never kept).
+    auto fetch = [&](uint64_t t0, unsigned char *__restrict__ dst) {
+        const __bf16 *tp = P + t0 * pstride * KDIM;
 #pragma unroll
-        for (int s = 0; s < TILE * 8 / 256; s++) {
-            unsigned i = tid + 256 * s, row = i / 8, chunk = i % 8;
-            uint64_t p = t0 + row;
-            pre[s] = p < n ? reinterpret_cast<const uint4 *>(P + p * KDIM)[chunk] : make_uint4(0, 0, 0, 0);
+        for (int s = 0; s < TILE / 32; s++) {
+            const unsigned row = (wave * (TILE / 32) + s) * 8 + lane / 8;
+            const unsigned chunk = (lane % 8) ^ (row % 8);
+            const __bf16 *src = tp + (row * pstride * KDIM + chunk * 8);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                             (__attribute__((address_space(3))) void *)(dst + (wave * (TILE / 32) + s) * 1024),
+                                             16, 0, 0);
         }
     };
-    auto stage = [&](int buf) {
-#pragma unroll
-        for (int s = 0; s < TILE * 8 / 256; s++) {
-            unsigned i = tid + 256 * s, row = i / 8, chunk = i % 8;
-            *reinterpret_cast<uint4 *>(tile[buf] + row * ROW_BYTES + chunk * 16) = pre[s];
-        }
-    };
-    fetch(first_tile * TILE);
-    stage(0);
+    // Tile indices advance by one and wrap (no division in the loop).
+    auto wrap_next = [&](uint32_t t) { return t + 1 == (uint32_t)n_tiles ? 0u : t + 1; };
+    uint32_t tcur = (uint32_t)first_tile, tfetch = wrap_next(tcur);
+    fetch((uint64_t)tcur * TILE, tile[0]);
+    if (n_tiles > 1) fetch((uint64_t)tfetch * TILE, tile[1]);
+    tfetch = wrap_next(tfetch);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
+    int cur = 0;
     for (uint64_t it = 0; it < n_tiles; it++) {
-        const int cur = (int)(it & 1);
-        const uint64_t t0 = ((first_tile + it) % n_tiles) * TILE;
-        if (it + 1 < n_tiles) fetch(((first_tile + it + 1) % n_tiles) * TILE);
-        const bool init = it < INIT_TILES;
-        if (wave_live) {
-            const unsigned char *tb = tile[cur];
-            f32x4 acc[2][NB];
-            auto mfmas = [&](int mb, f32x4 (&out)[NB]) {
-                const unsigned prow = mb * 16 + (lane % 16);
-                const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(tb + prow * ROW_BYTES + (lane / 16) * 16);
-                const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(tb + prow * ROW_BYTES + (4 + lane / 16) * 16);
+        const uint64_t t0 = (uint64_t)tcur * TILE;
+        const int nxt = cur == 2 ? 0 : cur + 1, after = nxt == 2 ? 0 : nxt + 1;
+        // tb: this tile, nx: the buffer tile t + 2 goes to; sk, sr, sd: the stage. The
+        // restrict qualifiers tell the compiler that the stage and the tile reads do
+        // not touch the buffer in flight, so it does not wait for it before them.
+        auto body = [&](const unsigned char *__restrict__ tb, unsigned char *__restrict__ nx,
+                        uint32_t *__restrict__ sk, uint32_t *__restrict__ sr, uint16_t *__restrict__ sd) {
+            if (wave_live) {
+#pragma nounroll
+                for (int sb = 0; sb < TILE / SPB; sb++) {
+                    // Keys of the super-block: acc[b][4 g + j] is point 16 g + 4 (lane / 16) + j
+                    // of the super-block, query 16 b + lane % 16 of the wave.
+                    f32x16 acc[NB];
+                    // All fragments first, then the first halves of K of every (g, b), then
+                    // the second halves: each accumulator's two MFMAs are MB * NB apart.
+                    bf16x8 a0[MB], a1[MB];
 #pragma unroll
-                for (int b = 0; b < NB; b++) {
-                    f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-                    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bq[b][0], z, 0, 0, 0);
-                    out[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bq[b][1], z, 0, 0, 0);
-                }
-            };
-            mfmas(0, acc[0]);
+                    for (int g = 0; g < MB; g++) {
+                        const unsigned prow = sb * SPB + g * 16 + (lane % 16);
+                        const unsigned char *rp = tb + prow * ROW_BYTES;
+                        a0[g] = *reinterpret_cast<const bf16x8 *>(rp + (((lane / 16) ^ (prow % 8)) * 16));
+                        a1[g] = *reinterpret_cast<const bf16x8 *>(rp + (((4 + lane / 16) ^ (prow % 8)) * 16));
+                    }
+                    f32x4 z[MB][NB];
 #pragma unroll
-            for (int mb = 0; mb < TILE / 16; mb++) {
-                const f32x4(&a)[NB] = acc[mb & 1];
-                if (mb + 1 < TILE / 16) mfmas(mb + 1, acc[(mb + 1) & 1]);
-                // Lane l, slot j: scan row t0 + 16 mb + 4 (l / 16) + j, query 16 b + l % 16.
-                const uint64_t p0 = t0 + mb * 16 + 4 * (lane / 16);
-                if (init) {
-                    // The first INIT rows of the scan go to buffer slot (tile, row); rows past
-                    // the end and the query itself as (+inf, none).
+                    for (int g = 0; g < MB; g++)
+#pragma unroll
+                        for (int b = 0; b < NB; b++) {
+                            const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+                            z[g][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[g], bq[b][0], zero, 0, 0, 0);
+                        }
+#pragma unroll
+                    for (int g = 0; g < MB; g++)
+#pragma unroll
+                        for (int b = 0; b < NB; b++) {
+                            const f32x4 r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[g], bq[b][1], z[g][b], 0, 0, 0);
+                            if (g == 0) acc[b].s0123 = r;
+                            else if (g == 1) acc[b].s4567 = r;
+                            else if (g == 2) acc[b].s89ab = r;
+                            else acc[b].scdef = r;
+                        }
+                    // One test for the super-block: the smallest of each lane's 16 keys of a
+                    // query against the query's threshold.
+                    bool hit[NB];
+                    bool any = false;
 #pragma unroll
                     for (int b = 0; b < NB; b++) {
-                        const uint64_t i = qbase + 16 * b + (lane % 16);
+                        float lo = fminf(fminf(acc[b][0], acc[b][1]), fminf(acc[b][2], acc[b][3]));
 #pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            const uint64_t p = p0 + j;
-                            if (i < nq) {
-                                const bool valid = p < n && p != qrow[b];
-                                const uint64_t slot = it * TILE + mb * 16 + 4 * (lane / 16) + j;
-                                buf_key[i * BCAP + slot] = valid ? a[b][j] : INFINITY;
-                                buf_row[i * BCAP + slot] = valid ? (uint32_t)p : 0xFFFFFFFFu;
+                        for (int g = 1; g < MB; g++)
+                            lo = fminf(lo, fminf(fminf(acc[b][4 * g], acc[b][4 * g + 1]),
+                                                 fminf(acc[b][4 * g + 2], acc[b][4 * g + 3])));
+                        hit[b] = lo <= thr[b];
+                        any = any || hit[b];
+                    }
+                    if (!__ballot(any)) continue;
+                    // Appends: the 4 lanes of a query take buffer slots in lane order after
+                    // its count, and stage slots after the wave's staged entries. The keys of
+                    // a column block are walked with a uniform index into a 16-register
+                    // vector (VGPR indexing), not unrolled.
+                    const uint64_t pbase = t0 + sb * SPB + 4 * (lane / 16);
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+                        if (!__ballot(hit[b])) continue;
+                        // The keys (g, j) with a hit in some lane, found group by group.
+                        uint32_t sites = 0;
+#pragma unroll
+                        for (int g = 0; g < MB; g++) {
+                            const float gm = fminf(fminf(acc[b][4 * g], acc[b][4 * g + 1]),
+                                                   fminf(acc[b][4 * g + 2], acc[b][4 * g + 3]));
+                            if (!__ballot(gm <= thr[b])) continue;
+#pragma unroll
+                            for (int j = 0; j < 4; j++)
+                                if (__ballot(acc[b][4 * g + j] <= thr[b])) sites |= 1u << (4 * g + j);
+                        }
+                        const uint32_t self = qrow[16 * b + (lane % 16)];
+                        const uint64_t same = 0x0001000100010001ull << (lane % 16);
+                        while (sites) {
+                            const int e = __builtin_ctz(sites);
+                            sites &= sites - 1;
+                            const float x = acc[b][e];
+                            const uint64_t p = pbase + 16 * (e / 4) + (e % 4);
+                            const bool h = x <= thr[b] && p < np && p * pstride != self;
+                            const uint64_t mh = __ballot(h);
+                            if (!mh) continue;
+                            if (staged + (unsigned)__popcll(mh) > SCAP) flush(sk, sr, sd);
+                            const uint64_t mq = mh & same;
+                            if (h) {
+                                const unsigned slot = staged + mbcnt(mh);
+                                sk[slot] = __float_as_uint(x);
+                                sr[slot] = (uint32_t)(p * pstride);
+                                sd[slot] = (uint16_t)(((16 * b + (lane % 16)) << 9) | (cb[b] + mbcnt(mq)));
                             }
+                            staged += (unsigned)__popcll(mh);
+                            cb[b] += (unsigned)__popcll(mq);
                         }
                     }
-                    continue;
                 }
-                bool any = false;
+                // Compact the buffers that the next tile could overflow, and those holding
+                // m entries and either no threshold yet or 2m + 64 entries (the threshold
+                // then tightens to their m-th key).
+                uint64_t need[NB];
+                bool any_need = false;
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
-                    const float lo = fminf(fminf(a[b][0], a[b][1]), fminf(a[b][2], a[b][3]));
-                    any = any || lo <= thr[b];
+                    const unsigned c = cb[b];
+                    need[b] = __ballot(c > BCAP - TILE || (c >= m && (thr[b] == INFINITY || c >= 2 * m + 64))) & 0xFFFFull;
+                    any_need = any_need || need[b];
                 }
-                if (__ballot(any)) {
+                if (any_need) {
+                    flush(sk, sr, sd);
+                    wave_sync_memory();
 #pragma unroll
                     for (int b = 0; b < NB; b++) {
-                        const unsigned ql = 16 * b + (lane % 16);
-                        const uint64_t i = qbase + ql;
-#pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            const uint64_t p = p0 + j;
-                            if (a[b][j] <= thr[b] && p < n && p != qrow[b] && i < nq) {
-                                const unsigned slot = atomicAdd(&cnt[ql], 1u);
-                                buf_key[i * BCAP + slot] = a[b][j];
-                                buf_row[i * BCAP + slot] = (uint32_t)p;
-                            }
+                        while (need[b]) {
+                            const unsigned l = __builtin_ctzll(need[b]);
+                            need[b] &= need[b] - 1;
+                            compact_query(b, l, false, sk, sr);
                         }
                     }
                 }
             }
-            if (init) {
-                if (it + 1 == INIT_TILES || it + 1 == n_tiles) {
-                    // The seed set is complete: every live query holds `seeded` slots.
-                    const unsigned seeded = (unsigned)((it + 1) * TILE);
-                    for (unsigned ql = 0; ql < QPW; ql++) {
-                        const uint64_t i = qbase + ql;
-                        if (i >= nq) break;
-                        float t = INFINITY;
-                        const unsigned kept = compact_buffer(buf_key + i * BCAP, buf_row + i * BCAP, seeded, m, &t,
-                                                             false, scratch_key[wave], scratch_row[wave]);
-                        if (lane == 0) {
-                            cnt[ql] = kept;
-                            thr_lds[ql] = t;
-                        }
-                        wave_sync();
-                    }
-                }
+            if (it + 2 < n_tiles) {
+                fetch((uint64_t)tfetch * TILE, nx);
+                __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4): all but tile t + 2 (tile t + 1 is in LDS)
             } else {
-                // Compact the buffers that the next tile could overflow.
-                wave_sync();
-                const bool need = lane < QPW && cnt[lane] > BCAP - TILE;
-                uint64_t mask = __ballot(need);
-                while (mask) {
-                    const unsigned ql = __builtin_ctzll(mask);
-                    mask &= mask - 1;
-                    const uint64_t i = qbase + ql;
-                    float t = thr_lds[ql];
-                    const unsigned kept = compact_buffer(buf_key + i * BCAP, buf_row + i * BCAP, cnt[ql], m, &t,
-                                                         false, scratch_key[wave], scratch_row[wave]);
-                    if (lane == 0) {
-                        cnt[ql] = kept;
-                        thr_lds[ql] = t;
-                    }
-                    wave_sync();
-                }
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
             }
-#pragma unroll
-            for (int b = 0; b < NB; b++) thr[b] = thr_lds[16 * b + (lane % 16)];
-        }
-        if (it + 1 < n_tiles) stage(cur ^ 1);
-        __syncthreads();
+        };
+        body(tile[cur], tile[after], stage_key[wave], stage_row[wave], stage_dst[wave]);
+        // A bare barrier: each wave has waited for its own part of tile t + 1 above,
+        // and the LDS the block shares in this loop is written by those loads only (a
+        // __syncthreads() fence would also wait for tile t + 2). The signal fences keep
+        // the compiler from moving memory operations across it.
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        cur = nxt;
+        tcur = wrap_next(tcur);
+        tfetch = wrap_next(tfetch);
     }
     if (!wave_live) return;
+    flush(stage_key[wave], stage_row[wave], stage_dst[wave]);
+    wave_sync_memory();
     for (unsigned ql = 0; ql < QPW; ql++) {
         const uint64_t i = qbase + ql;
         if (i >= nq) break;
-        float t = thr_lds[ql];
-        const unsigned kept = compact_buffer(buf_key + i * BCAP, buf_row + i * BCAP, cnt[ql], m, &t, true,
-                                             scratch_key[wave], scratch_row[wave]);
-        const uint64_t o = qout[i];
-        for (unsigned e = lane; e < m; e += 64) {
-            const uint32_t row = scratch_row[wave][e];
-            out_rows[o * m + e] = e < kept && row < n ? perm[row] : 0xFFFFFFFFu;
-            out_keys[o * m + e] = e < kept ? unorder_f32(scratch_key[wave][e]) : INFINITY;
+        const int b = (int)(ql / 16);
+        unsigned kept;
+        // b is uniform; the switch keeps the register arrays statically indexed.
+        switch (b) {
+            case 0: kept = compact_query(0, ql % 16, true, stage_key[wave], stage_row[wave]); break;
+            case 1: kept = compact_query(1, ql % 16, true, stage_key[wave], stage_row[wave]); break;
+            case 2: kept = compact_query(2, ql % 16, true, stage_key[wave], stage_row[wave]); break;
+            default: kept = compact_query(3, ql % 16, true, stage_key[wave], stage_row[wave]); break;
         }
-        if (lane == 0) out_count[o] = kept;
+        const float t = __int_as_float(__builtin_amdgcn_readlane(
+            __float_as_int(b == 0 ? thr[0] : b == 1 ? thr[1] : b == 2 ? thr[2] : thr[3]), ql % 16));
+        const uint64_t o = qout[i];
+        if (out_rows) {
+            for (unsigned e = lane; e < m; e += 64) {
+                const uint32_t row = stage_row[wave][e];
+                out_rows[o * m + e] = e < kept && row < 0xFFFFFFFFu ? perm[row] : 0xFFFFFFFFu;
+                out_keys[o * m + e] = e < kept ? unorder_f32(stage_key[wave][e]) : INFINITY;
+            }
+        }
+        if (lane == 0) {
+            out_count[o] = kept;
+            out_thr[o] = t;
+        }
         wave_sync();
     }
 }
@@ -447,8 +576,8 @@ __global__ __launch_bounds__(256) void search_bf16(const __bf16 *__restrict__ P,
 // ================================================================== host side ====
 
 void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t nq,
-                     uint32_t m, uint32_t *d_rows, float *d_keys, uint32_t *d_count, DevBuf<double> &err,
-                     double &err_max, double &acc_rel) {
+                     uint32_t m, uint32_t k, uint32_t *d_rows, float *d_keys, uint32_t *d_count, float *d_thr,
+                     DevBuf<double> &err, double &err_max, double &acc_rel) {
     KY_CHECK(prep.d + 6 <= KDIM, KY_E501, "neighbors: no BF16 kernel for more than 58 components");
     KY_CHECK(m >= 1 && m <= 64, KY_E215, "neighbors: candidates out of range");
     hipStream_t st = ctx.stream;
@@ -464,7 +593,11 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
     DevBuf<unsigned char> sort_tmp;
     sort_pairs(st, keys.ptr, skeys.ptr, rows.ptr, perm.ptr, n, 30, sort_tmp);
     // Rows in scan order.
-    DevBuf<__bf16> P(n * KDIM), Q(n * KDIM);
+    // P' has PAD_ROWS padding rows: a scan reads whole tiles, up to row
+    // (ceil(n / stride) + TILE) * stride.
+    const uint64_t PAD_ROWS = (uint64_t)(TILE + 1) * EST_STRIDE;
+    DevBuf<__bf16> P((n + PAD_ROWS) * KDIM), Q(n * KDIM);
+    pad_rows<<<div_up(PAD_ROWS * KDIM, 256), 256, 0, st>>>(P.ptr, n, PAD_ROWS, prep.d);
     err.alloc(2 * n);  // err, then delta (original rows)
     DevBuf<double> nb(n);
     DevBuf<unsigned long long> maxima(2);
@@ -504,12 +637,43 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
         qpos = qpos_buf.ptr;
         qout = qout_buf.ptr;
     }
-    DevBuf<float> bk(nq * BCAP);
-    DevBuf<uint32_t> br(nq * BCAP);
+    // Phase 1 (large inputs): each query's EST_RANK-th smallest key over every
+    // EST_STRIDE-th row of the scan order, an estimate of its final threshold that
+    // about EST_RANK * EST_STRIDE rows fall below. Phase 2 starts every query at that
+    // threshold, so its buffer takes only the rows at or below it (any starting
+    // threshold gives the same candidates; a low one only leaves fewer than m).
+    DevBuf<Entry> buf(nq * BCAP);
+    DevBuf<float> thr_est;
+    DevBuf<uint32_t> cnt_est;
     const unsigned blocks = div_up(nq, QPB);
-    search_bf16<<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, n, qpos, qout, nq, perm.ptr, m, bk.ptr, br.ptr, d_rows, d_keys,
-                                        d_count);
+    if (n >= EST_MIN_ROWS) {
+        thr_est.alloc(nq);
+        cnt_est.alloc(nq);
+        const uint64_t np = div_up(n, EST_STRIDE);
+        search_bf16<<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, perm.ptr, EST_RANK, nullptr,
+                                            buf.ptr, nullptr, nullptr, cnt_est.ptr, thr_est.ptr);
+        KY_HIP(hipGetLastError());
+    }
+    search_bf16<<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, n, 1, qpos, qout, nq, perm.ptr, m, thr_est.ptr, buf.ptr, d_rows,
+                                        d_keys, d_count, d_thr);
     KY_HIP(hipGetLastError());
+    if (thr_est.ptr) {
+        // Phase 3: the queries whose estimate left fewer than k rows at or below it,
+        // scanned again from no threshold (with k or more, the re-rank bounds the rows
+        // left out by the threshold itself).
+        DevBuf<uint32_t> re_pos(nq), re_out(nq), re_n(1);
+        KY_HIP(hipMemsetAsync(re_n.ptr, 0, sizeof(uint32_t), st));
+        short_queries<<<div_up(nq, 256), 256, 0, st>>>(qpos, qout, nq, d_count, k, re_pos.ptr, re_out.ptr, re_n.ptr);
+        KY_HIP(hipGetLastError());
+        uint32_t h_n = 0;
+        KY_HIP(hipMemcpyAsync(&h_n, re_n.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        KY_HIP(hipStreamSynchronize(st));
+        if (h_n) {
+            search_bf16<<<div_up(h_n, QPB), 256, 0, st>>>(P.ptr, Q.ptr, n, 1, re_pos.ptr, re_out.ptr, h_n, perm.ptr, m,
+                                                         nullptr, buf.ptr, d_rows, d_keys, d_count, d_thr);
+            KY_HIP(hipGetLastError());
+        }
+    }
     KY_HIP(hipStreamSynchronize(st));
 }
 
