@@ -72,8 +72,15 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="1m")
-    ap.add_argument("--search", default="exact", choices=["exact", "learned"])
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
+    ap.add_argument("--search", default="learned", choices=["exact", "learned"])
+    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16"])
+    ap.add_argument("--n-lists", type=int, default=4096)
+    ap.add_argument("--kmeans-iters", type=int, default=10)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--train-queries", type=int, default=200_000)
+    ap.add_argument("--confidence", type=float, default=0.0)
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the recall check and the plain FP32 kNN stage timed beside the learned path")
     ap.add_argument("--q", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -103,7 +110,18 @@ def main():
         if args.search == "exact":
             g = dev.neighbors(emb, k, precision=args.precision)
         else:
-            raise SystemExit("learned search is not built yet")
+            t0 = time.perf_counter()
+            index = dev.ivf_build(emb, n_lists=args.n_lists, kmeans_iters=args.kmeans_iters, seed=0, hidden=256,
+                                  train_queries=args.train_queries, batch=4096, epochs=args.epochs, lr=1e-3,
+                                  label_k=k - 1)
+            build_ms = (time.perf_counter() - t0) * 1e3
+            g = index.search(emb, k, n_probe=8, use_router=True, confidence=args.confidence)
+            g.info["build_ms"] = build_ms
+            g.info.update({f"build_{key}": v for key, v in index.info.items()})
+            g.info["search_ms"] = g.info["route_ms"] + g.info["search_ms"]
+            g.info["knn_total_ms"] = build_ms + g.info["total_ms"]
+            g.info["total_ms"] = g.info["knn_total_ms"]
+            index.free()
         return res, g, emb
 
     for i in range(args.warmup):
@@ -119,19 +137,27 @@ def main():
         pcas.append(res.timing)
         knns.append(g.info)
         log(f"step {i}: {dt * 1e3:.1f} ms (pca {res.timing['total_ms']:.1f} ms, knn {g.info['total_ms']:.1f} ms, "
-            f"search {g.info['search_ms']:.1f}, fallback rows {g.info['fallback']})")
+            f"search {g.info['search_ms']:.1f}, fallback rows {g.info['fallback']}) {g.info}")
         if i < args.steps - 1:
             emb.free()
     ms = 1e3 * float(np.mean(times))
     value = n / (ms / 1e3)
-    # Dominant kernel: the candidate key search, 2 n^2 d flops per full search.
     d = 50
-    search_ms = float(np.mean([x["search_ms"] for x in knns]))
-    flops = 2.0 * n * n * d
-    achieved = flops / (search_ms / 1e3) / 1e12
     peak = PEAKS_TFLOPS[args.precision]
+    if args.search == "exact":
+        # Dominant kernel: the candidate key search, 2 n^2 d flops per full search.
+        search_ms = float(np.mean([x["search_ms"] for x in knns]))
+        achieved = 2.0 * n * n * d / (search_ms / 1e3) / 1e12
+        roofline = {"bound": "mfma" if args.precision == "bf16" else "valu", "kernel": "candidate key search",
+                    "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
+    else:
+        roofline = None
     embedding = emb.download()
-    emb.free()
+
+    def mean_fields(rows):
+        keys = [key for key, v in rows[-1].items() if isinstance(v, (int, float))]
+        return {key: float(np.mean([r[key] for r in rows])) for key in keys}
+
     out = {
         "metric": "cells/s pca(50)+neighbors(15) @1M×2k",
         "value": value,
@@ -149,21 +175,27 @@ def main():
                                f"density {layer.density:.4f}, nnz {layer.nnz}",
                    "pca": f"n_components=50, block 60, q={args.q}, seed 0",
                    "neighbors": f"n_neighbors=15, euclidean, search={args.search}, candidates {args.precision}"},
-        "roofline": {"bound": "mfma" if args.precision == "bf16" else "valu", "kernel": "candidate key search",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None},
-        "breakdown": {
-            "pca_ms": float(np.mean([p["total_ms"] for p in pcas])),
-            "pca_pass_ms": pcas[-1]["pass_ms"],
-            "pca_dense_ms": float(np.mean([p["dense_ms"] for p in pcas])),
-            "knn_ms": float(np.mean([x["total_ms"] for x in knns])),
-            "knn_search_ms": search_ms,
-            "knn_rerank_ms": float(np.mean([x["rerank_ms"] for x in knns])),
-            "knn_fallback_ms": float(np.mean([x["fallback_ms"] for x in knns])),
-            "knn_fallback_rows": int(knns[-1]["fallback"]),
-            "knn_candidates": int(knns[-1]["candidates"]),
-        },
+        "roofline": roofline,
+        "breakdown": {"pca": mean_fields(pcas), "pca_pass_ms": pcas[-1]["pass_ms"], "knn": mean_fields(knns)},
     }
+    if args.search == "learned" and not args.no_compare:
+        # Outside the timed region: the exact graph (kyto's) for recall, and the plain
+        # FP32 exact kNN stage on the same embedding for the speed-up of the kNN stage.
+        exact = dev.neighbors(emb, k, precision="bf16")
+        got = g.rows()
+        ref = exact.rows()
+        hits = sum(np.intersect1d(a, b, assume_unique=True).size for a, b in zip(got, ref))
+        recall = hits / ref.size
+        plain = dev.neighbors(emb, k, precision="fp32")
+        learned_knn_ms = float(np.mean([x["knn_total_ms"] for x in knns]))
+        out["learned"] = {"recall_at_15": recall,
+                          "recall_definition": "share of the exact graph's 14 stored neighbours (self excluded) found",
+                          "exact_graph_equal_rows": int(sum(np.array_equal(a, b) for a, b in zip(got, ref))),
+                          "plain_fp32_knn_ms": plain.info["total_ms"], "learned_knn_ms": learned_knn_ms,
+                          "knn_speedup_vs_plain_fp32": plain.info["total_ms"] / learned_knn_ms}
+        log(f"recall@15 {recall:.5f}; plain FP32 kNN {plain.info['total_ms']:.1f} ms vs learned "
+            f"{learned_knn_ms:.1f} ms ({plain.info['total_ms'] / learned_knn_ms:.1f}x)")
+    emb.free()
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(layer, embedding, w.n_genes, args.q, threads)
