@@ -117,6 +117,10 @@ typedef struct ky_pca_timing {
     double dense_ms;    /* host FP64 steps */
     double total_ms;
     uint32_t n_passes;
+    uint32_t spmm_launches[2]; /* launches of the row product X M and the column product X^T W */
+    double spmm_ms[2];         /* their summed device time (HIP events on the library's stream) */
+    double spmm_bytes[2];      /* their summed algorithmic bytes: the sparse operand (8 B a
+                                  stored value + pointers), the dense operand and the output */
 } ky_pca_timing;
 
 int ky_pca(ky_ctx *ctx, const ky_csr *csr, uint32_t n_components, uint64_t seed,
@@ -132,6 +136,7 @@ typedef struct ky_knn_info {
     uint64_t certified;     /* rows whose candidates were proven to hold the k nearest */
     uint64_t fallback;      /* rows searched again exactly against every row */
     double search_ms, rerank_ms, fallback_ms, total_ms;
+    double scan_kernel_ms;  /* BF16: device time of the full candidate scan launch (HIP events) */
 } ky_knn_info;
 
 /* Exact neighbours of every row of `points` (its first `n_components` columns), as

@@ -40,13 +40,14 @@ class KytoError(RuntimeError):
 
 class PcaTiming(ctypes.Structure):
     _fields_ = [("upload_ms", c_double), ("pass_ms", c_double * 16), ("dense_ms", c_double),
-                ("total_ms", c_double), ("n_passes", c_uint32)]
+                ("total_ms", c_double), ("n_passes", c_uint32), ("spmm_launches", c_uint32 * 2),
+                ("spmm_ms", c_double * 2), ("spmm_bytes", c_double * 2)]
 
 
 class KnnInfo(ctypes.Structure):
     _fields_ = [("candidates", c_uint64), ("certified", c_uint64), ("fallback", c_uint64),
                 ("search_ms", c_double), ("rerank_ms", c_double), ("fallback_ms", c_double),
-                ("total_ms", c_double)]
+                ("total_ms", c_double), ("scan_kernel_ms", c_double)]
 
 
 class IvfParams(ctypes.Structure):

@@ -97,7 +97,7 @@ struct Csr {
     uint32_t chunk_rows = 0;
     uint32_t n_chunks = 0;
     DevBuf<uint32_t> col_ptr;
-    DevBuf<uint16_t> col_row;  // row within the chunk
+    DevBuf<uint32_t> col_row;  // row within the chunk
     DevBuf<float> col_val;
 };
 

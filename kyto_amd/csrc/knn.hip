@@ -515,7 +515,7 @@ struct EventTimer {
 
 void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t nq,
                      uint32_t m, uint32_t k, uint32_t *d_rows, float *d_keys, uint32_t *d_count, float *d_thr,
-                     DevBuf<double> &err, double &err_max, double &acc_rel);
+                     DevBuf<double> &err, double &err_max, double &acc_rel, double &scan_ms);
 
 // The checks of neighbors::search (neighbors.rs:226-312) that precede the search.
 uint32_t check_search(const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric) {
@@ -550,7 +550,7 @@ void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32
     KY_CHECK(prep.finite, KY_E215, "neighbors: the embedding holds a value that is not finite");
     DevBuf<uint8_t> cert(n);
     uint32_t m = 0;
-    double search_ms = 0, rerank_ms = 0, fallback_ms = 0;
+    double search_ms = 0, rerank_ms = 0, fallback_ms = 0, scan_ms = 0;
     std::vector<uint32_t> list;
     std::vector<uint32_t> hq;
     if (q_rows) {
@@ -572,7 +572,7 @@ void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32
         } else {
             c_thr.alloc(nq);
             bf16_candidates(ctx, pts, prep, q_rows, nq, m, k, c_rows.ptr, c_keys.ptr, c_count.ptr, c_thr.ptr, err,
-                            err_max, acc_rel);
+                            err_max, acc_rel, scan_ms);
         }
         search_ms = timer.stop();
         timer.start();
@@ -604,6 +604,7 @@ void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32
         info->search_ms = search_ms;
         info->rerank_ms = rerank_ms;
         info->fallback_ms = fallback_ms;
+        info->scan_kernel_ms = scan_ms;
     }
 }
 

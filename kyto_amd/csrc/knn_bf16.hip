@@ -306,6 +306,10 @@ constexpr unsigned SCAP = 64;  // staged appends per wavefront
 // Blocks are mapped to queries XCD by XCD (block b runs on XCD b % 8): the blocks
 // resident on one XCD hold consecutive queries, start their scans at nearby tiles
 // and read the same tiles through that XCD's L2 at about the same time.
+//
+// ESTIMATE only names the instantiation that computes the threshold estimates (the
+// code is the same), so that profiles tell the two scans apart.
+template <bool ESTIMATE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void search_bf16(
     const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q, uint64_t np, uint32_t pstride,
     const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq,
@@ -577,7 +581,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 
 void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t nq,
                      uint32_t m, uint32_t k, uint32_t *d_rows, float *d_keys, uint32_t *d_count, float *d_thr,
-                     DevBuf<double> &err, double &err_max, double &acc_rel) {
+                     DevBuf<double> &err, double &err_max, double &acc_rel, double &scan_ms) {
     KY_CHECK(prep.d + 6 <= KDIM, KY_E501, "neighbors: no BF16 kernel for more than 58 components");
     KY_CHECK(m >= 1 && m <= 64, KY_E215, "neighbors: candidates out of range");
     hipStream_t st = ctx.stream;
@@ -650,13 +654,18 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
         thr_est.alloc(nq);
         cnt_est.alloc(nq);
         const uint64_t np = div_up(n, EST_STRIDE);
-        search_bf16<<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, perm.ptr, EST_RANK, nullptr,
+        search_bf16<true><<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, perm.ptr, EST_RANK, nullptr,
                                             buf.ptr, nullptr, nullptr, cnt_est.ptr, thr_est.ptr);
         KY_HIP(hipGetLastError());
     }
-    search_bf16<<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, n, 1, qpos, qout, nq, perm.ptr, m, thr_est.ptr, buf.ptr, d_rows,
-                                        d_keys, d_count, d_thr);
+    hipEvent_t ev[2];
+    KY_HIP(hipEventCreate(&ev[0]));
+    KY_HIP(hipEventCreate(&ev[1]));
+    KY_HIP(hipEventRecord(ev[0], st));
+    search_bf16<false><<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, n, 1, qpos, qout, nq, perm.ptr, m, thr_est.ptr, buf.ptr,
+                                               d_rows, d_keys, d_count, d_thr);
     KY_HIP(hipGetLastError());
+    KY_HIP(hipEventRecord(ev[1], st));
     if (thr_est.ptr) {
         // Phase 3: the queries whose estimate left fewer than k rows at or below it,
         // scanned again from no threshold (with k or more, the re-rank bounds the rows
@@ -669,12 +678,17 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
         KY_HIP(hipMemcpyAsync(&h_n, re_n.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         KY_HIP(hipStreamSynchronize(st));
         if (h_n) {
-            search_bf16<<<div_up(h_n, QPB), 256, 0, st>>>(P.ptr, Q.ptr, n, 1, re_pos.ptr, re_out.ptr, h_n, perm.ptr, m,
+            search_bf16<false><<<div_up(h_n, QPB), 256, 0, st>>>(P.ptr, Q.ptr, n, 1, re_pos.ptr, re_out.ptr, h_n, perm.ptr, m,
                                                          nullptr, buf.ptr, d_rows, d_keys, d_count, d_thr);
             KY_HIP(hipGetLastError());
         }
     }
     KY_HIP(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    KY_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    scan_ms = ms;
 }
 
 }  // namespace ky

@@ -10,7 +10,7 @@
 //   integer SplitMix64 stream, rounded to f32, plus a column of ones (gene sums).
 // - row_project: one wavefront per row, lane j = column j: w_j = -offset_j +
 //   sum_e value_e * M[gene_e, j], exact f32 x f32 products accumulated in f64, the
-//   row's entries broadcast from lanes with readlane, M's rows read from L2.
+//   row's entries read with wave-uniform (scalar) loads, M's rows read from L2.
 //   Serves X Q - mu^T Q (product passes) and X V - mu^T V (the embedding).
 // - column_product: X^T W per chunk, one wavefront per (chunk, gene), lane = column,
 //   over the gene's run of the chunk (rows ascending) with W's rows of the chunk read
@@ -35,18 +35,6 @@ __device__ __forceinline__ uint64_t mix64(uint64_t v) {
     v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
     v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
     return v ^ (v >> 31);
-}
-
-template <typename T>
-__device__ __forceinline__ T bcast(T v, int lane) {
-    if constexpr (sizeof(T) == 4) {
-        int x = __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane);
-        return __builtin_bit_cast(T, x);
-    } else {
-        static_assert(sizeof(T) == 2, "4- or 2-byte values");
-        int x = __builtin_amdgcn_readlane(static_cast<int>(__builtin_bit_cast(uint16_t, v)), lane);
-        return __builtin_bit_cast(T, static_cast<uint16_t>(x));
-    }
 }
 
 // ----------------------------------------------------------------- validation --
@@ -79,7 +67,7 @@ __global__ void check_rows(const uint64_t *__restrict__ indptr, const uint32_t *
 __global__ void build_columns(const uint64_t *__restrict__ indptr, const uint32_t *__restrict__ indices,
                               const float *__restrict__ values, uint64_t n_rows, uint32_t n_cols,
                               uint32_t chunk_rows, uint32_t *__restrict__ col_ptr,
-                              uint16_t *__restrict__ col_row, float *__restrict__ col_val) {
+                              uint32_t *__restrict__ col_row, float *__restrict__ col_val) {
     extern __shared__ uint32_t lds[];
     const unsigned waves = blockDim.x / 64;
     uint32_t *cnt = lds;                         // [waves][n_cols]
@@ -132,7 +120,7 @@ __global__ void build_columns(const uint64_t *__restrict__ indptr, const uint32_
             uint32_t g = indices[e];
             uint32_t pos = mine[g];
             mine[g] = pos + 1;
-            col_row[pos] = (uint16_t)(r - r0);
+            col_row[pos] = (uint32_t)(r - r0);
             col_val[pos] = values[e];
         }
     }
@@ -161,70 +149,79 @@ __global__ void omega_rows(uint64_t n_rows, uint64_t seed, uint64_t cell_offset,
 
 // --------------------------------------------------------------- row project ----
 
+// Both sparse products walk a run of entries (gene or row index, value) that is the
+// same for the whole wavefront. The run is read through wave-uniform pointers, which
+// the compiler turns into scalar loads (8 entries a load), and each entry's dense row
+// is one buffer load whose row offset is a scalar register and whose lane offset is
+// fixed, so an entry costs one vector load, two conversions and one f64 fused
+// multiply-add, with UNROLL dense rows in flight. An f32 x f32 product is exact in
+// f64, so the fused multiply-add rounds exactly as the separate product and sum
+// would. Every lane loads (a branch around the loads would wait for each one): a lane
+// past the last column reads into the next dense row, or into the DENSE_PAD floats
+// after the last one, and its sum is never stored.
+constexpr int UNROLL = 8;
+// Floats allocated after every dense operand of the products (>= 64 * max CPL): the
+// lanes past the last column of the last row read them.
+constexpr size_t DENSE_PAD = 256;
+
+// The wave's index within its block as a wave-uniform value.
+__device__ __forceinline__ unsigned wave_in_block() {
+    return (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+}
+
+// A buffer resource over `bytes` bytes at p (loads past the end return 0).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t dense_rsrc(const float *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), 0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float dense_load(__amdgpu_buffer_rsrc_t r, uint32_t lane_bytes, uint32_t row_bytes) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_bytes, (int)row_bytes, 0));
+}
+
 // One wavefront per row; lane j holds columns j, j+64, ... (CPL of them, w <= 64*CPL).
-// out[r, j] = f32(-offsets[j] + sum_e f64(value_e) * f64(M[gene_e, j])).
+// out[r, j] = f32(-offsets[j] + sum_e f64(value_e) * f64(M[gene_e, j])), the entries
+// added in row order. M: m_rows x w.
 template <int CPL>
 __global__ __launch_bounds__(256) void row_project(const uint64_t *__restrict__ indptr,
                                                    const uint32_t *__restrict__ indices,
                                                    const float *__restrict__ values, uint64_t n_rows,
-                                                   const float *__restrict__ M, uint32_t w,
+                                                   const float *__restrict__ M, uint32_t m_rows, uint32_t w,
                                                    const double *__restrict__ offsets,
                                                    float *__restrict__ out, uint32_t ldo) {
-    const uint64_t r = blockIdx.x * (uint64_t)(blockDim.x / 64) + threadIdx.x / 64;
+    const uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / 64) + wave_in_block();
     if (r >= n_rows) return;
-    const int lane = lane_id();
+    const unsigned lane = lane_id();
+    const __amdgpu_buffer_rsrc_t mr = dense_rsrc(M, (m_rows * w + (uint32_t)DENSE_PAD) * 4u);
+    const uint32_t row_bytes = w * 4u;
     double acc[CPL];
 #pragma unroll
-    for (int i = 0; i < CPL; i++) {
-        unsigned j = lane + 64 * i;
-        acc[i] = j < w ? -offsets[j] : 0.0;
+    for (int c = 0; c < CPL; c++) {
+        const unsigned j = lane + 64 * c;
+        acc[c] = j < w ? -offsets[j] : 0.0;
     }
     const uint64_t s = indptr[r], e = indptr[r + 1];
-    for (uint64_t e0 = s; e0 < e; e0 += 64) {
-        const unsigned cnt = (unsigned)min<uint64_t>(64, e - e0);
-        uint32_t my_g = 0;
-        float my_v = 0.0f;
-        if ((unsigned)lane < cnt) {
-            my_g = indices[e0 + lane];
-            my_v = values[e0 + lane];
-        }
-        unsigned t = 0;
-        for (; t + 4 <= cnt; t += 4) {
-            uint32_t g[4];
-            float v[4];
+    const uint32_t *gi = indices + s, *ge = indices + e;
+    const float *vi = values + s;
+    for (; gi + UNROLL <= ge; gi += UNROLL, vi += UNROLL) {
+        float m[UNROLL][CPL];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                g[u] = bcast(my_g, t + u);
-                v[u] = bcast(my_v, t + u);
-            }
-            float m[4][CPL];
+        for (int u = 0; u < UNROLL; u++)
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int c = 0; c < CPL; c++) m[u][c] = dense_load(mr, 4u * (lane + 64 * c), gi[u] * row_bytes);
 #pragma unroll
-                for (int i = 0; i < CPL; i++) {
-                    unsigned j = lane + 64 * i;
-                    m[u][i] = j < w ? M[(size_t)g[u] * w + j] : 0.0f;
-                }
+        for (int u = 0; u < UNROLL; u++)
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int c = 0; c < CPL; c++) acc[c] = fma((double)vi[u], (double)m[u][c], acc[c]);
+    }
+    for (; gi < ge; gi++, vi++) {
 #pragma unroll
-                for (int i = 0; i < CPL; i++) acc[i] += (double)v[u] * (double)m[u][i];
-        }
-        for (; t < cnt; t++) {
-            uint32_t g = bcast(my_g, t);
-            float v = bcast(my_v, t);
-#pragma unroll
-            for (int i = 0; i < CPL; i++) {
-                unsigned j = lane + 64 * i;
-                float m = j < w ? M[(size_t)g * w + j] : 0.0f;
-                acc[i] += (double)v * (double)m;
-            }
-        }
+        for (int c = 0; c < CPL; c++)
+            acc[c] = fma((double)*vi, (double)dense_load(mr, 4u * (lane + 64 * c), *gi * row_bytes), acc[c]);
     }
 #pragma unroll
-    for (int i = 0; i < CPL; i++) {
-        unsigned j = lane + 64 * i;
-        if (j < w) out[r * ldo + j] = (float)acc[i];
+    for (int c = 0; c < CPL; c++) {
+        const unsigned j = lane + 64 * c;
+        if (j < w) out[r * ldo + j] = (float)acc[c];
     }
 }
 
@@ -232,12 +229,13 @@ __global__ __launch_bounds__(256) void row_project(const uint64_t *__restrict__ 
 
 // partial[(chunk * n_cols + g) * w + j] = sum over gene g's entries of the chunk (rows
 // ascending) of f64(value) * f64(W[row, j]). One wavefront per gene, 4 per block,
-// `genes_per_block` genes per block; blocks of one chunk land on one XCD.
+// `genes_per_block` genes per block; blocks of one chunk land on one XCD, so the
+// chunk's rows of W are read through that XCD's L2. W: n_rows x ldw.
 template <int CPL>
 __global__ __launch_bounds__(256) void column_product(const uint32_t *__restrict__ col_ptr,
-                                                      const uint16_t *__restrict__ col_row,
+                                                      const uint32_t *__restrict__ col_row,
                                                       const float *__restrict__ col_val, uint32_t n_cols,
-                                                      uint32_t n_chunks, uint32_t chunk_rows,
+                                                      uint32_t n_chunks, uint32_t chunk_rows, uint64_t n_rows,
                                                       const float *__restrict__ W, uint32_t w, uint32_t ldw,
                                                       uint32_t genes_per_block, double *__restrict__ partial) {
     const unsigned groups = (n_cols + genes_per_block - 1) / genes_per_block;
@@ -245,62 +243,41 @@ __global__ __launch_bounds__(256) void column_product(const uint32_t *__restrict
     const unsigned slot = local / groups, group = local % groups;
     const unsigned chunk = slot * 8 + xcd;
     if (chunk >= n_chunks) return;
-    const int lane = lane_id();
-    const unsigned wave = threadIdx.x / 64, waves = blockDim.x / 64;
+    const unsigned lane = lane_id();
+    const unsigned wave = wave_in_block(), waves = blockDim.x / 64;
     const unsigned gend = min((group + 1) * genes_per_block, n_cols);
     const uint32_t *cp = col_ptr + (size_t)chunk * (n_cols + 1);
-    const float *Wc = W + (size_t)chunk * chunk_rows * ldw;
+    const uint64_t r0 = (uint64_t)chunk * chunk_rows;
+    const uint32_t rows = (uint32_t)min<uint64_t>(chunk_rows, n_rows - r0);
+    const __amdgpu_buffer_rsrc_t wr = dense_rsrc(W + r0 * ldw, (rows * ldw + (uint32_t)DENSE_PAD) * 4u);
+    const uint32_t row_bytes = ldw * 4u;
     for (unsigned g = group * genes_per_block + wave; g < gend; g += waves) {
         double acc[CPL];
 #pragma unroll
-        for (int i = 0; i < CPL; i++) acc[i] = 0.0;
-        const uint32_t s = cp[g], e = cp[g + 1];
-        for (uint32_t e0 = s; e0 < e; e0 += 64) {
-            const unsigned cnt = min<uint32_t>(64, e - e0);
-            uint16_t my_r = 0;
-            float my_v = 0.0f;
-            if ((unsigned)lane < cnt) {
-                my_r = col_row[e0 + lane];
-                my_v = col_val[e0 + lane];
-            }
-            unsigned t = 0;
-            for (; t + 4 <= cnt; t += 4) {
-                uint32_t rr[4];
-                float v[4];
+        for (int c = 0; c < CPL; c++) acc[c] = 0.0;
+        const uint32_t *ri = col_row + cp[g], *re = col_row + cp[g + 1];
+        const float *vi = col_val + cp[g];
+        for (; ri + UNROLL <= re; ri += UNROLL, vi += UNROLL) {
+            float m[UNROLL][CPL];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    rr[u] = bcast(my_r, t + u);
-                    v[u] = bcast(my_v, t + u);
-                }
-                float m[4][CPL];
+            for (int u = 0; u < UNROLL; u++)
 #pragma unroll
-                for (int u = 0; u < 4; u++)
+                for (int c = 0; c < CPL; c++) m[u][c] = dense_load(wr, 4u * (lane + 64 * c), ri[u] * row_bytes);
 #pragma unroll
-                    for (int i = 0; i < CPL; i++) {
-                        unsigned j = lane + 64 * i;
-                        m[u][i] = j < w ? Wc[(size_t)rr[u] * ldw + j] : 0.0f;
-                    }
+            for (int u = 0; u < UNROLL; u++)
 #pragma unroll
-                for (int u = 0; u < 4; u++)
+                for (int c = 0; c < CPL; c++) acc[c] = fma((double)vi[u], (double)m[u][c], acc[c]);
+        }
+        for (; ri < re; ri++, vi++) {
 #pragma unroll
-                    for (int i = 0; i < CPL; i++) acc[i] += (double)v[u] * (double)m[u][i];
-            }
-            for (; t < cnt; t++) {
-                uint32_t rr = bcast(my_r, t);
-                float v = bcast(my_v, t);
-#pragma unroll
-                for (int i = 0; i < CPL; i++) {
-                    unsigned j = lane + 64 * i;
-                    float m = j < w ? Wc[(size_t)rr * ldw + j] : 0.0f;
-                    acc[i] += (double)v * (double)m;
-                }
-            }
+            for (int c = 0; c < CPL; c++)
+                acc[c] = fma((double)*vi, (double)dense_load(wr, 4u * (lane + 64 * c), *ri * row_bytes), acc[c]);
         }
         double *out = partial + ((size_t)chunk * n_cols + g) * w;
 #pragma unroll
-        for (int i = 0; i < CPL; i++) {
-            unsigned j = lane + 64 * i;
-            if (j < w) out[j] = acc[i];
+        for (int c = 0; c < CPL; c++) {
+            const unsigned j = lane + 64 * c;
+            if (j < w) out[j] = acc[c];
         }
     }
 }
@@ -344,25 +321,73 @@ __global__ void column_sums(const float *__restrict__ W, uint64_t n_rows, uint32
 }
 
 // Per (chunk, gene): the centred squares sum_e (f64(v_e) - mu_g)^2 over the chunk's
-// stored values of the gene, and their count.
-__global__ void gene_squares(const uint32_t *__restrict__ col_ptr, const float *__restrict__ col_val,
-                             uint32_t n_cols, uint32_t n_chunks, const double *__restrict__ means,
-                             double *__restrict__ sq_partial, double *__restrict__ count_partial) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+// stored values of the gene, and their count. One wavefront per (chunk, gene): lane l
+// sums the run's entries l, l + 64, ... in order, then a fixed butterfly adds the
+// lanes (the same bits on every run).
+__global__ __launch_bounds__(256) void gene_squares(const uint32_t *__restrict__ col_ptr,
+                                                    const float *__restrict__ col_val, uint32_t n_cols,
+                                                    uint32_t n_chunks, const double *__restrict__ means,
+                                                    double *__restrict__ sq_partial,
+                                                    double *__restrict__ count_partial) {
+    const uint64_t i = (uint64_t)blockIdx.x * (blockDim.x / 64) + wave_in_block();
     if (i >= (uint64_t)n_chunks * n_cols) return;
     const unsigned c = (unsigned)(i / n_cols), g = (unsigned)(i % n_cols);
+    const unsigned lane = lane_id();
     const uint32_t *cp = col_ptr + (size_t)c * (n_cols + 1);
     const double mu = means[g];
-    double s = 0.0;
-    for (uint32_t e = cp[g]; e < cp[g + 1]; e++) {
-        double d = (double)col_val[e] - mu;
-        s += d * d;
+    const uint32_t s = cp[g], e = cp[g + 1];
+    double acc = 0.0;
+    for (uint32_t k = s + lane; k < e; k += 64) {
+        const double d = (double)col_val[k] - mu;
+        acc += d * d;
     }
-    sq_partial[i] = s;
-    count_partial[i] = (double)(cp[g + 1] - cp[g]);
+#pragma unroll
+    for (int off = 32; off >= 1; off /= 2) acc += __shfl_xor(acc, off);
+    if (lane == 0) {
+        sq_partial[i] = acc;
+        count_partial[i] = (double)(e - s);
+    }
 }
 
 // ================================================================== host side ====
+
+// Device time of the sparse products of one pca() call: HIP events around each launch
+// on the library's stream, summed per kernel once the call has synchronised (kind 0:
+// row_project, kind 1: column_product). Set for the duration of pca_run on its thread.
+// bytes[kind] sums each launch's algorithmic bytes: the sparse operand it streams
+// (8 bytes a stored value, plus row or column pointers), the dense operand read once
+// and the output written once.
+struct SpmmClock {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[2];
+    double bytes[2] = {0.0, 0.0};
+    void begin(int kind, hipStream_t st, double algorithmic_bytes) {
+        bytes[kind] += algorithmic_bytes;
+        hipEvent_t a, b;
+        KY_HIP(hipEventCreate(&a));
+        KY_HIP(hipEventCreate(&b));
+        ev[kind].push_back({a, b});
+        KY_HIP(hipEventRecord(a, st));
+    }
+    void end(int kind, hipStream_t st) { KY_HIP(hipEventRecord(ev[kind].back().second, st)); }
+    double total(int kind) {
+        double t = 0.0;
+        for (auto &p : ev[kind]) {
+            KY_HIP(hipEventSynchronize(p.second));
+            float ms = 0.0f;
+            KY_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+            t += ms;
+        }
+        return t;
+    }
+    ~SpmmClock() {
+        for (auto &v : ev)
+            for (auto &p : v) {
+                (void)hipEventDestroy(p.first);
+                (void)hipEventDestroy(p.second);
+            }
+    }
+};
+static thread_local SpmmClock *t_clock = nullptr;
 
 static unsigned lds_limit(const Context &ctx) {
     return (unsigned)std::min<size_t>(ctx.props.sharedMemPerBlock, 64 * 1024);
@@ -447,18 +472,23 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
     const unsigned groups = div_up(csr.n_cols, gpb);
     const unsigned slots = div_up(csr.n_chunks, 8);
     const unsigned blocks = 8 * slots * groups;
+    if (t_clock)
+        t_clock->begin(1, st,
+                       8.0 * csr.nnz + 4.0 * csr.n_chunks * (csr.n_cols + 1.0) + 4.0 * csr.n_rows * w +
+                           8.0 * csr.n_chunks * csr.n_cols * w);
     if (w <= 64)
         column_product<1><<<blocks, 256, 0, st>>>(csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols,
-                                                  csr.n_chunks, csr.chunk_rows, ctx.wbuf.ptr, w, ldw, gpb,
+                                                  csr.n_chunks, csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, gpb,
                                                   ctx.partials.ptr);
     else if (w <= 128)
         column_product<2><<<blocks, 256, 0, st>>>(csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols,
-                                                  csr.n_chunks, csr.chunk_rows, ctx.wbuf.ptr, w, ldw, gpb,
+                                                  csr.n_chunks, csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, gpb,
                                                   ctx.partials.ptr);
     else
         column_product<4><<<blocks, 256, 0, st>>>(csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols,
-                                                  csr.n_chunks, csr.chunk_rows, ctx.wbuf.ptr, w, ldw, gpb,
+                                                  csr.n_chunks, csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, gpb,
                                                   ctx.partials.ptr);
+    if (t_clock) t_clock->end(1, st);
     KY_HIP(hipGetLastError());
     ordered_sum(st, ctx.partials.ptr, csr.n_chunks, stride, ctx.partials.ptr + stride * csr.n_chunks,
                 ctx.reduced.ptr);
@@ -484,15 +514,19 @@ static void launch_row_project(Csr &csr, const float *M, uint32_t w, const doubl
     const unsigned rows_per_block = 4;
     const unsigned blocks = div_up(csr.n_rows, rows_per_block);
     KY_CHECK(w <= 256, KY_E215, "pca: more than 256 columns in one pass");
+    if (t_clock)
+        t_clock->begin(0, ctx.stream,
+                       8.0 * csr.nnz + 8.0 * (csr.n_rows + 1.0) + 4.0 * csr.n_cols * w + 4.0 * csr.n_rows * w);
     if (w <= 64)
         row_project<1><<<blocks, 256, 0, ctx.stream>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows,
-                                                       M, w, offsets, out, ldo);
+                                                       M, csr.n_cols, w, offsets, out, ldo);
     else if (w <= 128)
         row_project<2><<<blocks, 256, 0, ctx.stream>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows,
-                                                       M, w, offsets, out, ldo);
+                                                       M, csr.n_cols, w, offsets, out, ldo);
     else
         row_project<4><<<blocks, 256, 0, ctx.stream>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows,
-                                                       M, w, offsets, out, ldo);
+                                                       M, csr.n_cols, w, offsets, out, ldo);
+    if (t_clock) t_clock->end(0, ctx.stream);
     KY_HIP(hipGetLastError());
 }
 
@@ -501,7 +535,7 @@ void pca_sketch(Csr &csr, uint64_t seed, uint64_t cell_offset, uint32_t l, doubl
     Context &ctx = *csr.ctx;
     KY_CHECK(l >= 1 && l + 1 <= 256, KY_E215, "pca: sketch width out of range");
     const uint32_t w = l + 1, ldw = w;
-    ctx.wbuf.reserve((size_t)csr.n_rows * ldw);
+    ctx.wbuf.reserve((size_t)csr.n_rows * ldw + DENSE_PAD);
     uint64_t total = csr.n_rows * (uint64_t)w;
     omega_rows<<<div_up(total, 256), 256, 0, ctx.stream>>>(csr.n_rows, seed, cell_offset, l, ctx.wbuf.ptr, ldw);
     KY_HIP(hipGetLastError());
@@ -519,12 +553,12 @@ void pca_product(Csr &csr, const double *q, uint32_t w, const double *block_mean
     KY_CHECK(w >= 1 && w <= 256, KY_E215, "pca: block width out of range");
     std::vector<float> q32((size_t)csr.n_cols * w);
     for (size_t i = 0; i < q32.size(); i++) q32[i] = (float)q[i];
-    ctx.dense_f32.reserve(q32.size());
+    ctx.dense_f32.reserve(q32.size() + DENSE_PAD);
     ctx.dense_f64.reserve(w);
     KY_HIP(hipMemcpyAsync(ctx.dense_f32.ptr, q32.data(), sizeof(float) * q32.size(), hipMemcpyHostToDevice,
                           ctx.stream));
     KY_HIP(hipMemcpyAsync(ctx.dense_f64.ptr, block_means, sizeof(double) * w, hipMemcpyHostToDevice, ctx.stream));
-    ctx.wbuf.reserve((size_t)csr.n_rows * w);
+    ctx.wbuf.reserve((size_t)csr.n_rows * w + DENSE_PAD);
     launch_row_project(csr, ctx.dense_f32.ptr, w, ctx.dense_f64.ptr, ctx.wbuf.ptr, w);
     transposed_product(csr, w, w, product, sums);
 }
@@ -535,7 +569,7 @@ void pca_project(Csr &csr, const double *v, uint32_t k, const double *mean_proje
     KY_CHECK(k >= 1 && k <= 256, KY_E215, "pca: n_components out of range");
     std::vector<float> v32((size_t)csr.n_cols * k);
     for (size_t i = 0; i < v32.size(); i++) v32[i] = (float)v[i];
-    ctx.dense_f32.reserve(v32.size());
+    ctx.dense_f32.reserve(v32.size() + DENSE_PAD);
     ctx.dense_f64.reserve(k + csr.n_cols);
     hipStream_t st = ctx.stream;
     KY_HIP(hipMemcpyAsync(ctx.dense_f32.ptr, v32.data(), sizeof(float) * v32.size(), hipMemcpyHostToDevice, st));
@@ -552,7 +586,7 @@ void pca_project(Csr &csr, const double *v, uint32_t k, const double *mean_proje
         const size_t cells = (size_t)csr.n_chunks * csr.n_cols;
         ctx.partials.reserve(2 * cells);
         ctx.reduced.reserve(2 * (size_t)csr.n_cols);
-        gene_squares<<<div_up(cells, 256), 256, 0, st>>>(csr.col_ptr.ptr, csr.col_val.ptr, csr.n_cols, csr.n_chunks,
+        gene_squares<<<div_up(cells, 4), 256, 0, st>>>(csr.col_ptr.ptr, csr.col_val.ptr, csr.n_cols, csr.n_chunks,
                                                          ctx.dense_f64.ptr + k, ctx.partials.ptr,
                                                          ctx.partials.ptr + cells);
         KY_HIP(hipGetLastError());
@@ -586,6 +620,11 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
              ky_pca_timing *timing) {
     const double t_start = now_ms();
     double dense_ms = 0.0;
+    SpmmClock clock;
+    t_clock = &clock;
+    struct Reset {
+        ~Reset() { t_clock = nullptr; }
+    } reset;
     const uint64_t n = csr.n_rows;
     const size_t genes = csr.n_cols;
     // check_cells (pca.rs:876-920).
@@ -690,6 +729,11 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
         timing->dense_ms = dense_ms;
         timing->total_ms = now_ms() - t_start;
         timing->n_passes = pass_idx;
+        for (int kind = 0; kind < 2; kind++) {
+            timing->spmm_ms[kind] = clock.total(kind);
+            timing->spmm_launches[kind] = (uint32_t)clock.ev[kind].size();
+            timing->spmm_bytes[kind] = clock.bytes[kind];
+        }
     }
 }
 

@@ -118,7 +118,10 @@ class Device:
                                  N.ptr(means, N.f64p), byref(passes), byref(timing)))
         dev = DenseMatrix._adopt(self, dh, layer.n_rows, k) if keep_on_device else None
         t = {"pass_ms": [timing.pass_ms[i] for i in range(min(16, timing.n_passes))],
-             "dense_ms": timing.dense_ms, "total_ms": timing.total_ms}
+             "dense_ms": timing.dense_ms, "total_ms": timing.total_ms,
+             "row_product_ms": timing.spmm_ms[0], "row_product_launches": timing.spmm_launches[0],
+             "column_product_ms": timing.spmm_ms[1], "column_product_launches": timing.spmm_launches[1],
+             "row_product_bytes": timing.spmm_bytes[0], "column_product_bytes": timing.spmm_bytes[1]}
         return PcaOutput(emb, loadings, variance, ratio, means, int(passes.value), t, dev)
 
     def pca_sketch(self, layer, seed, l, cell_offset=0):
