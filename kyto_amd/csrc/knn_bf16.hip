@@ -282,7 +282,6 @@ __device__ unsigned compact_buffer(Entry *__restrict__ buf, unsigned cnt, unsign
 constexpr int LEAD_TILES = 16;        // tiles scanned before a block's own position
 constexpr uint64_t EST_MIN_ROWS = 1 << 17;  // threshold estimates from a sample above this size
 constexpr uint32_t EST_STRIDE = 16;         // the sample: every 16th row of the scan order
-constexpr uint32_t EST_RANK = 10;           // its 10th smallest key: ~160 rows at or below
 
 constexpr unsigned SCAP = 64;  // staged appends per wavefront
 
@@ -306,10 +305,6 @@ constexpr unsigned SCAP = 64;  // staged appends per wavefront
 // Blocks are mapped to queries XCD by XCD (block b runs on XCD b % 8): the blocks
 // resident on one XCD hold consecutive queries, start their scans at nearby tiles
 // and read the same tiles through that XCD's L2 at about the same time.
-//
-// ESTIMATE only names the instantiation that computes the threshold estimates (the
-// code is the same), so that profiles tell the two scans apart.
-template <bool ESTIMATE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void search_bf16(
     const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q, uint64_t np, uint32_t pstride,
     const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq,
@@ -577,6 +572,146 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     }
 }
 
+// ----------------------------------------------------------- threshold estimate ----
+
+constexpr int EST_SUBSETS = 32;  // per query: 4 lanes x 8 keys of a super-block
+constexpr int EST_PICK = 9;      // the 9th smallest subset minimum
+
+// A starting threshold for each query: the points (every pstride-th row of the scan
+// order) fall into EST_SUBSETS disjoint subsets by their position within a
+// super-block (the lane and register that hold the key), each lane keeps the running
+// minimum of its keys per subset in registers, and the EST_PICK-th smallest of a
+// query's subset minima is its threshold: EST_PICK subsets hold a point at or below
+// it, so at least EST_PICK - 1 rows other than the query itself do. No appends and no
+// compactions: the scan is the MFMA keys and one v_min a key. A starting threshold
+// only steers how much the candidate scan keeps; the candidates do not depend on it.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void estimate_bf16(
+    const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q, uint64_t np, uint32_t pstride,
+    const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq, float *__restrict__ out_thr) {
+    __shared__ __attribute__((aligned(16))) unsigned char tile[3][TILE * ROW_BYTES];
+    const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
+    const unsigned nblk = gridDim.x, xcd = blockIdx.x % 8, per = nblk / 8, extra = nblk % 8;
+    const unsigned lblk = xcd * per + min(xcd, extra) + blockIdx.x / 8;
+    const uint64_t qbase = (uint64_t)lblk * QPB + (uint64_t)wave * QPW;
+    bf16x8 bq[NB][2];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const uint64_t ic = min<uint64_t>(qbase + 16 * b + (lane % 16), nq - 1);
+        const uint32_t r = qpos ? qpos[ic] : (uint32_t)ic;
+        const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Q + (uint64_t)r * KDIM);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) bq[b][ks] = src[4 * ks + lane / 16];
+    }
+    float rmin[NB][4 * MB];
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+        for (int e = 0; e < 4 * MB; e++) rmin[b][e] = INFINITY;
+    const uint64_t n_tiles = (np + TILE - 1) / TILE;
+    const uint64_t q0 = min<uint64_t>((uint64_t)lblk * QPB, nq - 1);
+    const uint64_t own = ((qpos ? (uint64_t)qpos[q0] : q0) / pstride) / TILE;
+    // Tiles as in search_bf16: global_load_lds into three buffers, 16-byte chunks
+    // XOR-swizzled by row; rows past the last point read P's padding rows.
+    auto fetch = [&](uint64_t t0, unsigned char *__restrict__ dst) {
+        const __bf16 *tp = P + t0 * pstride * KDIM;
+#pragma unroll
+        for (int s = 0; s < TILE / 32; s++) {
+            const unsigned row = (wave * (TILE / 32) + s) * 8 + lane / 8;
+            const unsigned chunk = (lane % 8) ^ (row % 8);
+            const __bf16 *src = tp + (row * pstride * KDIM + chunk * 8);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                             (__attribute__((address_space(3))) void *)(dst + (wave * (TILE / 32) + s) * 1024),
+                                             16, 0, 0);
+        }
+    };
+    auto wrap_next = [&](uint32_t t) { return t + 1 == (uint32_t)n_tiles ? 0u : t + 1; };
+    uint32_t tcur = (uint32_t)min<uint64_t>(own, n_tiles - 1), tfetch = wrap_next(tcur);
+    fetch((uint64_t)tcur * TILE, tile[0]);
+    if (n_tiles > 1) fetch((uint64_t)tfetch * TILE, tile[1]);
+    tfetch = wrap_next(tfetch);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __syncthreads();
+    int cur = 0;
+    for (uint64_t it = 0; it < n_tiles; it++) {
+        const int nxt = cur == 2 ? 0 : cur + 1, after = nxt == 2 ? 0 : nxt + 1;
+        auto body = [&](const unsigned char *__restrict__ tb, unsigned char *__restrict__ nx) {
+#pragma nounroll
+            for (int sb = 0; sb < TILE / SPB; sb++) {
+                bf16x8 a0[MB], a1[MB];
+#pragma unroll
+                for (int g = 0; g < MB; g++) {
+                    const unsigned prow = sb * SPB + g * 16 + (lane % 16);
+                    const unsigned char *rp = tb + prow * ROW_BYTES;
+                    a0[g] = *reinterpret_cast<const bf16x8 *>(rp + (((lane / 16) ^ (prow % 8)) * 16));
+                    a1[g] = *reinterpret_cast<const bf16x8 *>(rp + (((4 + lane / 16) ^ (prow % 8)) * 16));
+                }
+                f32x4 z[MB][NB];
+#pragma unroll
+                for (int g = 0; g < MB; g++)
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+                        const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+                        z[g][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[g], bq[b][0], zero, 0, 0, 0);
+                    }
+#pragma unroll
+                for (int g = 0; g < MB; g++)
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+                        const f32x4 r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[g], bq[b][1], z[g][b], 0, 0, 0);
+#pragma unroll
+                        for (int j = 0; j < 4; j++) rmin[b][4 * g + j] = fminf(rmin[b][4 * g + j], r[j]);
+                    }
+            }
+            if (it + 2 < n_tiles) {
+                fetch((uint64_t)tfetch * TILE, nx);
+                __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4): all but tile t + 2
+            } else {
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            }
+        };
+        body(tile[cur], tile[after]);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        cur = nxt;
+        tcur = wrap_next(tcur);
+        tfetch = wrap_next(tfetch);
+    }
+    __syncthreads();
+    // The subset minima through LDS (over the tiles, no longer read): subset s of the
+    // wave's query q at mins[s * (QPW + 1) + q].
+    static_assert(4 * EST_SUBSETS * (QPW + 1) * sizeof(float) <= sizeof(tile), "subset minima fit the tiles");
+    static_assert(EST_SUBSETS == 4 * 4 * MB, "a subset per (lane group, register)");
+    float *mins = reinterpret_cast<float *>(&tile[0][0]) + wave * EST_SUBSETS * (QPW + 1);
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+        for (int e = 0; e < 4 * MB; e++) mins[((lane / 16) * 4 * MB + e) * (QPW + 1) + 16 * b + (lane % 16)] = rmin[b][e];
+    wave_sync();
+    // Lane q: the EST_PICK-th smallest of its query's minima (remove the smallest
+    // EST_PICK - 1 times).
+    float v[EST_SUBSETS];
+#pragma unroll
+    for (int s = 0; s < EST_SUBSETS; s++) v[s] = mins[s * (QPW + 1) + lane];
+    float t = INFINITY;
+#pragma unroll
+    for (int pick = 0; pick < EST_PICK; pick++) {
+        t = INFINITY;
+        int at = 0;
+#pragma unroll
+        for (int s = 0; s < EST_SUBSETS; s++)
+            if (v[s] < t) {
+                t = v[s];
+                at = s;
+            }
+#pragma unroll
+        for (int s = 0; s < EST_SUBSETS; s++)
+            if (s == at) v[s] = INFINITY;
+    }
+    const uint64_t i = qbase + lane;
+    if (i < nq) out_thr[qout[i]] = t;
+}
+
 // ================================================================== host side ====
 
 void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t nq,
@@ -641,28 +776,25 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
         qpos = qpos_buf.ptr;
         qout = qout_buf.ptr;
     }
-    // Phase 1 (large inputs): each query's EST_RANK-th smallest key over every
-    // EST_STRIDE-th row of the scan order, an estimate of its final threshold that
-    // about EST_RANK * EST_STRIDE rows fall below. Phase 2 starts every query at that
-    // threshold, so its buffer takes only the rows at or below it (any starting
-    // threshold gives the same candidates; a low one only leaves fewer than m).
+    // Phase 1 (large inputs): each query's starting threshold from the subset minima
+    // over every EST_STRIDE-th row of the scan order (estimate_bf16). Phase 2 starts
+    // every query at that threshold, so its buffer takes only the rows at or below it
+    // (any starting threshold gives the same candidates; a low one only leaves fewer
+    // than m).
     DevBuf<Entry> buf(nq * BCAP);
     DevBuf<float> thr_est;
-    DevBuf<uint32_t> cnt_est;
     const unsigned blocks = div_up(nq, QPB);
     if (n >= EST_MIN_ROWS) {
         thr_est.alloc(nq);
-        cnt_est.alloc(nq);
         const uint64_t np = div_up(n, EST_STRIDE);
-        search_bf16<true><<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, perm.ptr, EST_RANK, nullptr,
-                                            buf.ptr, nullptr, nullptr, cnt_est.ptr, thr_est.ptr);
+        estimate_bf16<<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, thr_est.ptr);
         KY_HIP(hipGetLastError());
     }
     hipEvent_t ev[2];
     KY_HIP(hipEventCreate(&ev[0]));
     KY_HIP(hipEventCreate(&ev[1]));
     KY_HIP(hipEventRecord(ev[0], st));
-    search_bf16<false><<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, n, 1, qpos, qout, nq, perm.ptr, m, thr_est.ptr, buf.ptr,
+    search_bf16<<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, n, 1, qpos, qout, nq, perm.ptr, m, thr_est.ptr, buf.ptr,
                                                d_rows, d_keys, d_count, d_thr);
     KY_HIP(hipGetLastError());
     KY_HIP(hipEventRecord(ev[1], st));
@@ -678,7 +810,7 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
         KY_HIP(hipMemcpyAsync(&h_n, re_n.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         KY_HIP(hipStreamSynchronize(st));
         if (h_n) {
-            search_bf16<false><<<div_up(h_n, QPB), 256, 0, st>>>(P.ptr, Q.ptr, n, 1, re_pos.ptr, re_out.ptr, h_n, perm.ptr, m,
+            search_bf16<<<div_up(h_n, QPB), 256, 0, st>>>(P.ptr, Q.ptr, n, 1, re_pos.ptr, re_out.ptr, h_n, perm.ptr, m,
                                                          nullptr, buf.ptr, d_rows, d_keys, d_count, d_thr);
             KY_HIP(hipGetLastError());
         }

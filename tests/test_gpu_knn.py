@@ -115,3 +115,28 @@ def test_nearest_keys_are_the_smallest_approximate_keys(device):
         d[q] = np.inf
         true = np.sort(d)[:20]
         assert np.abs(keys[i].astype(np.float64) - true).max() < 1e-3 * (1 + true.max())
+
+
+def _large_clustered(n, d, clusters, seed):
+    rng = np.random.default_rng(seed)
+    centres = rng.standard_normal((clusters, d)).astype(np.float32) * 3
+    lab = rng.integers(0, clusters, n)
+    return (centres[lab] + rng.standard_normal((n, d)).astype(np.float32)).astype(np.float32)
+
+
+def test_large_input_threshold_estimate(device):
+    # Above 2^17 rows the BF16 scan starts every query at a sampled threshold: the graph
+    # must still be the FP32 path's (kyto's exact graph), and the oracle's on sampled rows.
+    pts = _large_clustered(140_000, 50, 12, seed=21)
+    g16 = _exact(device, pts, 15, "bf16")
+    g32 = _exact(device, pts, 15, "fp32")
+    assert np.array_equal(g16.rows(), g32.rows())
+    assert np.array_equal(g16.distances.view(np.uint32), g32.distances.view(np.uint32))
+    first, count = 70_000, 300
+    keys, rows, _ = oracle.knn_exact_range(pts, 14, first, count)
+    o = np.argsort(rows, axis=1)
+    want_rows = np.take_along_axis(rows, o, 1)
+    want_dist = np.sqrt(np.take_along_axis(keys, o, 1)).astype(np.float32)
+    got = g16.rows()[first:first + count]
+    assert np.array_equal(got, want_rows)
+    assert np.array_equal(g16.distances.reshape(-1, 14)[first:first + count], want_dist)
