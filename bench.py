@@ -66,6 +66,24 @@ def cpu_baseline(layer, embedding, n_genes, q, threads):
     }
 
 
+def copy_calibration(nbytes=256 << 20):
+    """One plain device-to-device copy of `nbytes` through the HIP runtime (its copy
+    kernel reads and writes exactly nbytes): profiles divide the HBM counters of the
+    path's kernels by what the counters read for this copy."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copy calibration: {nbytes} bytes device to device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +97,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--train-queries", type=int, default=200_000)
     ap.add_argument("--confidence", type=float, default=0.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with one device-to-device copy of 256 MiB (the HBM counters' reference kernel)")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the recall check and the plain FP32 kNN stage timed beside the learned path")
     ap.add_argument("--q", type=int, default=4)
@@ -214,6 +234,8 @@ def main():
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(layer, embedding, w.n_genes, args.q, threads)
     print(json.dumps(out), flush=True)
+    if args.copy_calibration:
+        copy_calibration()
 
 
 if __name__ == "__main__":

@@ -16,11 +16,15 @@
 namespace ky {
 
 static thread_local std::string g_last_error;
+// The stream of the context the current call works on (set by use_device): every call
+// synchronises it before it returns, so no work of a call is pending afterwards (the
+// caching allocator relies on it, memory.cpp).
+static thread_local hipStream_t t_call_stream = nullptr;
 
 void set_last_error(const std::string &message) { g_last_error = message; }
 
 template <typename F>
-static int guarded(F &&f) {
+static int run_guarded(F &&f) {
     try {
         f();
         g_last_error.clear();
@@ -40,11 +44,28 @@ static int guarded(F &&f) {
     }
 }
 
+template <typename F>
+static int guarded(F &&f) {
+    int status = run_guarded(f);
+    if (t_call_stream) {
+        hipError_t e = hipStreamSynchronize(t_call_stream);
+        t_call_stream = nullptr;
+        if (e != hipSuccess && status == KY_OK) {
+            g_last_error = std::string("device failure in hipStreamSynchronize: ") + hipGetErrorString(e);
+            status = KY_E505;
+        }
+    }
+    return status;
+}
+
 static void need(const void *p, const char *what) {
     if (!p) throw Error(KY_E215, std::string(what) + " is null");
 }
 
-static void use_device(const Context &ctx) { KY_HIP(hipSetDevice(ctx.device)); }
+static void use_device(const Context &ctx) {
+    KY_HIP(hipSetDevice(ctx.device));
+    t_call_stream = ctx.stream;
+}
 
 }  // namespace ky
 
@@ -93,6 +114,7 @@ int ky_close(ky_ctx *ctx) {
         hipStream_t st = ctx->stream;
         delete ctx;
         if (st) (void)hipStreamDestroy(st);
+        device_cache_trim();
     });
 }
 
@@ -176,9 +198,7 @@ int ky_dense_download(const ky_dense *m, float *host) {
         need(m, "matrix");
         need(host, "host");
         use_device(*m->ctx);
-        KY_HIP(hipMemcpyAsync(host, m->data.ptr, sizeof(float) * m->rows * m->cols, hipMemcpyDeviceToHost,
-                              m->ctx->stream));
-        KY_HIP(hipStreamSynchronize(m->ctx->stream));
+        download(*m->ctx, host, m->data.ptr, sizeof(float) * m->rows * m->cols);
     });
 }
 

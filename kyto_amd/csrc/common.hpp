@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <mutex>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -36,6 +37,11 @@ void set_last_error(const std::string &message);
         if (!(cond)) throw ::ky::Error(code, msg);  \
     } while (0)
 
+// The caching allocator under DevBuf (memory.cpp).
+void *device_alloc(size_t bytes);
+void device_free(void *p);
+void device_cache_trim();
+
 // A device allocation owned by one object.
 template <typename T>
 struct DevBuf {
@@ -54,14 +60,14 @@ struct DevBuf {
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) KY_HIP(hipMalloc(reinterpret_cast<void **>(&ptr), sizeof(T) * count));
+        if (count) ptr = static_cast<T *>(device_alloc(sizeof(T) * count));
     }
     // Grow only (keeps the allocation when it is already large enough).
     void reserve(size_t count) {
         if (count > n) alloc(count);
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) device_free(ptr);
         ptr = nullptr;
         n = 0;
     }
@@ -80,7 +86,19 @@ struct Context {
     DevBuf<double> dense_f64;
     DevBuf<float> wbuf;
     DevBuf<double> sums;
+    // Pinned staging for downloads (memory.cpp) and the events of its two halves.
+    void *staging = nullptr;
+    hipEvent_t staged[2] = {nullptr, nullptr};
+    ~Context() {
+        if (staged[0]) (void)hipEventDestroy(staged[0]);
+        if (staged[1]) (void)hipEventDestroy(staged[1]);
+        if (staging) (void)hipHostFree(staging);
+    }
 };
+
+// Device-to-host copy of `bytes` into pageable memory through the pinned staging
+// buffer; returns once the data is in dst.
+void download(Context &ctx, void *dst, const void *src, size_t bytes);
 
 struct Csr {
     Context *ctx = nullptr;

@@ -629,9 +629,8 @@ void neighbors_run(Context &ctx, const Dense &points, uint32_t n_components, uin
     DevBuf<uint32_t> g_idx(n * k);
     DevBuf<float> g_dist(n * k);
     knn_graph(ctx, keys.ptr, rows.ptr, n, k, g_idx.ptr, g_dist.ptr);
-    KY_HIP(hipMemcpyAsync(h_idx, g_idx.ptr, sizeof(uint32_t) * n * k, hipMemcpyDeviceToHost, st));
-    KY_HIP(hipMemcpyAsync(h_dist, g_dist.ptr, sizeof(float) * n * k, hipMemcpyDeviceToHost, st));
-    KY_HIP(hipStreamSynchronize(st));
+    download(ctx, h_idx, g_idx.ptr, sizeof(uint32_t) * n * k);
+    download(ctx, h_dist, g_dist.ptr, sizeof(float) * n * k);
     if (info) info->total_ms = now_ms() - t_start;
 }
 

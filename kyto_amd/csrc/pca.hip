@@ -599,10 +599,8 @@ void pca_project(Csr &csr, const double *v, uint32_t k, const double *mean_proje
         KY_HIP(hipMemcpyAsync(squares, ctx.reduced.ptr, sizeof(double) * csr.n_cols, hipMemcpyDeviceToHost, st));
         KY_HIP(hipMemcpyAsync(counts.data(), ctx.reduced.ptr + csr.n_cols, sizeof(double) * csr.n_cols,
                               hipMemcpyDeviceToHost, st));
-        if (host_embedding)
-            KY_HIP(hipMemcpyAsync(host_embedding, emb->data.ptr, sizeof(float) * csr.n_rows * k,
-                                  hipMemcpyDeviceToHost, st));
         KY_HIP(hipStreamSynchronize(st));
+        if (host_embedding) download(ctx, host_embedding, emb->data.ptr, sizeof(float) * csr.n_rows * k);
         for (uint32_t g = 0; g < csr.n_cols; g++) stored[g] = (uint64_t)counts[g];
     } catch (...) {
         delete static_cast<ky_dense *>(emb);
