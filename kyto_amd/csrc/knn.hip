@@ -109,10 +109,15 @@ __global__ void pack_columns(const float *__restrict__ x, uint64_t n, uint32_t d
 // original rows `a` (d floats a row); keys kept as f64 bits (non-negative).
 // Queries: q_list[i] when q_list is not null, else q_first + i, for i < q_count.
 // Output per query i: `keep` (key, row) pairs ascending and the count.
+//
+// With n_seg > 1 the points are cut into n_seg segments of whole tiles: block b scans
+// segment b % n_seg for the queries of group b / n_seg, and writes its `keep` pairs at
+// slot i * n_seg + segment (merge_segments then keeps each query's smallest), so a
+// short list of queries still fills the device.
 template <bool EXACT, int DP, int CAP>
 __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, const float *__restrict__ offs,
                                                    uint64_t n, uint32_t d, const uint32_t *__restrict__ q_list,
-                                                   uint64_t q_first, uint64_t q_count, uint32_t keep,
+                                                   uint64_t q_first, uint64_t q_count, uint32_t keep, uint32_t n_seg,
                                                    uint32_t *__restrict__ out_rows, void *__restrict__ out_keys,
                                                    uint32_t *__restrict__ out_count) {
     using K = typename std::conditional<EXACT, unsigned long long, uint32_t>::type;
@@ -124,7 +129,10 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
     __shared__ uint32_t bcnt[QB], trow[QB], qrow[QB];
     __shared__ K tkey[QB];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
-    const uint64_t i0 = (uint64_t)blockIdx.x * QB;
+    const uint64_t i0 = (uint64_t)(blockIdx.x / n_seg) * QB;
+    const unsigned seg = blockIdx.x % n_seg;
+    const uint64_t seg_len = ((n + n_seg - 1) / n_seg + TP - 1) / TP * TP;
+    const uint64_t p_begin = min<uint64_t>((uint64_t)seg * seg_len, n), p_end = min<uint64_t>(p_begin + seg_len, n);
     const unsigned stride = EXACT ? d : DP;
     if (tid < QB) {
         uint64_t i = i0 + tid;
@@ -148,9 +156,9 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
     unsigned nq = 0;  // this wave's live queries
     for (unsigned qi = 0; qi < 16; qi++)
         if (qrow[wave * 16 + qi] != 0xFFFFFFFFu) nq = qi + 1;
-    for (uint64_t t0 = 0; t0 < n; t0 += TP) {
+    for (uint64_t t0 = p_begin; t0 < p_end; t0 += TP) {
         const uint64_t p = t0 + lane;
-        const bool valid = p < n;
+        const bool valid = p < p_end;
         float pr[DP];
         if (valid) {
             if constexpr (!EXACT) {
@@ -228,8 +236,9 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
         uint32_t *br = brow + ql * CAP;
         compact<K, CAP>(bk, br, &bcnt[ql], &tkey[ql], &trow[ql], keep);
         const unsigned cnt = bcnt[ql];
-        const uint64_t i = i0 + ql;
-        // Output slot: the query's position in the list (or range).
+        const uint64_t i = (i0 + ql) * n_seg + seg;
+        // Output slot: the query's position in the list (or range), times n_seg, plus
+        // the segment.
         for (unsigned e = lane; e < keep; e += 64) {
             uint32_t row = e < cnt ? br[e] : 0xFFFFFFFFu;
             out_rows[i * keep + e] = row;
@@ -240,6 +249,38 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
         }
         if (lane == 0) out_count[i] = cnt;
     }
+}
+
+// Per query (one wavefront): the smallest `keep` (key, row) pairs of its n_seg
+// segments' lists (f64 keys, ascending), in order. N (a power of two, at least
+// n_seg * keep) entries of LDS a wave.
+template <int N>
+__global__ __launch_bounds__(64) void merge_segments(const uint32_t *__restrict__ seg_rows,
+                                                     const double *__restrict__ seg_keys,
+                                                     const uint32_t *__restrict__ seg_count, uint64_t q_count,
+                                                     uint32_t n_seg, uint32_t keep, uint32_t *__restrict__ out_rows,
+                                                     double *__restrict__ out_keys, uint32_t *__restrict__ out_count) {
+    __shared__ unsigned long long key[N];
+    __shared__ uint32_t row[N];
+    const uint64_t q = blockIdx.x;
+    if (q >= q_count) return;
+    const unsigned lane = lane_id();
+    for (unsigned e = lane; e < N; e += 64) {
+        const unsigned s = e / keep, j = e % keep;
+        bool ok = s < n_seg && j < seg_count[q * n_seg + s];
+        key[e] = ok ? (unsigned long long)__double_as_longlong(seg_keys[(q * n_seg + s) * keep + j]) : ~0ull;
+        row[e] = ok ? seg_rows[(q * n_seg + s) * keep + j] : ~0u;
+    }
+    wave_sync();
+    wave_bitonic<unsigned long long>(key, row, N);
+    unsigned total = 0;
+    for (unsigned s = 0; s < n_seg; s++) total += seg_count[q * n_seg + s];
+    const unsigned cnt = min(total, keep);
+    for (unsigned e = lane; e < keep; e += 64) {
+        out_rows[q * keep + e] = e < cnt ? row[e] : 0xFFFFFFFFu;
+        out_keys[q * keep + e] = e < cnt ? __longlong_as_double((long long)key[e]) : INFINITY;
+    }
+    if (lane == 0) out_count[q] = cnt;
 }
 
 // ------------------------------------------------------------------- re-rank ----
@@ -371,10 +412,10 @@ __global__ void graph_rows(const double *__restrict__ keys, const uint32_t *__re
 template <bool EXACT, int CAP>
 static void launch_scan(int dp, hipStream_t st, unsigned blocks, const float *a, const float *offs, uint64_t n,
                         uint32_t d, const uint32_t *list, uint64_t q_first, uint64_t q_count, uint32_t keep,
-                        uint32_t *out_rows, void *out_keys, uint32_t *out_count) {
-#define KY_SCAN(DPV)                                                                                          \
-    scan_points<EXACT, DPV, CAP><<<blocks, 256, 0, st>>>(a, offs, n, d, list, q_first, q_count, keep, out_rows, \
-                                                         out_keys, out_count)
+                        uint32_t *out_rows, void *out_keys, uint32_t *out_count, uint32_t n_seg = 1) {
+#define KY_SCAN(DPV)                                                                                   \
+    scan_points<EXACT, DPV, CAP><<<blocks * n_seg, 256, 0, st>>>(a, offs, n, d, list, q_first, q_count, keep, \
+                                                                 n_seg, out_rows, out_keys, out_count)
     switch (dp) {
         case 8: KY_SCAN(8); break;
         case 16: KY_SCAN(16); break;
@@ -454,8 +495,22 @@ void knn_exact_rows(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint
     DevBuf<uint32_t> rows_tmp(count * k), cnt_tmp(count);
     DevBuf<double> keys_tmp(count * k);
     const unsigned blocks = div_up(count, QB);
-    launch_scan<true, 128>(padded_width(d), st, blocks, pts, nullptr, n, d, d_list, 0, count, k, rows_tmp.ptr,
-                           keys_tmp.ptr, cnt_tmp.ptr);
+    // Segments of the points: enough blocks to fill the device (about 2048), each
+    // segment at least 16 tiles, n_seg * k within the merge's 1024 entries.
+    uint32_t n_seg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(2048, blocks), n / (16 * TP)));
+    n_seg = std::min<uint32_t>(n_seg, 1024 / k);
+    if (n_seg <= 1) {
+        launch_scan<true, 128>(padded_width(d), st, blocks, pts, nullptr, n, d, d_list, 0, count, k, rows_tmp.ptr,
+                               keys_tmp.ptr, cnt_tmp.ptr);
+    } else {
+        DevBuf<uint32_t> seg_rows(count * n_seg * k), seg_cnt(count * n_seg);
+        DevBuf<double> seg_keys(count * n_seg * k);
+        launch_scan<true, 128>(padded_width(d), st, blocks, pts, nullptr, n, d, d_list, 0, count, k, seg_rows.ptr,
+                               seg_keys.ptr, seg_cnt.ptr, n_seg);
+        merge_segments<1024><<<(unsigned)count, 64, 0, st>>>(seg_rows.ptr, seg_keys.ptr, seg_cnt.ptr, count, n_seg,
+                                                              k, rows_tmp.ptr, keys_tmp.ptr, cnt_tmp.ptr);
+        KY_HIP(hipGetLastError());
+    }
     scatter_exact<<<div_up(count * k, 256), 256, 0, st>>>(d_list, count, k, rows_tmp.ptr, keys_tmp.ptr, d_keys, d_rows);
     KY_HIP(hipGetLastError());
 }

@@ -368,7 +368,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const uint64_t q0 = min<uint64_t>((uint64_t)lblk * QPB, nq - 1);
     const uint64_t own = ((qpos ? (uint64_t)qpos[q0] : q0) / pstride) / TILE;
     const uint64_t lead = min<uint64_t>(LEAD_TILES, n_tiles - 1);
-    const uint64_t first_tile = (min<uint64_t>(own, n_tiles - 1) + n_tiles - lead) % n_tiles;
+    // Without a starting threshold a block starts a few tiles before its own queries'
+    // position (their near neighbours first tighten the thresholds early); with one,
+    // every block starts at tile 0, so the blocks resident on an XCD read the same
+    // tiles at about the same time through its L2.
+    const uint64_t first_tile = thr_init ? 0 : (min<uint64_t>(own, n_tiles - 1) + n_tiles - lead) % n_tiles;
     // Tiles go straight from global memory to LDS (global_load_lds_dwordx4) into three
     // buffers: tile t + 2 is requested once tile t's keys are done, so it has the
     // whole of tile t + 1 to arrive. A tile is 128 rows of 128 bytes with the 16-byte
@@ -545,17 +549,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     for (unsigned ql = 0; ql < QPW; ql++) {
         const uint64_t i = qbase + ql;
         if (i >= nq) break;
-        const int b = (int)(ql / 16);
-        unsigned kept;
-        // b is uniform; the switch keeps the register arrays statically indexed.
-        switch (b) {
-            case 0: kept = compact_query(0, ql % 16, true, stage_key[wave], stage_row[wave]); break;
-            case 1: kept = compact_query(1, ql % 16, true, stage_key[wave], stage_row[wave]); break;
-            case 2: kept = compact_query(2, ql % 16, true, stage_key[wave], stage_row[wave]); break;
-            default: kept = compact_query(3, ql % 16, true, stage_key[wave], stage_row[wave]); break;
-        }
-        const float t = __int_as_float(__builtin_amdgcn_readlane(
-            __float_as_int(b == 0 ? thr[0] : b == 1 ? thr[1] : b == 2 ? thr[2] : thr[3]), ql % 16));
+        unsigned kept = 0;
+        float t = 0.0f;
+        // ql / 16 is uniform; the unrolled test keeps the register arrays statically indexed.
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+            if ((unsigned)b == ql / 16) {
+                kept = compact_query(b, ql % 16, true, stage_key[wave], stage_row[wave]);
+                t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr[b]), ql % 16));
+            }
         const uint64_t o = qout[i];
         if (out_rows) {
             for (unsigned e = lane; e < m; e += 64) {
@@ -574,8 +576,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 
 // ----------------------------------------------------------- threshold estimate ----
 
-constexpr int EST_SUBSETS = 32;  // per query: 4 lanes x 8 keys of a super-block
-constexpr int EST_PICK = 9;      // the 9th smallest subset minimum
+constexpr int EST_NB = 4;                     // column blocks (16 queries each) per wavefront
+constexpr int EST_MB = 2;                     // row blocks (16 points each) per super-block
+constexpr int EST_SPB = 16 * EST_MB;          // points per super-block
+constexpr int EST_QPW = 16 * EST_NB;          // queries per wavefront
+constexpr int EST_QPB = 4 * EST_QPW;          // queries per block
+constexpr int EST_SUBSETS = 4 * 4 * EST_MB;   // per query: 4 lanes x 4 EST_MB keys of a super-block
+constexpr int EST_PICK = 9;                   // the 9th smallest subset minimum
 
 // A starting threshold for each query: the points (every pstride-th row of the scan
 // order) fall into EST_SUBSETS disjoint subsets by their position within a
@@ -592,23 +599,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
     const unsigned nblk = gridDim.x, xcd = blockIdx.x % 8, per = nblk / 8, extra = nblk % 8;
     const unsigned lblk = xcd * per + min(xcd, extra) + blockIdx.x / 8;
-    const uint64_t qbase = (uint64_t)lblk * QPB + (uint64_t)wave * QPW;
-    bf16x8 bq[NB][2];
+    const uint64_t qbase = (uint64_t)lblk * EST_QPB + (uint64_t)wave * EST_QPW;
+    bf16x8 bq[EST_NB][2];
 #pragma unroll
-    for (int b = 0; b < NB; b++) {
+    for (int b = 0; b < EST_NB; b++) {
         const uint64_t ic = min<uint64_t>(qbase + 16 * b + (lane % 16), nq - 1);
         const uint32_t r = qpos ? qpos[ic] : (uint32_t)ic;
         const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Q + (uint64_t)r * KDIM);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) bq[b][ks] = src[4 * ks + lane / 16];
     }
-    float rmin[NB][4 * MB];
+    float rmin[EST_NB][4 * EST_MB];
 #pragma unroll
-    for (int b = 0; b < NB; b++)
+    for (int b = 0; b < EST_NB; b++)
 #pragma unroll
-        for (int e = 0; e < 4 * MB; e++) rmin[b][e] = INFINITY;
+        for (int e = 0; e < 4 * EST_MB; e++) rmin[b][e] = INFINITY;
     const uint64_t n_tiles = (np + TILE - 1) / TILE;
-    const uint64_t q0 = min<uint64_t>((uint64_t)lblk * QPB, nq - 1);
+    const uint64_t q0 = min<uint64_t>((uint64_t)lblk * EST_QPB, nq - 1);
     const uint64_t own = ((qpos ? (uint64_t)qpos[q0] : q0) / pstride) / TILE;
     // Tiles as in search_bf16: global_load_lds into three buffers, 16-byte chunks
     // XOR-swizzled by row; rows past the last point read P's padding rows.
@@ -636,27 +643,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         const int nxt = cur == 2 ? 0 : cur + 1, after = nxt == 2 ? 0 : nxt + 1;
         auto body = [&](const unsigned char *__restrict__ tb, unsigned char *__restrict__ nx) {
 #pragma nounroll
-            for (int sb = 0; sb < TILE / SPB; sb++) {
-                bf16x8 a0[MB], a1[MB];
+            for (int sb = 0; sb < TILE / EST_SPB; sb++) {
+                bf16x8 a0[EST_MB], a1[EST_MB];
 #pragma unroll
-                for (int g = 0; g < MB; g++) {
-                    const unsigned prow = sb * SPB + g * 16 + (lane % 16);
+                for (int g = 0; g < EST_MB; g++) {
+                    const unsigned prow = sb * EST_SPB + g * 16 + (lane % 16);
                     const unsigned char *rp = tb + prow * ROW_BYTES;
                     a0[g] = *reinterpret_cast<const bf16x8 *>(rp + (((lane / 16) ^ (prow % 8)) * 16));
                     a1[g] = *reinterpret_cast<const bf16x8 *>(rp + (((4 + lane / 16) ^ (prow % 8)) * 16));
                 }
-                f32x4 z[MB][NB];
+                f32x4 z[EST_MB][EST_NB];
 #pragma unroll
-                for (int g = 0; g < MB; g++)
+                for (int g = 0; g < EST_MB; g++)
 #pragma unroll
-                    for (int b = 0; b < NB; b++) {
+                    for (int b = 0; b < EST_NB; b++) {
                         const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
                         z[g][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[g], bq[b][0], zero, 0, 0, 0);
                     }
 #pragma unroll
-                for (int g = 0; g < MB; g++)
+                for (int g = 0; g < EST_MB; g++)
 #pragma unroll
-                    for (int b = 0; b < NB; b++) {
+                    for (int b = 0; b < EST_NB; b++) {
                         const f32x4 r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[g], bq[b][1], z[g][b], 0, 0, 0);
 #pragma unroll
                         for (int j = 0; j < 4; j++) rmin[b][4 * g + j] = fminf(rmin[b][4 * g + j], r[j]);
@@ -679,20 +686,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     }
     __syncthreads();
     // The subset minima through LDS (over the tiles, no longer read): subset s of the
-    // wave's query q at mins[s * (QPW + 1) + q].
-    static_assert(4 * EST_SUBSETS * (QPW + 1) * sizeof(float) <= sizeof(tile), "subset minima fit the tiles");
-    static_assert(EST_SUBSETS == 4 * 4 * MB, "a subset per (lane group, register)");
-    float *mins = reinterpret_cast<float *>(&tile[0][0]) + wave * EST_SUBSETS * (QPW + 1);
+    // wave's query q at mins[s * (EST_QPW + 1) + q].
+    static_assert(4 * EST_SUBSETS * (EST_QPW + 1) * sizeof(float) <= sizeof(tile), "subset minima fit the tiles");
+    float *mins = reinterpret_cast<float *>(&tile[0][0]) + wave * EST_SUBSETS * (EST_QPW + 1);
 #pragma unroll
-    for (int b = 0; b < NB; b++)
+    for (int b = 0; b < EST_NB; b++)
 #pragma unroll
-        for (int e = 0; e < 4 * MB; e++) mins[((lane / 16) * 4 * MB + e) * (QPW + 1) + 16 * b + (lane % 16)] = rmin[b][e];
+        for (int e = 0; e < 4 * EST_MB; e++) mins[((lane / 16) * 4 * EST_MB + e) * (EST_QPW + 1) + 16 * b + (lane % 16)] = rmin[b][e];
     wave_sync();
     // Lane q: the EST_PICK-th smallest of its query's minima (remove the smallest
     // EST_PICK - 1 times).
     float v[EST_SUBSETS];
 #pragma unroll
-    for (int s = 0; s < EST_SUBSETS; s++) v[s] = mins[s * (QPW + 1) + lane];
+    for (int s = 0; s < EST_SUBSETS; s++) v[s] = mins[s * (EST_QPW + 1) + lane];
     float t = INFINITY;
 #pragma unroll
     for (int pick = 0; pick < EST_PICK; pick++) {
@@ -787,7 +793,7 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
     if (n >= EST_MIN_ROWS) {
         thr_est.alloc(nq);
         const uint64_t np = div_up(n, EST_STRIDE);
-        estimate_bf16<<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, thr_est.ptr);
+        estimate_bf16<<<div_up(nq, EST_QPB), 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, thr_est.ptr);
         KY_HIP(hipGetLastError());
     }
     hipEvent_t ev[2];
