@@ -31,6 +31,8 @@ HIP_FLAGS = [
 # MFMA kernels keep their accumulators in ordinary VGPRs (no accumulator-register
 # copies before the epilogue reads them).
 # knn_bf16.hip compares keys that are never NaN: no NaN quieting before v_min_f32.
+# Host code (the dense FP64 steps) for AVX2 hosts; no contraction there either.
+HOST_FLAGS = {"dense.cpp": ["-Xarch_host", "-march=x86-64-v3"]}
 MFMA_FLAGS = {
     "knn_bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1", "-fno-honor-nans", "-mno-amdgpu-ieee"],
     "router.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
@@ -49,7 +51,7 @@ def _compile(src: str) -> str:
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     path = os.path.join(CSRC, src)
     # .cpp sources hold host code only; hipcc compiles them with the HIP headers.
-    cmd = [HIPCC, *HIP_FLAGS, *MFMA_FLAGS.get(src, []), "-c", path, "-o", out]
+    cmd = [HIPCC, *HIP_FLAGS, *MFMA_FLAGS.get(src, []), *HOST_FLAGS.get(src, []), "-c", path, "-o", out]
     _run(cmd)
     return out
 

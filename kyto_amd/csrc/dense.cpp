@@ -5,39 +5,142 @@
 // - thin SVD: Householder QR, then one-sided Jacobi (Hestenes) on the small R;
 // - symmetric eigenpairs: Householder tridiagonalisation and the implicit QL method
 //   with Wilkinson-style shifts (the EISPACK tred2/tql2 pair).
-// Products are cut over a fixed number of row groups run on std::threads; each
-// output element is summed in one fixed order, so the bits do not depend on the
-// thread count.
+// Products, reflections and rotations are cut over a persistent pool of worker
+// threads; each output element is computed in one fixed order, so the bits do not
+// depend on the thread count.
 #include "dense.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 
 namespace ky {
 
-static unsigned worker_count() {
-    unsigned n = std::thread::hardware_concurrency();
-    return std::max(1u, std::min(n, 32u));
-}
+// A persistent pool of worker threads for the dense steps. run(tasks, f) calls f(t)
+// for every t in [0, tasks), task t on worker t % size() (the caller is worker 0), and
+// returns when every task is done; each output element is written by one task in a
+// fixed order, so the bits do not depend on the number of workers. Workers spin for a
+// short while between runs (the steps of a factorisation come microseconds apart),
+// then sleep. One run at a time (a mutex serialises callers).
+class Pool {
+  public:
+    static Pool &get() {
+        static Pool pool;
+        return pool;
+    }
+    unsigned size() const { return workers_ + 1; }
+    template <typename F>
+    void run(size_t tasks, F &&f) {
+        if (tasks == 0) return;
+        if (workers_ == 0 || tasks == 1) {
+            for (size_t t = 0; t < tasks; t++) f(t);
+            return;
+        }
+        std::lock_guard<std::mutex> serial(run_mutex_);
+        job_ = [&f](size_t t) { f(t); };
+        tasks_ = tasks;
+        done_.store(0, std::memory_order_relaxed);
+        {
+            std::lock_guard<std::mutex> lock(sleep_mutex_);
+            epoch_.fetch_add(1, std::memory_order_release);
+        }
+        sleep_cv_.notify_all();
+        for (size_t t = 0; t < tasks; t += size()) job_(t);
+        while (done_.load(std::memory_order_acquire) != workers_) spin_pause();
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lock(sleep_mutex_);
+            stop_ = true;
+            epoch_.fetch_add(1, std::memory_order_release);
+        }
+        sleep_cv_.notify_all();
+        for (auto &t : threads_) t.join();
+    }
 
+  private:
+    Pool() {
+        unsigned n = std::thread::hardware_concurrency();
+        workers_ = std::max(1u, std::min(n, 32u)) - 1;
+        for (unsigned w = 1; w <= workers_; w++) threads_.emplace_back([this, w] { loop(w); });
+    }
+    static void spin_pause() { __builtin_ia32_pause(); }
+    void loop(unsigned w) {
+        uint64_t seen = 0;
+        for (;;) {
+            uint64_t e = epoch_.load(std::memory_order_acquire);
+            for (int spin = 0; e == seen && spin < 200000; spin++) {
+                spin_pause();
+                e = epoch_.load(std::memory_order_acquire);
+            }
+            if (e == seen) {
+                std::unique_lock<std::mutex> lock(sleep_mutex_);
+                sleep_cv_.wait(lock, [&] { return epoch_.load(std::memory_order_acquire) != seen; });
+                e = epoch_.load(std::memory_order_acquire);
+            }
+            seen = e;
+            if (stop_) return;
+            for (size_t t = w; t < tasks_; t += size()) job_(t);
+            done_.fetch_add(1, std::memory_order_acq_rel);
+        }
+    }
+    unsigned workers_ = 0;
+    std::vector<std::thread> threads_;
+    std::function<void(size_t)> job_;
+    size_t tasks_ = 0;
+    std::atomic<uint64_t> epoch_{0};
+    std::atomic<unsigned> done_{0};
+    bool stop_ = false;
+    std::mutex run_mutex_, sleep_mutex_;
+    std::condition_variable sleep_cv_;
+};
+
+// f(begin, end) over [0, n) in contiguous ranges, one a worker (serially below 64).
 template <typename F>
 static void parallel_for(size_t n, F &&f) {
-    unsigned t = worker_count();
-    if (n < 64 || t == 1) {
+    Pool &pool = Pool::get();
+    const size_t w = pool.size();
+    if (n < 64 || w == 1) {
         f(size_t(0), n);
         return;
     }
-    t = static_cast<unsigned>(std::min<size_t>(t, n / 16 + 1));
-    std::vector<std::thread> threads;
-    size_t per = (n + t - 1) / t;
-    for (unsigned i = 0; i < t; i++) {
-        size_t a = i * per, b = std::min(n, a + per);
-        if (a >= b) break;
-        threads.emplace_back([&, a, b] { f(a, b); });
+    const size_t per = (n + w - 1) / w;
+    pool.run(w, [&](size_t t) {
+        const size_t a = t * per, b = std::min(n, a + per);
+        if (a < b) f(a, b);
+    });
+}
+
+// f(begin, end) over [0, n) when the work (n items of `cost` flops) is worth sharing
+// out, else serially. Range bounds are multiples of `align` items (8 doubles: no two
+// workers write one cache line).
+template <typename F>
+static void parallel_if(size_t n, size_t cost, F &&f, size_t align = 1) {
+    if (n * cost < (size_t(1) << 15) || n < 2) {
+        f(size_t(0), n);
+        return;
     }
-    for (auto &th : threads) th.join();
+    Pool &pool = Pool::get();
+    size_t per = (n + pool.size() - 1) / pool.size();
+    per = (per + align - 1) / align * align;
+    const size_t w = (n + per - 1) / per;
+    pool.run(w, [&](size_t t) {
+        const size_t a = t * per, b = std::min(n, a + per);
+        if (a < b) f(a, b);
+    });
+}
+
+// sqrt(a^2 + b^2): directly when neither square can overflow or lose all its bits,
+// else std::hypot.
+static inline double safe_hypot(double a, double b) {
+    const double m = std::max(std::fabs(a), std::fabs(b));
+    if (m < 1e150 && m > 1e-150) return std::sqrt(a * a + b * b);
+    return std::hypot(a, b);
 }
 
 // Dot product in four interleaved partial sums (a fixed order: the same bits on every
@@ -138,11 +241,13 @@ static void householder_qr(const Mat &a, Mat &q, Mat &r) {
         const double beta = vtv > 0 ? 2.0 / vtv : 0.0;
         betas[j] = beta;
         // Apply (I - beta v v^T) to columns j..n-1 (rows j..m-1).
-        for (size_t c = j; c < n; c++) {
-            double *cc = &col[c * m];
-            const double sdot = beta * dot4(v + j, cc + j, m - j);
-            for (size_t i = j; i < m; i++) cc[i] -= sdot * v[i];
-        }
+        parallel_if(n - j, 4 * (m - j), [&](size_t c0, size_t c1) {
+            for (size_t c = j + c0; c < j + c1; c++) {
+                double *cc = &col[c * m];
+                const double sdot = beta * dot4(v + j, cc + j, m - j);
+                for (size_t i = j; i < m; i++) cc[i] -= sdot * v[i];
+            }
+        });
     }
     for (size_t i = 0; i < n; i++)
         for (size_t c = i; c < n; c++) r.at(i, c) = col[c * m + i];
@@ -154,11 +259,13 @@ static void householder_qr(const Mat &a, Mat &q, Mat &r) {
         const double beta = betas[jj];
         if (beta == 0.0) continue;
         const double *v = &vs[jj * m];
-        for (size_t c = 0; c < n; c++) {
-            double *cc = &qc[c * m];
-            const double sdot = beta * dot4(v + jj, cc + jj, m - jj);
-            for (size_t i = jj; i < m; i++) cc[i] -= sdot * v[i];
-        }
+        parallel_if(n, 4 * (m - jj), [&](size_t c0, size_t c1) {
+            for (size_t c = c0; c < c1; c++) {
+                double *cc = &qc[c * m];
+                const double sdot = beta * dot4(v + jj, cc + jj, m - jj);
+                for (size_t i = jj; i < m; i++) cc[i] -= sdot * v[i];
+            }
+        });
     }
     q = Mat(m, n);
     for (size_t i = 0; i < m; i++)
@@ -217,6 +324,29 @@ static void jacobi_svd(const Mat &a, Mat &u, std::vector<double> &s) {
     }
 }
 
+// A lower bound of the smallest singular value of an upper triangular r (n x n):
+// 1 / |r^-1|_F, with r^-1 by back substitution; 0 when a diagonal entry is 0 or the
+// inverse is not finite.
+static double min_singular_bound(const Mat &r) {
+    const size_t n = r.rows;
+    if (n == 0) return INFINITY;
+    std::vector<double> inv(n * n, 0.0);  // column c of r^-1 at inv[c * n ..]
+    for (size_t c = 0; c < n; c++) {
+        double *x = &inv[c * n];
+        for (size_t i = c + 1; i-- > 0;) {
+            const double diag = r.at(i, i);
+            if (diag == 0.0) return 0.0;
+            double acc = i == c ? 1.0 : 0.0;
+            for (size_t j = i + 1; j <= c; j++) acc -= r.at(i, j) * x[j];
+            x[i] = acc / diag;
+        }
+    }
+    double ss = 0.0;
+    for (double v : inv) ss += v * v;
+    if (!std::isfinite(ss) || ss == 0.0) return 0.0;
+    return 1.0 / std::sqrt(ss);
+}
+
 Mat extend_basis(const Mat &basis, const Mat &block) {
     size_t rows = block.rows;
     double scale = frobenius(block);
@@ -236,6 +366,12 @@ Mat extend_basis(const Mat &basis, const Mat &block) {
     std::vector<double> s;
     KY_CHECK(rows >= n, KY_E505, "pca: a Krylov block wider than the genes (internal)");
     householder_qr(rest, q, r);
+    // Full rank (every singular value of R above the drop tolerance) keeps the whole
+    // block, and the new columns may then be Q itself: span(Q U) = span(Q) for a square
+    // orthogonal U, and the passes and the Rayleigh-Ritz step depend on the basis only
+    // through its span. sigma_min(R) >= 1 / |R^-1|_F (back substitution) proves it
+    // without the SVD; otherwise the SVD decides which directions to drop.
+    if (min_singular_bound(r) > DROP_TOLERANCE * scale) return q;
     jacobi_svd(r, ur, s);
     size_t keep = 0;
     while (keep < s.size() && s[keep] > DROP_TOLERANCE * scale) keep++;
@@ -246,11 +382,13 @@ Mat extend_basis(const Mat &basis, const Mat &block) {
     return mul_nn(q, ukeep);
 }
 
-// Symmetric eigendecomposition: a (n x n) = V diag(d) V^T, d ascending. Householder
-// reduction to tridiagonal form on full row-major storage (a symmetric matrix-vector
-// product and a rank-2 update per column, both along contiguous rows), the
-// reflections accumulated into the eigenvector columns, then the implicit QL method.
-static void symmetric_eigen(const Mat &a, std::vector<double> &d, Mat &vout) {
+// Symmetric eigendecomposition a (n x n) = V diag(d) V^T: every eigenvalue (d
+// ascending) and the eigenvectors of the `want` largest (vout, n x want, largest
+// first). Householder reduction to tridiagonal form on full row-major storage (a
+// symmetric matrix-vector product and a rank-2 update per column, both along
+// contiguous rows), then the implicit QL method; only the wanted eigenvectors are
+// formed, from the recorded rotations and reflections.
+static void symmetric_eigen(const Mat &a, size_t want, std::vector<double> &d, Mat &vout) {
     const size_t n = a.rows;
     d.assign(n, 0.0);
     std::vector<double> e(n, 0.0);
@@ -273,37 +411,32 @@ static void symmetric_eigen(const Mat &a, std::vector<double> &d, Mat &vout) {
         const double beta = 2.0 / dot4(v.data(), v.data(), m);
         betas[j] = beta;
         // p = beta A' v, K = beta/2 v.p, w = p - K v, A' -= v w^T + w v^T.
-        for (size_t r = 0; r < m; r++) p[r] = beta * dot4(A.row(j + 1 + r) + j + 1, v.data(), m);
+        parallel_if(m, 2 * m, [&](size_t r0, size_t r1) {
+            for (size_t r = r0; r < r1; r++) p[r] = beta * dot4(A.row(j + 1 + r) + j + 1, v.data(), m);
+        }, 8);
         const double K = 0.5 * beta * dot4(v.data(), p.data(), m);
         for (size_t r = 0; r < m; r++) w[r] = p[r] - K * v[r];
-        for (size_t r = 0; r < m; r++) {
-            double *ar = A.row(j + 1 + r) + j + 1;
-            const double vr = v[r], wr = w[r];
-            for (size_t c = 0; c < m; c++) ar[c] -= vr * w[c] + wr * v[c];
-        }
+        parallel_if(m, 4 * m, [&](size_t r0, size_t r1) {
+            for (size_t r = r0; r < r1; r++) {
+                double *ar = A.row(j + 1 + r) + j + 1;
+                const double vr = v[r], wr = w[r];
+                for (size_t c = 0; c < m; c++) ar[c] -= vr * w[c] + wr * v[c];
+            }
+        });
         A.at(j + 1, j) = alpha;
         A.at(j, j + 1) = alpha;
     }
     for (size_t j = 0; j < n; j++) d[j] = A.at(j, j);
     for (size_t j = 0; j + 1 < n; j++) e[j] = A.at(j + 1, j);
     e[n - 1] = 0.0;
-    // Eigenvector columns Z[c] (contiguous): start from Q = H_0 ... H_{n-3}, whose
-    // column c is Q e_c, applied from the last reflection back.
-    std::vector<std::vector<double>> Z(n, std::vector<double>(n, 0.0));
-    for (size_t c = 0; c < n; c++) Z[c][c] = 1.0;
-    for (size_t jj = n >= 2 ? n - 2 : 0; jj-- > 0;) {
-        if (betas[jj] == 0.0) continue;
-        const std::vector<double> &v = vs[jj];
-        const size_t m = n - jj - 1;
-        for (size_t c = 0; c < n; c++) {
-            double *z = Z[c].data() + jj + 1;
-            const double s = betas[jj] * dot4(v.data(), z, m);
-            if (s != 0.0)
-                for (size_t r = 0; r < m; r++) z[r] -= s * v[r];
-        }
-    }
-    // Implicit QL on the tridiagonal matrix (e[i] couples i and i + 1); the
-    // rotations act on the contiguous eigenvector columns Z.
+    // Implicit QL on the tridiagonal matrix (e[i] couples i and i + 1), its rotations
+    // recorded: the eigenvector matrix is Q R_1 ... R_N (Q the reflections above, R_t
+    // the t-th rotation acting on columns i, i + 1).
+    struct Rotation {
+        size_t i;
+        double c, s;
+    };
+    std::vector<Rotation> rotations;
     double f = 0.0, tst1 = 0.0;
     const double eps = 2.220446049250313e-16;
     for (size_t l = 0; l < n; l++) {
@@ -321,7 +454,7 @@ static void symmetric_eigen(const Mat &a, std::vector<double> &d, Mat &vout) {
                     throw Error(KY_E215, "pca: the Rayleigh-Ritz eigendecomposition did not converge");
                 double g = d[l];
                 double p = (d[l + 1] - g) / (2.0 * e[l]);
-                double r = std::hypot(p, 1.0);
+                double r = safe_hypot(p, 1.0);
                 if (p < 0) r = -r;
                 d[l] = e[l] / (p + r);
                 d[l + 1] = e[l] * (p + r);
@@ -339,18 +472,13 @@ static void symmetric_eigen(const Mat &a, std::vector<double> &d, Mat &vout) {
                     s2 = s;
                     g = c * e[ii];
                     h = c * p;
-                    r = std::hypot(p, e[ii]);
+                    r = safe_hypot(p, e[ii]);
                     e[ii + 1] = s * r;
                     s = e[ii] / r;
                     c = p / r;
                     p = c * d[ii] - s * g;
                     d[ii + 1] = h + s * (c * g + s * d[ii]);
-                    double *z1 = Z[ii + 1].data(), *z0 = Z[ii].data();
-                    for (size_t k = 0; k < n; k++) {
-                        h = z1[k];
-                        z1[k] = s * z0[k] + c * h;
-                        z0[k] = c * z0[k] - s * h;
-                    }
+                    rotations.push_back({ii, c, s});
                 }
                 p = -s * s2 * c3 * el1 * e[l] / dl1;
                 e[l] = s * p;
@@ -360,15 +488,49 @@ static void symmetric_eigen(const Mat &a, std::vector<double> &d, Mat &vout) {
         d[l] = d[l] + f;
         e[l] = 0.0;
     }
+    // The eigenvalues ascending (ties in index order), then the `want` largest's
+    // vectors, largest first: column j is Q R_1 ... R_N e_{s_j}, the rotations applied
+    // to the unit vectors from the last back (each a pair of rows of the n x want
+    // block), then the reflections from the first back.
     std::vector<size_t> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return d[x] < d[y]; });
     std::vector<double> ds(n);
-    vout = Mat(n, n);
-    for (size_t k = 0; k < n; k++) {
-        ds[k] = d[order[k]];
-        for (size_t i = 0; i < n; i++) vout.at(i, k) = Z[order[k]][i];
+    for (size_t k = 0; k < n; k++) ds[k] = d[order[k]];
+    want = std::min(want, n);
+    std::vector<double> Y(n * want, 0.0);  // row-major n x want
+    for (size_t j = 0; j < want; j++) Y[order[n - 1 - j] * want + j] = 1.0;
+    for (size_t t = rotations.size(); t-- > 0;) {
+        const Rotation &rot = rotations[t];
+        double *__restrict y0 = &Y[rot.i * want];
+        double *__restrict y1 = &Y[(rot.i + 1) * want];
+        const double c = rot.c, sn = rot.s;
+        for (size_t j = 0; j < want; j++) {
+            const double a0 = y0[j], a1 = y1[j];
+            y0[j] = c * a0 + sn * a1;
+            y1[j] = c * a1 - sn * a0;
+        }
     }
+    std::vector<double> acc(want);
+    for (size_t jj = n >= 2 ? n - 2 : 0; jj-- > 0;) {
+        if (betas[jj] == 0.0) continue;
+        const std::vector<double> &v = vs[jj];
+        const size_t m = n - jj - 1;
+        std::fill(acc.begin(), acc.end(), 0.0);
+        for (size_t r = 0; r < m; r++) {
+            const double vr = v[r];
+            const double *y = &Y[(jj + 1 + r) * want];
+            for (size_t j = 0; j < want; j++) acc[j] += vr * y[j];
+        }
+        for (size_t j = 0; j < want; j++) acc[j] *= betas[jj];
+        for (size_t r = 0; r < m; r++) {
+            const double vr = v[r];
+            double *y = &Y[(jj + 1 + r) * want];
+            for (size_t j = 0; j < want; j++) y[j] -= acc[j] * vr;
+        }
+    }
+    vout = Mat(n, want);
+    std::copy(Y.begin(), Y.end(), vout.v.begin());
     d = ds;
 }
 
@@ -441,15 +603,10 @@ void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<
             if (!std::isfinite(v))
                 throw Error(KY_E215, "pca: the Rayleigh-Ritz eigendecomposition did not converge (a value that is not finite)");
         std::vector<double> d;
-        Mat vecs;
-        symmetric_eigen(sym, d, vecs);
+        Mat chosen;
         size_t taken = std::min(k, p);
-        Mat chosen(p, taken);
-        for (size_t c = 0; c < taken; c++) {
-            size_t idx = p - 1 - c;
-            eigenvalues.push_back(std::max(d[idx], 0.0));
-            for (size_t i = 0; i < p; i++) chosen.at(i, c) = vecs.at(i, idx);
-        }
+        symmetric_eigen(sym, taken, d, chosen);
+        for (size_t c = 0; c < taken; c++) eigenvalues.push_back(std::max(d[p - 1 - c], 0.0));
         loadings = mul_nn(basis, chosen);
     }
     size_t missing = k - eigenvalues.size();

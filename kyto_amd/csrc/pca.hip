@@ -180,19 +180,19 @@ __device__ __forceinline__ float dense_load(__amdgpu_buffer_rsrc_t r, uint32_t l
 
 // One wavefront per row; lane j holds columns j, j+64, ... (CPL of them, w <= 64*CPL).
 // out[r, j] = f32(-offsets[j] + sum_e f64(value_e) * f64(M[gene_e, j])), the entries
-// added in row order. M: m_rows x w.
+// added in row order. M: m_rows x w, rows ldm floats apart.
 template <int CPL>
 __global__ __launch_bounds__(256) void row_project(const uint64_t *__restrict__ indptr,
                                                    const uint32_t *__restrict__ indices,
                                                    const float *__restrict__ values, uint64_t n_rows,
                                                    const float *__restrict__ M, uint32_t m_rows, uint32_t w,
-                                                   const double *__restrict__ offsets,
+                                                   uint32_t ldm, const double *__restrict__ offsets,
                                                    float *__restrict__ out, uint32_t ldo) {
     const uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / 64) + wave_in_block();
     if (r >= n_rows) return;
     const unsigned lane = lane_id();
-    const __amdgpu_buffer_rsrc_t mr = dense_rsrc(M, (m_rows * w + (uint32_t)DENSE_PAD) * 4u);
-    const uint32_t row_bytes = w * 4u;
+    const __amdgpu_buffer_rsrc_t mr = dense_rsrc(M, (m_rows * ldm + (uint32_t)DENSE_PAD) * 4u);
+    const uint32_t row_bytes = ldm * 4u;
     double acc[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; c++) {
@@ -508,7 +508,12 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
     KY_HIP(hipStreamSynchronize(st));
 }
 
-static void launch_row_project(Csr &csr, const float *M, uint32_t w, const double *offsets, float *out,
+// Dense operands of the products keep their rows 64 floats (256 bytes: two whole
+// 128-byte lines) apart or a multiple of that, so each gathered row touches exactly
+// the lines it needs.
+static uint32_t padded_row(uint32_t w) { return (w + 63) / 64 * 64; }
+
+static void launch_row_project(Csr &csr, const float *M, uint32_t w, uint32_t ldm, const double *offsets, float *out,
                                uint32_t ldo) {
     Context &ctx = *csr.ctx;
     const unsigned rows_per_block = 4;
@@ -519,13 +524,13 @@ static void launch_row_project(Csr &csr, const float *M, uint32_t w, const doubl
                        8.0 * csr.nnz + 8.0 * (csr.n_rows + 1.0) + 4.0 * csr.n_cols * w + 4.0 * csr.n_rows * w);
     if (w <= 64)
         row_project<1><<<blocks, 256, 0, ctx.stream>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows,
-                                                       M, csr.n_cols, w, offsets, out, ldo);
+                                                       M, csr.n_cols, w, ldm, offsets, out, ldo);
     else if (w <= 128)
         row_project<2><<<blocks, 256, 0, ctx.stream>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows,
-                                                       M, csr.n_cols, w, offsets, out, ldo);
+                                                       M, csr.n_cols, w, ldm, offsets, out, ldo);
     else
         row_project<4><<<blocks, 256, 0, ctx.stream>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows,
-                                                       M, csr.n_cols, w, offsets, out, ldo);
+                                                       M, csr.n_cols, w, ldm, offsets, out, ldo);
     if (t_clock) t_clock->end(0, ctx.stream);
     KY_HIP(hipGetLastError());
 }
@@ -534,7 +539,7 @@ void pca_sketch(Csr &csr, uint64_t seed, uint64_t cell_offset, uint32_t l, doubl
                 double *omega_sums) {
     Context &ctx = *csr.ctx;
     KY_CHECK(l >= 1 && l + 1 <= 256, KY_E215, "pca: sketch width out of range");
-    const uint32_t w = l + 1, ldw = w;
+    const uint32_t w = l + 1, ldw = padded_row(w);
     ctx.wbuf.reserve((size_t)csr.n_rows * ldw + DENSE_PAD);
     uint64_t total = csr.n_rows * (uint64_t)w;
     omega_rows<<<div_up(total, 256), 256, 0, ctx.stream>>>(csr.n_rows, seed, cell_offset, l, ctx.wbuf.ptr, ldw);
@@ -551,24 +556,28 @@ void pca_sketch(Csr &csr, uint64_t seed, uint64_t cell_offset, uint32_t l, doubl
 void pca_product(Csr &csr, const double *q, uint32_t w, const double *block_means, double *product, double *sums) {
     Context &ctx = *csr.ctx;
     KY_CHECK(w >= 1 && w <= 256, KY_E215, "pca: block width out of range");
-    std::vector<float> q32((size_t)csr.n_cols * w);
-    for (size_t i = 0; i < q32.size(); i++) q32[i] = (float)q[i];
+    const uint32_t ld = padded_row(w);
+    std::vector<float> q32((size_t)csr.n_cols * ld, 0.0f);
+    for (size_t g = 0; g < csr.n_cols; g++)
+        for (size_t j = 0; j < w; j++) q32[g * ld + j] = (float)q[g * w + j];
     ctx.dense_f32.reserve(q32.size() + DENSE_PAD);
     ctx.dense_f64.reserve(w);
     KY_HIP(hipMemcpyAsync(ctx.dense_f32.ptr, q32.data(), sizeof(float) * q32.size(), hipMemcpyHostToDevice,
                           ctx.stream));
     KY_HIP(hipMemcpyAsync(ctx.dense_f64.ptr, block_means, sizeof(double) * w, hipMemcpyHostToDevice, ctx.stream));
-    ctx.wbuf.reserve((size_t)csr.n_rows * w + DENSE_PAD);
-    launch_row_project(csr, ctx.dense_f32.ptr, w, ctx.dense_f64.ptr, ctx.wbuf.ptr, w);
-    transposed_product(csr, w, w, product, sums);
+    ctx.wbuf.reserve((size_t)csr.n_rows * ld + DENSE_PAD);
+    launch_row_project(csr, ctx.dense_f32.ptr, w, ld, ctx.dense_f64.ptr, ctx.wbuf.ptr, ld);
+    transposed_product(csr, w, ld, product, sums);
 }
 
 void pca_project(Csr &csr, const double *v, uint32_t k, const double *mean_projection, const double *means,
                  float *host_embedding, Dense **embedding, double *squares, uint64_t *stored) {
     Context &ctx = *csr.ctx;
     KY_CHECK(k >= 1 && k <= 256, KY_E215, "pca: n_components out of range");
-    std::vector<float> v32((size_t)csr.n_cols * k);
-    for (size_t i = 0; i < v32.size(); i++) v32[i] = (float)v[i];
+    const uint32_t ld = padded_row(k);
+    std::vector<float> v32((size_t)csr.n_cols * ld, 0.0f);
+    for (size_t g = 0; g < csr.n_cols; g++)
+        for (size_t j = 0; j < k; j++) v32[g * ld + j] = (float)v[g * k + j];
     ctx.dense_f32.reserve(v32.size() + DENSE_PAD);
     ctx.dense_f64.reserve(k + csr.n_cols);
     hipStream_t st = ctx.stream;
@@ -581,7 +590,7 @@ void pca_project(Csr &csr, const double *v, uint32_t k, const double *mean_proje
         emb->rows = csr.n_rows;
         emb->cols = k;
         emb->data.alloc((size_t)csr.n_rows * k);
-        launch_row_project(csr, ctx.dense_f32.ptr, k, ctx.dense_f64.ptr, emb->data.ptr, k);
+        launch_row_project(csr, ctx.dense_f32.ptr, k, ld, ctx.dense_f64.ptr, emb->data.ptr, k);
         // Squares and counts per gene.
         const size_t cells = (size_t)csr.n_chunks * csr.n_cols;
         ctx.partials.reserve(2 * cells);
