@@ -53,11 +53,18 @@ __device__ __forceinline__ float unorder_f32(uint32_t u) {
 // kyto's exact key (kernels.rs:235-247, pair_sum :422-448, combine_lanes :290-299):
 // component j's f64 difference squared into lane j % 8, then the fixed tree. This
 // translation unit is compiled without contraction, so no product is fused.
+// The components go eight at a time so that each lane sum stays in a register (a
+// lane index that is not a compile-time constant would put l in scratch memory).
 __device__ __forceinline__ double exact_sq_distance(const float *a, const float *b, unsigned d) {
     double l[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (unsigned j = 0; j < d; j++) {
-        double t = (double)a[j] - (double)b[j];
-        l[j & 7] += t * t;
+    for (unsigned j0 = 0; j0 < d; j0 += 8) {
+#pragma unroll
+        for (unsigned u = 0; u < 8; u++) {
+            if (j0 + u < d) {
+                double t = (double)a[j0 + u] - (double)b[j0 + u];
+                l[u] += t * t;
+            }
+        }
     }
     double h0 = l[0] + l[4], h1 = l[1] + l[5], h2 = l[2] + l[6], h3 = l[3] + l[7];
     return (h0 + h2) + (h1 + h3);

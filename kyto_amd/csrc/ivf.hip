@@ -201,8 +201,13 @@ __global__ __launch_bounds__(256) void ivf_scan(const float *__restrict__ pts, u
     __shared__ float qv[4][64];
     __shared__ uint32_t bcnt[4], trow[4];
     __shared__ unsigned long long tkey[4];
+    // The wave's 64 points of the current chunk, rows `ps` floats apart (odd: lanes
+    // reading their own rows fall on different banks).
+    extern __shared__ float chunk_lds[];
     const unsigned wave = threadIdx.x / 64, lane = lane_id();
-    const uint64_t q = (uint64_t)blockIdx.x * 4 + wave;
+    const uint32_t ps = d | 1u;
+    float *cp = chunk_lds + wave * 64 * ps;
+    const uint64_t q = (uint64_t)blockIdx.x * (blockDim.x / 64) + wave;
     if (q >= n) return;
     if (lane < d) qv[wave][lane] = pts[q * d + lane];
     if (lane == 0) {
@@ -219,12 +224,33 @@ __global__ __launch_bounds__(256) void ivf_scan(const float *__restrict__ pts, u
                 compact<unsigned long long, CAP>(bkey[wave], brow[wave], &bcnt[wave], &tkey[wave], &trow[wave], k);
             const uint32_t idx = e0 + lane;
             const bool valid = idx < e;
+            // The chunk's points (contiguous in list order) staged through LDS with
+            // coalesced loads.
+            const uint32_t cnt = min(64u, e - e0);
+            const float *src = list_pts + (uint64_t)e0 * d;
+            const uint32_t total = cnt * d;
+            // Sixteen loads a lane in flight at a time, then their stores.
+            for (uint32_t t0 = 0; t0 < total; t0 += 64 * 16) {
+                float v[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) {
+                    const uint32_t t = t0 + u * 64 + lane;
+                    v[u] = t < total ? src[t] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < 16; u++) {
+                    const uint32_t t = t0 + u * 64 + lane;
+                    if (t < total) cp[(t / d) * ps + t % d] = v[u];
+                }
+            }
+            wave_sync();
             uint32_t row = 0xFFFFFFFFu;
             unsigned long long kb = ~0ull;
             if (valid) {
                 row = list_rows[idx];
-                kb = (unsigned long long)__double_as_longlong(exact_sq_distance(qv[wave], list_pts + (uint64_t)idx * d, d));
+                kb = (unsigned long long)__double_as_longlong(exact_sq_distance(qv[wave], cp + lane * ps, d));
             }
+            wave_sync();
             const unsigned long long tk = tkey[wave];
             const uint32_t tr = trow[wave];
             const bool hit = valid && row != q && (kb < tk || (kb == tk && row < tr));
@@ -513,7 +539,10 @@ void ivf_search(Context &ctx, const ky_ivf &ivf, const Dense &points, uint32_t n
     timer.start();
     DevBuf<double> keys(n * k);
     DevBuf<uint32_t> rows(n * k), count(n);
-    ivf_scan<<<div_up(n, 4), 256, 0, st>>>(x, d, n, ivf.list_ptr.ptr, ivf.list_rows.ptr, ivf.list_pts.ptr,
+    // Four wavefronts a block when their point chunks fit 48 KB of LDS, else two.
+    const unsigned scan_waves = sizeof(float) * 4 * 64 * (d | 1u) <= 48 * 1024 + 4096 ? 4 : 2;
+    ivf_scan<<<div_up(n, scan_waves), 64 * scan_waves, sizeof(float) * scan_waves * 64 * (d | 1u), st>>>(
+        x, d, n, ivf.list_ptr.ptr, ivf.list_rows.ptr, ivf.list_pts.ptr,
                                            probes.ptr, P, k, keys.ptr, rows.ptr, count.ptr);
     KY_HIP(hipGetLastError());
     std::vector<float> hconf(n);

@@ -6,8 +6,8 @@
 // GEMM: C[M x N] (f32, row stride ldc) = op(A)[M x K] op(B)[K x N], with A(i, k) at
 // A[i * sai + k * sak] and B(k, j) at B[k * sbk + j * sbj] (f32 in memory, rounded to
 // bf16 when staged in LDS: BF16 weights and activations, FP32 accumulation). Block
-// tile 64 x 64, K step 32, four wavefronts each computing 32 x 32 with four
-// v_mfma_f32_16x16x32_bf16 per K step.
+// tile 128 x 128, K step 32, four wavefronts each computing 64 x 64 with sixteen
+// v_mfma_f32_16x16x32_bf16 a K step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,94 +23,180 @@ namespace ky {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int GT = 64;     // block tile (rows and columns)
+constexpr int GT = 128;    // block tile (rows and columns)
 constexpr int GK = 32;     // K step
 constexpr int GKP = 40;    // LDS row stride in bf16 (32 + 8 padding: 80 bytes)
 
+// One operand tile (GT rows x GK of K, "row" = i for A, j for B) from f32 memory into
+// LDS as bf16 [row][k]. KFAST: the operand is contiguous along k (thread t reads 16
+// consecutive k of row t / 2). Else it is contiguous along the rows: with VEC (rows
+// and the k stride 16-byte aligned) thread t reads a 4 x 4 block (rows 4 (t % 32)..,
+// k 4 (t / 32)..) as four 16-byte loads, coalesced across the threads, and writes it
+// transposed as four 8-byte rows; without, 16 consecutive rows at k = t / 8.
+// Out-of-range elements are 0.
+template <bool KFAST>
+__device__ __forceinline__ void load_tile(const float *__restrict__ X, int64_t s_row, int64_t s_k, int row0, int rows,
+                                          int k0, int K, bool vec, __bf16 *__restrict__ lds) {
+    const int t = threadIdx.x;
+    if constexpr (KFAST) {
+        const int r = t / 2, kb = (t % 2) * 16;
+        const int gr = row0 + r;
+        __bf16 v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int gk = k0 + kb + u;
+            v[u] = (__bf16)((gr < rows && gk < K) ? X[gr * s_row + gk * s_k] : 0.0f);
+        }
+        bf16x8 *dst = reinterpret_cast<bf16x8 *>(lds + r * GKP + kb);
+        dst[0] = bf16x8{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+        dst[1] = bf16x8{v[8], v[9], v[10], v[11], v[12], v[13], v[14], v[15]};
+    } else if (vec) {
+        const int rq = t % 32, kq = t / 32;
+        const int gr = row0 + 4 * rq;
+        float4 blk[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) {
+            const int gk = k0 + 4 * kq + kk;
+            if (gk < K && gr + 3 < rows) {
+                blk[kk] = *reinterpret_cast<const float4 *>(X + (int64_t)gk * s_k + gr);
+            } else {
+                float e[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) e[u] = (gk < K && gr + u < rows) ? X[(int64_t)gk * s_k + gr + u] : 0.0f;
+                blk[kk] = make_float4(e[0], e[1], e[2], e[3]);
+            }
+        }
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<bf16x4 *>(lds + (4 * rq + 0) * GKP + 4 * kq) =
+            bf16x4{(__bf16)blk[0].x, (__bf16)blk[1].x, (__bf16)blk[2].x, (__bf16)blk[3].x};
+        *reinterpret_cast<bf16x4 *>(lds + (4 * rq + 1) * GKP + 4 * kq) =
+            bf16x4{(__bf16)blk[0].y, (__bf16)blk[1].y, (__bf16)blk[2].y, (__bf16)blk[3].y};
+        *reinterpret_cast<bf16x4 *>(lds + (4 * rq + 2) * GKP + 4 * kq) =
+            bf16x4{(__bf16)blk[0].z, (__bf16)blk[1].z, (__bf16)blk[2].z, (__bf16)blk[3].z};
+        *reinterpret_cast<bf16x4 *>(lds + (4 * rq + 3) * GKP + 4 * kq) =
+            bf16x4{(__bf16)blk[0].w, (__bf16)blk[1].w, (__bf16)blk[2].w, (__bf16)blk[3].w};
+    } else {
+        const int kk = t / 8, rb = (t % 8) * 16;
+        const int gk = k0 + kk;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int gr = row0 + rb + u;
+            lds[(rb + u) * GKP + kk] = (__bf16)((gr < rows && gk < K) ? X[gr * s_row + gk * s_k] : 0.0f);
+        }
+    }
+}
+
+// Block tile GT x GT, four wavefronts each computing 64 x 64 with sixteen
+// v_mfma_f32_16x16x32_bf16 a K step. Split K: block z sums K steps [z * kc, (z + 1) kc)
+// and, with n_split > 1, writes its raw partial product to C + z * M * N (row stride
+// N); gemm_split_sum then adds the parts in order and applies the epilogue.
+template <bool A_KFAST, bool B_KFAST>
 __global__ __launch_bounds__(256) void gemm_bf16(const float *__restrict__ A, int64_t sai, int64_t sak,
                                                  const float *__restrict__ B, int64_t sbk, int64_t sbj,
-                                                 float *__restrict__ C, int64_t ldc, int M, int N, int K,
+                                                 float *__restrict__ C, int64_t ldc, int M, int N, int K, int kc,
+                                                 int n_split, bool a_vec, bool b_vec,
                                                  const float *__restrict__ bias, int bias_mode, int relu) {
     __shared__ __attribute__((aligned(16))) __bf16 As[GT * GKP];  // [row][k]
     __shared__ __attribute__((aligned(16))) __bf16 Bs[GT * GKP];  // [col][k]
     const int tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
     const int row0 = blockIdx.y * GT, col0 = blockIdx.x * GT;
-    const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;  // the wave's 32 x 32 sub-tile
-    f32x4 acc[2][2];
+    const int wr = (wave / 2) * 64, wc = (wave % 2) * 64;  // the wave's 64 x 64 sub-tile
+    const int kbeg = blockIdx.z * kc, kend = min(K, kbeg + kc);
+    f32x4 acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // Loads walk the operand's contiguous dimension across threads.
-    const bool a_rows_fast = sai == 1 && sak != 1;
-    const bool b_cols_fast = sbj == 1 && sbk != 1;
-    for (int k0 = 0; k0 < K; k0 += GK) {
-        for (int e = tid; e < GT * GK; e += 256) {
-            int r, kk;
-            if (a_rows_fast) {
-                r = e % GT;
-                kk = e / GT;
-            } else {
-                kk = e % GK;
-                r = e / GK;
-            }
-            const int gi = row0 + r, gk = k0 + kk;
-            float v = (gi < M && gk < K) ? A[gi * sai + gk * sak] : 0.0f;
-            As[r * GKP + kk] = (__bf16)v;
-        }
-        for (int e = tid; e < GT * GK; e += 256) {
-            int c, kk;
-            if (b_cols_fast) {
-                c = e % GT;
-                kk = e / GT;
-            } else {
-                kk = e % GK;
-                c = e / GK;
-            }
-            const int gj = col0 + c, gk = k0 + kk;
-            float v = (gj < N && gk < K) ? B[gk * sbk + gj * sbj] : 0.0f;
-            Bs[c * GKP + kk] = (__bf16)v;
-        }
+        for (int b = 0; b < 4; b++) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = kbeg; k0 < kend; k0 += GK) {
+        load_tile<A_KFAST>(A, sai, sak, row0, M, k0, kend, a_vec, As);
+        load_tile<B_KFAST>(B, sbj, sbk, col0, N, k0, kend, b_vec, Bs);
         __syncthreads();
-        bf16x8 af[2], bfr[2];
+        bf16x8 af[4], bfr[4];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int a = 0; a < 4; a++)
             af[a] = *reinterpret_cast<const bf16x8 *>(&As[(wr + 16 * a + lane % 16) * GKP + 8 * (lane / 16)]);
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+        for (int b = 0; b < 4; b++)
             bfr[b] = *reinterpret_cast<const bf16x8 *>(&Bs[(wc + 16 * b + lane % 16) * GKP + 8 * (lane / 16)]);
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int a = 0; a < 4; a++)
 #pragma unroll
-            for (int b = 0; b < 2; b++)
+            for (int b = 0; b < 4; b++)
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
         __syncthreads();
     }
+    float *out = n_split > 1 ? C + (int64_t)blockIdx.z * M * N : C;
+    const int64_t ld = n_split > 1 ? N : ldc;
     // D layout: lane l, slot j -> row 4 (l / 16) + j, column l % 16 of the 16 x 16 tile.
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+        for (int b = 0; b < 4; b++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int gi = row0 + wr + 16 * a + 4 * (lane / 16) + j;
                 const int gj = col0 + wc + 16 * b + lane % 16;
                 if (gi < M && gj < N) {
                     float v = acc[a][b][j];
-                    if (bias_mode == 1) v += bias[gj];      // per column
-                    else if (bias_mode == 2) v += bias[gi]; // per row
-                    if (relu) v = fmaxf(v, 0.0f);
-                    C[gi * ldc + gj] = v;
+                    if (n_split == 1) {
+                        if (bias_mode == 1) v += bias[gj];      // per column
+                        else if (bias_mode == 2) v += bias[gi]; // per row
+                        if (relu) v = fmaxf(v, 0.0f);
+                    }
+                    out[gi * ld + gj] = v;
                 }
             }
 }
 
+// C = sum over the n_split parts (in order) + bias, ReLU.
+__global__ void gemm_split_sum(const float *__restrict__ parts, int n_split, int M, int N, float *__restrict__ C,
+                               int64_t ldc, const float *__restrict__ bias, int bias_mode, int relu) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)M * N) return;
+    const int gi = (int)(t / N), gj = (int)(t % N);
+    float v = 0.0f;
+    for (int z = 0; z < n_split; z++) v += parts[(int64_t)z * M * N + t];
+    if (bias_mode == 1) v += bias[gj];
+    else if (bias_mode == 2) v += bias[gi];
+    if (relu) v = fmaxf(v, 0.0f);
+    C[gi * ldc + gj] = v;
+}
+
 void gemm(hipStream_t st, const float *A, int64_t sai, int64_t sak, const float *B, int64_t sbk, int64_t sbj,
-          float *C, int64_t ldc, int M, int N, int K, const float *bias, int bias_mode, bool relu) {
+          float *C, int64_t ldc, int M, int N, int K, const float *bias, int bias_mode, bool relu,
+          DevBuf<float> *split_scratch) {
     if (M <= 0 || N <= 0) return;
-    dim3 grid(div_up(N, GT), div_up(M, GT));
-    gemm_bf16<<<grid, 256, 0, st>>>(A, sai, sak, B, sbk, sbj, C, ldc, M, N, K, bias, bias_mode, relu ? 1 : 0);
+    const bool a_kfast = sak == 1, b_kfast = sbk == 1;
+    auto aligned = [](const float *p, int64_t s) { return ((uintptr_t)p % 16 == 0) && s % 4 == 0; };
+    const bool a_vec = !a_kfast && sai == 1 && aligned(A, sak);
+    const bool b_vec = !b_kfast && sbj == 1 && aligned(B, sbk);
+    // Split K when the output tiles alone would leave most of the device idle.
+    const unsigned tiles = div_up(N, GT) * div_up(M, GT);
+    int n_split = 1;
+    if (split_scratch && tiles < 256 && K >= 8 * GK)
+        n_split = (int)std::min<int64_t>({8, div_up(256, tiles), K / (2 * GK)});
+    const int kc = n_split > 1 ? (int)div_up(div_up(K, n_split), GK) * GK : K;
+    n_split = n_split > 1 ? (int)div_up(K, kc) : 1;
+    float *out = C;
+    if (n_split > 1) {
+        split_scratch->reserve((size_t)n_split * M * N);
+        out = split_scratch->ptr;
+    }
+    dim3 grid(div_up(N, GT), div_up(M, GT), n_split);
+    const int r = relu ? 1 : 0;
+#define KY_GEMM(AK, BK)                                                                                          \
+    gemm_bf16<AK, BK><<<grid, 256, 0, st>>>(A, sai, sak, B, sbk, sbj, out, ldc, M, N, K, kc, n_split, a_vec, b_vec, \
+                                           bias, bias_mode, r)
+    if (a_kfast && b_kfast) KY_GEMM(true, true);
+    else if (a_kfast) KY_GEMM(true, false);
+    else if (b_kfast) KY_GEMM(false, true);
+    else KY_GEMM(false, false);
+#undef KY_GEMM
     KY_HIP(hipGetLastError());
+    if (n_split > 1) {
+        gemm_split_sum<<<div_up((int64_t)M * N, 256), 256, 0, st>>>(out, n_split, M, N, C, ldc, bias, bias_mode, r);
+        KY_HIP(hipGetLastError());
+    }
 }
 
 // ------------------------------------------------------------ row reductions ----
@@ -260,17 +346,37 @@ void relu_mask(hipStream_t st, float *G, const float *H, int64_t count) {
     KY_HIP(hipGetLastError());
 }
 
-// out[j] = sum over rows i (in order) of G[i * cols + j].
-__global__ void col_sum(const float *__restrict__ G, int64_t rows, int cols, float *__restrict__ out) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
+// out[j] = sum over rows i of G[i * cols + j]: partial sums over chunks of CS_ROWS
+// rows (each in row order), then the chunks' sums in order (the same bits on every
+// run). Block (64 columns x 4 chunks): lanes read consecutive columns.
+constexpr int CS_ROWS = 64;
+
+__global__ __launch_bounds__(256) void col_sum_chunks(const float *__restrict__ G, int64_t rows, int cols,
+                                                      float *__restrict__ partial) {
+    const int j = blockIdx.x * 64 + lane_id();
+    const int64_t chunk = (int64_t)blockIdx.y * 4 + threadIdx.x / 64;
+    const int64_t r0 = chunk * CS_ROWS;
+    if (j >= cols || r0 >= rows) return;
+    const int64_t r1 = min<int64_t>(r0 + CS_ROWS, rows);
+    float s = 0.0f;
+    for (int64_t i = r0; i < r1; i++) s += G[i * cols + j];
+    partial[chunk * cols + j] = s;
+}
+
+__global__ void col_sum_total(const float *__restrict__ partial, int64_t chunks, int cols, float *__restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cols) return;
     float s = 0.0f;
-    for (int64_t i = 0; i < rows; i++) s += G[i * cols + j];
+    for (int64_t c = 0; c < chunks; c++) s += partial[c * cols + j];
     out[j] = s;
 }
 
-void col_sum(hipStream_t st, const float *G, int64_t rows, int cols, float *out) {
-    col_sum<<<div_up(cols, 256), 256, 0, st>>>(G, rows, cols, out);
+void col_sum(hipStream_t st, const float *G, int64_t rows, int cols, float *out, DevBuf<float> &scratch) {
+    const int64_t chunks = (rows + CS_ROWS - 1) / CS_ROWS;
+    scratch.reserve((size_t)chunks * cols);
+    dim3 grid(div_up(cols, 64), div_up(chunks, 4));
+    col_sum_chunks<<<grid, 256, 0, st>>>(G, rows, cols, scratch.ptr);
+    col_sum_total<<<div_up(cols, 256), 256, 0, st>>>(scratch.ptr, chunks, cols, out);
     KY_HIP(hipGetLastError());
 }
 
@@ -356,18 +462,18 @@ void Router::backward(hipStream_t st, const float *X, int64_t rows, const float 
                       float *dH2, float *dH1) {
     DevBuf<float> &p = const_cast<DevBuf<float> &>(params);
     // Layer 3: dW3 = H2^T dL, db3 = colsum dL, dH2 = dL W3^T (masked by H2 > 0).
-    gemm(st, H2, 1, hidden, dL, n_out, 1, W(2, grads), n_out, hidden, n_out, (int)rows, nullptr, 0, false);
-    col_sum(st, dL, rows, n_out, b(2, grads));
+    gemm(st, H2, 1, hidden, dL, n_out, 1, W(2, grads), n_out, hidden, n_out, (int)rows, nullptr, 0, false, &split_scratch);
+    col_sum(st, dL, rows, n_out, b(2, grads), colsum_scratch);
     gemm(st, dL, n_out, 1, W(2, p), 1, n_out, dH2, hidden, (int)rows, hidden, n_out, nullptr, 0, false);
     relu_mask(st, dH2, H2, rows * hidden);
     // Layer 2.
-    gemm(st, H1, 1, hidden, dH2, hidden, 1, W(1, grads), hidden, hidden, hidden, (int)rows, nullptr, 0, false);
-    col_sum(st, dH2, rows, hidden, b(1, grads));
+    gemm(st, H1, 1, hidden, dH2, hidden, 1, W(1, grads), hidden, hidden, hidden, (int)rows, nullptr, 0, false, &split_scratch);
+    col_sum(st, dH2, rows, hidden, b(1, grads), colsum_scratch);
     gemm(st, dH2, hidden, 1, W(1, p), 1, hidden, dH1, hidden, (int)rows, hidden, hidden, nullptr, 0, false);
     relu_mask(st, dH1, H1, rows * hidden);
     // Layer 1.
-    gemm(st, X, 1, d_in, dH1, hidden, 1, W(0, grads), hidden, d_in, hidden, (int)rows, nullptr, 0, false);
-    col_sum(st, dH1, rows, hidden, b(0, grads));
+    gemm(st, X, 1, d_in, dH1, hidden, 1, W(0, grads), hidden, d_in, hidden, (int)rows, nullptr, 0, false, &split_scratch);
+    col_sum(st, dH1, rows, hidden, b(0, grads), colsum_scratch);
 }
 
 void Router::adam_step(hipStream_t st, float lr) {

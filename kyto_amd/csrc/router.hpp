@@ -5,14 +5,16 @@
 
 namespace ky {
 
+// split_scratch (optional): device scratch for splitting K when the output is small.
 void gemm(hipStream_t st, const float *A, int64_t sai, int64_t sak, const float *B, int64_t sbk, int64_t sbj,
-          float *C, int64_t ldc, int M, int N, int K, const float *bias, int bias_mode, bool relu);
+          float *C, int64_t ldc, int M, int N, int K, const float *bias, int bias_mode, bool relu,
+          DevBuf<float> *split_scratch = nullptr);
 void row_top(hipStream_t st, const float *S, int64_t rows, int cols, int T, uint32_t *top_idx, float *top_val,
              float *lse);
 void softmax_xent(hipStream_t st, float *L, int64_t rows, int cols, const uint32_t *t_idx, const float *t_w, int tk,
                   float inv_batch, float *loss);
 void relu_mask(hipStream_t st, float *G, const float *H, int64_t count);
-void col_sum(hipStream_t st, const float *G, int64_t rows, int cols, float *out);
+void col_sum(hipStream_t st, const float *G, int64_t rows, int cols, float *out, DevBuf<float> &scratch);
 void adam(hipStream_t st, float *w, const float *g, float *m, float *v, int64_t count, float lr, int step);
 
 // d_in -> hidden -> hidden -> n_out, ReLU between, f32 master parameters laid out
@@ -22,6 +24,8 @@ struct Router {
     size_t n_params = 0;
     int step = 0;
     DevBuf<float> params, grads, m1, m2;
+    DevBuf<float> colsum_scratch;  // the bias gradients' chunk sums
+    DevBuf<float> split_scratch;   // split-K parts of the weight gradients
     void init(Context &ctx, uint32_t in, uint32_t hid, uint32_t out, uint64_t seed);
     float *W(int layer, DevBuf<float> &buf) const;
     float *b(int layer, DevBuf<float> &buf) const;
