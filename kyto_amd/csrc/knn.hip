@@ -57,31 +57,53 @@ __global__ void embed_reduce(const double *__restrict__ partial, uint32_t d, uin
     mean[j] = s * inv_n;
 }
 
-// Per row: a = f32(x - mean) (zero padded to dp), sq = sum f64(a)^2 in order, offset
-// f32(sq), finite flag, and the largest sq (as bits: non-negative doubles order as
-// their bits).
-__global__ void embed_centre(const float *__restrict__ x, uint64_t n, uint32_t d, uint32_t ld,
-                             const double *__restrict__ mean, uint32_t dp, float *__restrict__ cent,
-                             float *__restrict__ offs, double *__restrict__ sqn, unsigned long long *__restrict__ largest,
-                             unsigned *__restrict__ bad) {
-    uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    double sq = 0.0;
-    bool finite = true;
-    for (uint32_t j = 0; j < dp; j++) {
+// Per row: a = f32(x - mean) (zero padded to dp <= 64), sq = sum f64(a)^2 (lane j
+// holds component j; a fixed butterfly adds the lanes), offset f32(sq), the finite
+// flag, and the largest sq (as bits: non-negative doubles order as their bits). One
+// wavefront a row, EMBED_ROWS rows a block, one atomic a block.
+constexpr unsigned EMBED_ROWS = 64;
+
+__global__ __launch_bounds__(256) void embed_centre(const float *__restrict__ x, uint64_t n, uint32_t d, uint32_t ld,
+                                                    const double *__restrict__ mean, uint32_t dp,
+                                                    float *__restrict__ cent, float *__restrict__ offs,
+                                                    double *__restrict__ sqn, unsigned long long *__restrict__ largest,
+                                                    unsigned *__restrict__ bad) {
+    __shared__ double wave_max[4];
+    __shared__ unsigned wave_bad[4];
+    const unsigned lane = lane_id(), wave = threadIdx.x / 64;
+    const double mu = lane < d ? mean[lane] : 0.0;
+    double mx = 0.0;
+    bool ok = true;
+    for (unsigned i = wave; i < EMBED_ROWS; i += 4) {
+        const uint64_t r = (uint64_t)blockIdx.x * EMBED_ROWS + i;
+        if (r >= n) break;
         float a = 0.0f;
-        if (j < d) {
-            float v = x[r * ld + j];
-            finite = finite && isfinite(v);
-            a = (float)((double)v - mean[j]);
-            sq += (double)a * (double)a;
+        if (lane < d) {
+            const float v = x[r * ld + lane];
+            ok = ok && isfinite(v);
+            a = (float)((double)v - mu);
         }
-        cent[r * dp + j] = a;
+        if (lane < dp) cent[r * dp + lane] = a;
+        double sq = (double)a * (double)a;
+#pragma unroll
+        for (int o = 32; o >= 1; o /= 2) sq += __shfl_xor(sq, o);
+        if (lane == 0) {
+            offs[r] = (float)sq;
+            sqn[r] = sq;
+        }
+        mx = fmax(mx, sq);
     }
-    offs[r] = (float)sq;
-    sqn[r] = sq;
-    if (!finite) atomicOr(bad, 1u);
-    atomicMax(largest, (unsigned long long)__double_as_longlong(sq));
+    const bool wave_ok = __ballot(!ok) == 0;
+    if (lane == 0) {
+        wave_max[wave] = mx;
+        wave_bad[wave] = wave_ok ? 0u : 1u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = fmax(fmax(wave_max[0], wave_max[1]), fmax(wave_max[2], wave_max[3]));
+        atomicMax(largest, (unsigned long long)__double_as_longlong(m));
+        if (wave_bad[0] | wave_bad[1] | wave_bad[2] | wave_bad[3]) atomicOr(bad, 1u);
+    }
 }
 
 // slack[r] = factor (sqrt(sq_r) + largest)^2 + underflow (neighbors_exact.rs:76-89).
@@ -110,6 +132,10 @@ __global__ void pack_columns(const float *__restrict__ x, uint64_t n, uint32_t d
 // Queries: q_list[i] when q_list is not null, else q_first + i, for i < q_count.
 // Output per query i: `keep` (key, row) pairs ascending and the count.
 //
+// EXACT with seed_keys / seed_rows (n x keep, at the query's row, ascending): the
+// query starts at the threshold of its seed's keep-th pair (inclusive), so a search
+// that already holds keep exact candidates keeps only the rows that can replace them.
+//
 // With n_seg > 1 the points are cut into n_seg segments of whole tiles: block b scans
 // segment b % n_seg for the queries of group b / n_seg, and writes its `keep` pairs at
 // slot i * n_seg + segment (merge_segments then keeps each query's smallest), so a
@@ -118,6 +144,8 @@ template <bool EXACT, int DP, int CAP>
 __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, const float *__restrict__ offs,
                                                    uint64_t n, uint32_t d, const uint32_t *__restrict__ q_list,
                                                    uint64_t q_first, uint64_t q_count, uint32_t keep, uint32_t n_seg,
+                                                   const double *__restrict__ seed_keys,
+                                                   const uint32_t *__restrict__ seed_rows,
                                                    uint32_t *__restrict__ out_rows, void *__restrict__ out_keys,
                                                    uint32_t *__restrict__ out_count) {
     using K = typename std::conditional<EXACT, unsigned long long, uint32_t>::type;
@@ -142,6 +170,18 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
         bcnt[tid] = 0;
         tkey[tid] = ~K(0);
         trow[tid] = ~0u;
+        if constexpr (EXACT) {
+            // A seed (the keep-th (key, row) of a known set of rows) bounds the
+            // query's keep-th pair from above: only pairs at or below it are kept.
+            if (seed_keys && qg != 0xFFFFFFFFu) {
+                const double sk = seed_keys[(uint64_t)qg * keep + keep - 1];
+                const uint32_t sr = seed_rows[(uint64_t)qg * keep + keep - 1];
+                if (isfinite(sk) && sr != 0xFFFFFFFFu) {
+                    tkey[tid] = (K)__double_as_longlong(sk);
+                    trow[tid] = sr + 1;
+                }
+            }
+        }
         sqo[tid] = (!EXACT && qg != 0xFFFFFFFFu) ? offs[qg] : 0.0f;
     }
     __syncthreads();
@@ -412,10 +452,12 @@ __global__ void graph_rows(const double *__restrict__ keys, const uint32_t *__re
 template <bool EXACT, int CAP>
 static void launch_scan(int dp, hipStream_t st, unsigned blocks, const float *a, const float *offs, uint64_t n,
                         uint32_t d, const uint32_t *list, uint64_t q_first, uint64_t q_count, uint32_t keep,
-                        uint32_t *out_rows, void *out_keys, uint32_t *out_count, uint32_t n_seg = 1) {
+                        uint32_t *out_rows, void *out_keys, uint32_t *out_count, uint32_t n_seg = 1,
+                        const double *seed_keys = nullptr, const uint32_t *seed_rows = nullptr) {
 #define KY_SCAN(DPV)                                                                                   \
     scan_points<EXACT, DPV, CAP><<<blocks * n_seg, 256, 0, st>>>(a, offs, n, d, list, q_first, q_count, keep, \
-                                                                 n_seg, out_rows, out_keys, out_count)
+                                                                 n_seg, seed_keys, seed_rows, out_rows, out_keys, \
+                                                                 out_count)
     switch (dp) {
         case 8: KY_SCAN(8); break;
         case 16: KY_SCAN(16); break;
@@ -468,7 +510,7 @@ void knn_prepare(Context &ctx, const float *x, uint64_t n, uint32_t d, KnnPrep &
     DevBuf<unsigned> bad(1);
     KY_HIP(hipMemsetAsync(largest.ptr, 0, sizeof(unsigned long long), st));
     KY_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned), st));
-    embed_centre<<<div_up(n, 256), 256, 0, st>>>(x, n, d, d, prep.mean.ptr, prep.dp, prep.cent.ptr, prep.offs.ptr,
+    embed_centre<<<div_up(n, EMBED_ROWS), 256, 0, st>>>(x, n, d, d, prep.mean.ptr, prep.dp, prep.cent.ptr, prep.offs.ptr,
                                                   prep.sqn.ptr, largest.ptr, bad.ptr);
     KY_HIP(hipGetLastError());
     unsigned long long lb = 0;
@@ -489,7 +531,7 @@ void knn_prepare(Context &ctx, const float *x, uint64_t n, uint32_t d, KnnPrep &
 
 // Exact search of the listed rows against every row; results into keys/rows (n x k).
 void knn_exact_rows(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32_t k, const uint32_t *d_list,
-                    uint64_t count, double *d_keys, uint32_t *d_rows) {
+                    uint64_t count, double *d_keys, uint32_t *d_rows, bool seeded) {
     if (count == 0) return;
     hipStream_t st = ctx.stream;
     DevBuf<uint32_t> rows_tmp(count * k), cnt_tmp(count);
@@ -499,14 +541,16 @@ void knn_exact_rows(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint
     // segment at least 16 tiles, n_seg * k within the merge's 1024 entries.
     uint32_t n_seg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(2048, blocks), n / (16 * TP)));
     n_seg = std::min<uint32_t>(n_seg, 1024 / k);
+    const double *sk = seeded ? d_keys : nullptr;
+    const uint32_t *sr = seeded ? d_rows : nullptr;
     if (n_seg <= 1) {
         launch_scan<true, 128>(padded_width(d), st, blocks, pts, nullptr, n, d, d_list, 0, count, k, rows_tmp.ptr,
-                               keys_tmp.ptr, cnt_tmp.ptr);
+                               keys_tmp.ptr, cnt_tmp.ptr, 1, sk, sr);
     } else {
         DevBuf<uint32_t> seg_rows(count * n_seg * k), seg_cnt(count * n_seg);
         DevBuf<double> seg_keys(count * n_seg * k);
         launch_scan<true, 128>(padded_width(d), st, blocks, pts, nullptr, n, d, d_list, 0, count, k, seg_rows.ptr,
-                               seg_keys.ptr, seg_cnt.ptr, n_seg);
+                               seg_keys.ptr, seg_cnt.ptr, n_seg, sk, sr);
         merge_segments<1024><<<(unsigned)count, 64, 0, st>>>(seg_rows.ptr, seg_keys.ptr, seg_cnt.ptr, count, n_seg,
                                                               k, rows_tmp.ptr, keys_tmp.ptr, cnt_tmp.ptr);
         KY_HIP(hipGetLastError());
@@ -648,7 +692,9 @@ void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32
     if (!list.empty()) {
         DevBuf<uint32_t> d_list(list.size());
         KY_HIP(hipMemcpyAsync(d_list.ptr, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, st));
-        knn_exact_rows(ctx, pts, n, d, k, d_list.ptr, list.size(), keys, rows);
+        // Rows that went through the re-rank hold their candidates' k smallest exact
+        // pairs: an upper bound of their k-th pair that seeds the exact search.
+        knn_exact_rows(ctx, pts, n, d, k, d_list.ptr, list.size(), keys, rows, prep.filter_ok);
         KY_HIP(hipStreamSynchronize(st));
     }
     fallback_ms = timer.stop();

@@ -30,8 +30,11 @@ void knn_rerank(Context &ctx, const float *pts, const KnnPrep &prep, const uint3
 void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32_t k, int precision,
                   const uint32_t *q_rows, uint64_t nq, double *keys, uint32_t *rows, ky_knn_info *info);
 uint32_t check_search(const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric);
+// seeded: d_keys / d_rows already hold, at each listed row, k exact (key, row) pairs
+// ascending (a re-ranked candidate set, INFINITY where missing); the search keeps only
+// rows at or below the k-th of them.
 void knn_exact_rows(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32_t k, const uint32_t *d_list,
-                    uint64_t count, double *d_keys, uint32_t *d_rows);
+                    uint64_t count, double *d_keys, uint32_t *d_rows, bool seeded = false);
 void knn_graph(Context &ctx, const double *d_keys, const uint32_t *d_rows, uint64_t n, uint32_t k, uint32_t *g_idx,
                float *g_dist);
 void knn_pack(Context &ctx, const float *x, uint64_t n, uint32_t d, uint32_t ld, float *out);

@@ -63,46 +63,80 @@ constexpr int QPB = 4 * QPW;    // queries per block
 // ---------------------------------------------------------------- preparation ----
 
 // Row i of the scan order is original row perm[i]: P', Q' written at i, err and |b|
-// at the original row.
-__global__ void bf16_rows(const float *__restrict__ x, const double *__restrict__ mean,
-                          const float *__restrict__ cent, const uint32_t *__restrict__ perm, uint64_t n, uint32_t d,
-                          uint32_t dp, __bf16 *__restrict__ P, __bf16 *__restrict__ Q, double *__restrict__ err,
-                          double *__restrict__ nb, unsigned long long *__restrict__ err_max,
-                          unsigned long long *__restrict__ nb_max) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t r = perm[i];
-    double o = 0.0, e2 = 0.0;
-    __bf16 *pr = P + i * KDIM, *qr = Q + i * KDIM;
-    for (uint32_t j = 0; j < d; j++) {
-        __bf16 b = (__bf16)cent[r * dp + j];
-        double bv = (double)(float)b;
-        double exact = (double)x[r * d + j] - mean[j];
-        double t = bv - exact;
-        e2 += t * t;
-        o += bv * bv;
-        pr[j] = (__bf16)(-2.0f * (float)b);
-        qr[j] = b;
+// at the original row. One wavefront a row (lane j: component j; fixed butterflies
+// add o = |b|^2 and err^2 over the lanes), BROWS rows a block, one atomic a block for
+// the largest err and |b|.
+constexpr unsigned BROWS = 64;
+
+__global__ __launch_bounds__(256) void bf16_rows(const float *__restrict__ x, const double *__restrict__ mean,
+                                                 const float *__restrict__ cent, const uint32_t *__restrict__ perm,
+                                                 uint64_t n, uint32_t d, uint32_t dp, __bf16 *__restrict__ P,
+                                                 __bf16 *__restrict__ Q, double *__restrict__ err,
+                                                 double *__restrict__ nb, unsigned long long *__restrict__ err_max,
+                                                 unsigned long long *__restrict__ nb_max) {
+    __shared__ double wmax[2][4];
+    const unsigned lane = lane_id(), wave = threadIdx.x / 64;
+    const double mu = lane < d ? mean[lane] : 0.0;
+    double emx = 0.0, nmx = 0.0;
+    for (unsigned t = wave; t < BROWS; t += 4) {
+        const uint64_t i = (uint64_t)blockIdx.x * BROWS + t;
+        if (i >= n) break;
+        const uint64_t r = perm[i];
+        __bf16 b = (__bf16)0.0f;
+        double o = 0.0, e2 = 0.0;
+        if (lane < d) {
+            b = (__bf16)cent[r * dp + lane];
+            const double bv = (double)(float)b;
+            const double exact = (double)x[r * d + lane] - mu;
+            const double tt = bv - exact;
+            e2 = tt * tt;
+            o = bv * bv;
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off /= 2) {
+            o += __shfl_xor(o, off);
+            e2 += __shfl_xor(e2, off);
+        }
+        const __bf16 hi = (__bf16)(float)o;
+        const double r1 = o - (double)(float)hi;
+        const __bf16 mid = (__bf16)(float)r1;
+        const double r2 = r1 - (double)(float)mid;
+        const __bf16 lo = (__bf16)(float)r2;
+        const __bf16 one = (__bf16)1.0f, zero = (__bf16)0.0f;
+        __bf16 pv = zero, qv = zero;
+        if (lane < d) {
+            pv = (__bf16)(-2.0f * (float)b);
+            qv = b;
+        } else if (lane < d + 3) {
+            const __bf16 part = lane == d ? hi : lane == d + 1 ? mid : lo;
+            pv = part;
+            qv = one;
+        } else if (lane < d + 6) {
+            const unsigned u = lane - (d + 3);
+            pv = one;
+            qv = u == 0 ? hi : u == 1 ? mid : lo;
+        }
+        P[i * KDIM + lane] = pv;
+        Q[i * KDIM + lane] = qv;
+        if (lane == 0) {
+            const double e = sqrt(e2), nrm = sqrt(o);
+            err[r] = e;
+            nb[r] = nrm;
+            emx = fmax(emx, e);
+            nmx = fmax(nmx, nrm);
+        }
     }
-    __bf16 hi = (__bf16)(float)o;
-    double r1 = o - (double)(float)hi;
-    __bf16 mid = (__bf16)(float)r1;
-    double r2 = r1 - (double)(float)mid;
-    __bf16 lo = (__bf16)(float)r2;
-    const __bf16 one = (__bf16)1.0f, zero = (__bf16)0.0f;
-    pr[d] = hi; pr[d + 1] = mid; pr[d + 2] = lo;
-    pr[d + 3] = one; pr[d + 4] = one; pr[d + 5] = one;
-    qr[d] = one; qr[d + 1] = one; qr[d + 2] = one;
-    qr[d + 3] = hi; qr[d + 4] = mid; qr[d + 5] = lo;
-    for (uint32_t j = d + 6; j < KDIM; j++) {
-        pr[j] = zero;
-        qr[j] = zero;
+    if (lane == 0) {
+        wmax[0][wave] = emx;
+        wmax[1][wave] = nmx;
     }
-    double e = sqrt(e2), nrm = sqrt(o);
-    err[r] = e;
-    nb[r] = nrm;
-    atomicMax(err_max, (unsigned long long)__double_as_longlong(e));
-    atomicMax(nb_max, (unsigned long long)__double_as_longlong(nrm));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double e = fmax(fmax(wmax[0][0], wmax[0][1]), fmax(wmax[0][2], wmax[0][3]));
+        const double m = fmax(fmax(wmax[1][0], wmax[1][1]), fmax(wmax[1][2], wmax[1][3]));
+        atomicMax(err_max, (unsigned long long)__double_as_longlong(e));
+        atomicMax(nb_max, (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 // Padding rows of P' (after the last row): key 1e30 against every query.
@@ -113,13 +147,23 @@ __global__ void pad_rows(__bf16 *__restrict__ P, uint64_t first, uint64_t count,
     P[first * KDIM + t] = (__bf16)(k == d ? 1e30f : 0.0f);
 }
 
-// Largest |value| of each of the first `c` components (bits of non-negative floats
-// order as the floats).
-__global__ void col_absmax(const float *__restrict__ cent, uint64_t n, uint32_t dp, uint32_t c,
-                           unsigned *__restrict__ out) {
-    uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    for (uint32_t j = 0; j < c; j++) atomicMax(out + j, __float_as_uint(fabsf(cent[r * dp + j])));
+// Largest |value| of each of the first `c` (<= 3) components (bits of non-negative
+// floats order as the floats): a grid-stride loop, a block maximum, one atomic a block
+// and component.
+__global__ __launch_bounds__(256) void col_absmax(const float *__restrict__ cent, uint64_t n, uint32_t dp, uint32_t c,
+                                                  unsigned *__restrict__ out) {
+    __shared__ unsigned part[3][256];
+    unsigned m[3] = {0u, 0u, 0u};
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x)
+        for (uint32_t j = 0; j < c; j++) m[j] = max(m[j], __float_as_uint(fabsf(cent[r * dp + j])));
+    for (uint32_t j = 0; j < 3; j++) part[j][threadIdx.x] = m[j];
+    __syncthreads();
+    for (unsigned s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (uint32_t j = 0; j < 3; j++) part[j][threadIdx.x] = max(part[j][threadIdx.x], part[j][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < c) atomicMax(out + threadIdx.x, part[threadIdx.x][0]);
 }
 
 // The Morton code of a row's first three components, each quantised to 10 bits over
@@ -156,21 +200,6 @@ __global__ void query_positions(const uint32_t *__restrict__ q_rows, uint64_t nq
     if (i >= nq) return;
     keys[i] = inv[q_rows[i]];
     vals[i] = (uint32_t)i;
-}
-
-// The queries (scan-order row, output slot) whose count is below `need`, listed in
-// any order.
-__global__ void short_queries(const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq,
-                              const uint32_t *__restrict__ count, uint32_t need, uint32_t *__restrict__ pos,
-                              uint32_t *__restrict__ out, uint32_t *__restrict__ n_out) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    const uint32_t o = qout[i];
-    if (count[o] < need) {
-        const uint32_t e = atomicAdd(n_out, 1u);
-        pos[e] = qpos ? qpos[i] : (uint32_t)i;
-        out[e] = o;
-    }
 }
 
 __global__ void bf16_delta(const double *__restrict__ nb, uint64_t n, double nb_max, double c,
@@ -731,7 +760,7 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
     DevBuf<unsigned> absmax(3);
     KY_HIP(hipMemsetAsync(absmax.ptr, 0, 3 * sizeof(unsigned), st));
     const uint32_t c = std::min<uint32_t>(3, prep.d);
-    col_absmax<<<div_up(n, 256), 256, 0, st>>>(prep.cent.ptr, n, prep.dp, c, absmax.ptr);
+    col_absmax<<<std::min<unsigned>(1024, div_up(n, 256)), 256, 0, st>>>(prep.cent.ptr, n, prep.dp, c, absmax.ptr);
     DevBuf<uint32_t> keys(n), rows(n), skeys(n), perm(n);
     morton_keys<<<div_up(n, 256), 256, 0, st>>>(prep.cent.ptr, n, prep.dp, c, absmax.ptr, keys.ptr, rows.ptr);
     KY_HIP(hipGetLastError());
@@ -747,7 +776,7 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
     DevBuf<double> nb(n);
     DevBuf<unsigned long long> maxima(2);
     KY_HIP(hipMemsetAsync(maxima.ptr, 0, 2 * sizeof(unsigned long long), st));
-    bf16_rows<<<div_up(n, 256), 256, 0, st>>>(pts, prep.mean.ptr, prep.cent.ptr, perm.ptr, n, prep.d, prep.dp, P.ptr,
+    bf16_rows<<<div_up(n, BROWS), 256, 0, st>>>(pts, prep.mean.ptr, prep.cent.ptr, perm.ptr, n, prep.d, prep.dp, P.ptr,
                                                Q.ptr, err.ptr, nb.ptr, maxima.ptr, maxima.ptr + 1);
     KY_HIP(hipGetLastError());
     unsigned long long hm[2];
@@ -804,23 +833,9 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
                                                d_rows, d_keys, d_count, d_thr);
     KY_HIP(hipGetLastError());
     KY_HIP(hipEventRecord(ev[1], st));
-    if (thr_est.ptr) {
-        // Phase 3: the queries whose estimate left fewer than k rows at or below it,
-        // scanned again from no threshold (with k or more, the re-rank bounds the rows
-        // left out by the threshold itself).
-        DevBuf<uint32_t> re_pos(nq), re_out(nq), re_n(1);
-        KY_HIP(hipMemsetAsync(re_n.ptr, 0, sizeof(uint32_t), st));
-        short_queries<<<div_up(nq, 256), 256, 0, st>>>(qpos, qout, nq, d_count, k, re_pos.ptr, re_out.ptr, re_n.ptr);
-        KY_HIP(hipGetLastError());
-        uint32_t h_n = 0;
-        KY_HIP(hipMemcpyAsync(&h_n, re_n.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        KY_HIP(hipStreamSynchronize(st));
-        if (h_n) {
-            search_bf16<<<div_up(h_n, QPB), 256, 0, st>>>(P.ptr, Q.ptr, n, 1, re_pos.ptr, re_out.ptr, h_n, perm.ptr, m,
-                                                         nullptr, buf.ptr, d_rows, d_keys, d_count, d_thr);
-            KY_HIP(hipGetLastError());
-        }
-    }
+    // A query whose starting threshold left fewer than k rows at or below it ends with
+    // fewer than k candidates: the re-rank leaves it uncertified and the exact search
+    // takes it (knn.hip).
     KY_HIP(hipStreamSynchronize(st));
     float ms = 0.0f;
     KY_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
