@@ -67,6 +67,17 @@ int ky_device_info(ky_ctx *ctx, char *name, size_t name_len, uint64_t *total_byt
  * pca/gpu.rs:62-70). */
 int ky_csr_upload(ky_ctx *ctx, const uint64_t *indptr, const uint32_t *indices,
                   const float *values, uint64_t n_rows, uint32_t n_cols, ky_csr **out);
+/* Upload a counts CSR and make the layer on the device: normalize_total(target_sum)
+ * when target_sum > 0, then log1p when `log1p` is 1, in place of the host producing
+ * the f32 layer (crates/kyto-exec/src/elementwise.rs:1-24, normalize_divisor
+ * :262-269, kernels/log1p.rs): row totals over kyto's 8 f64 lanes, divisor
+ * f32(total) / f32(target_sum) (0 becomes 1), f32 division, ln(1 + v) in f64 rounded
+ * once. KY_E215 for a negative or non-finite target_sum, and as ky_csr_upload. */
+int ky_csr_upload_counts(ky_ctx *ctx, const uint64_t *indptr, const uint32_t *indices, const float *counts,
+                         uint64_t n_rows, uint32_t n_cols, double target_sum, int log1p, ky_csr **out);
+/* The resident layer's values (nnz floats, in row order): the layer a
+ * ky_csr_upload_counts made, for a sink of the layer or a check of its bits. */
+int ky_csr_values(const ky_csr *csr, float *values);
 int ky_csr_free(ky_csr *csr);
 int ky_csr_shape(const ky_csr *csr, uint64_t *n_rows, uint32_t *n_cols, uint64_t *nnz);
 

@@ -2,7 +2,8 @@
 kyto's plan API (crates/kyto-py/src/atlas.rs:822-966, 1622-1720).
 
     import kyto_amd as ky
-    out = (ky.scan(layer)                       # a CSR layer (cells x genes)
+    out = (ky.scan(counts)                      # a CSR layer (cells x genes)
+             .normalize_total(1e4).log1p()      # atlas.rs:578-632, made on the device
              .pca(n_components=50, seed=0)      # atlas.rs:822-856
              .neighbors(n_neighbors=15)         # atlas.rs:857-966
              .on(device="gpu")                  # atlas.rs:1622-1636
@@ -95,6 +96,15 @@ def _as_layer(x) -> _Layer:
 
 
 @dataclass(frozen=True)
+class _Map:
+    """normalize_total(target_sum) or log1p over a layer (atlas.rs:578-630)."""
+    kind: str          # "normalize_total" or "log1p"
+    layer: str
+    into: str
+    target_sum: float | None = None
+
+
+@dataclass(frozen=True)
 class _Pca:
     layer: str | None
     into: str
@@ -126,18 +136,58 @@ class Atlas:
 
     # ----------------------------------------------------------------- steps --
 
+    def _latest_layer(self) -> str:
+        """The layer the latest normalize_total / log1p of the plan wrote, else the
+        default layer (atlas.rs:610-620)."""
+        maps = [s for s in self.steps if isinstance(s, _Map)]
+        return maps[-1].into if maps else self.default_layer
+
+    def _layer_names(self) -> set:
+        return set(self.layers) | {s.into for s in self.steps if isinstance(s, _Map)}
+
+    def _add_map(self, kind, layer, into, default_into, target_sum=None) -> "Atlas":
+        name = layer if layer is not None else self._latest_layer()
+        if name not in self._layer_names():
+            raise KytoError(109, f"{kind}: layer {name!r} is not a layer of the plan")
+        if into is None:
+            if default_into in self._layer_names():
+                raise KytoError(220, f"{kind}: the plan already has the layer {default_into!r}; pass into= to name "
+                                     f"the output")
+            into = default_into
+        return replace(self, steps=self.steps + (_Map(kind, name, into, target_sum),))
+
+    def normalize_total(self, target_sum=None, *, layer=None, into=None) -> "Atlas":
+        """atlas.rs:578-591: each cell's values divided by its total over target_sum
+        (kyto's f32 steps, on the device). The median target (target_sum=None) needs a
+        statistics pass over every cell that this path does not run: KY-E501."""
+        if target_sum is None:
+            raise KytoError(KY_E501, "normalize_total: the median target (target_sum=None) is not on the device "
+                                     "path; pass target_sum=")
+        if isinstance(target_sum, bool) or not isinstance(target_sum, (int, float, np.integer, np.floating)):
+            raise _invalid("normalize_total", "target_sum", target_sum, "not a number", "a float above 0")
+        target_sum = float(target_sum)
+        if not (np.isfinite(target_sum) and target_sum > 0):
+            raise _invalid("normalize_total", "target_sum", target_sum, "not a finite number above 0",
+                           "a float above 0")
+        return self._add_map("normalize_total", layer, into, "normalized", target_sum)
+
+    def log1p(self, *, layer=None, into=None) -> "Atlas":
+        """atlas.rs:628-632: log(1 + x) of every value (kyto's log1p, on the device)."""
+        return self._add_map("log1p", layer, into, "log1p")
+
     def pca(self, *, layer=None, into=None, n_components=None, seed=None, genes=None) -> "Atlas":
-        """atlas.rs:822-856: n_components defaults to 50, seed is required."""
+        """atlas.rs:822-856: n_components defaults to 50, seed is required; layer None
+        reads the layer the latest normalize_total / log1p wrote, else the default."""
         if seed is None:
             raise _invalid("pca", "seed", None, "a seed is required (R7)", "an int of at least 0")
         seed = _integer("pca", "seed", seed, 0)
         n_components = 50 if n_components is None else _integer("pca", "n_components", n_components, 1)
-        name = layer if layer is not None else self.default_layer
-        if name not in self.layers:
+        name = layer if layer is not None else self._latest_layer()
+        if name not in self._layer_names():
             raise KytoError(109, f"pca: layer {name!r} is not a layer of the plan")
         if genes is not None:
             genes = np.asarray(genes, bool)
-            if genes.shape != (self.layers[name].n_genes,):
+            if genes.shape != (self.layers[self._source(name)[0]].n_genes,):
                 raise _invalid("pca", "genes", "mask", "not one flag per gene", "a boolean mask over the genes")
             if not genes.any():
                 raise _invalid("pca", "genes", "mask", "the predicate is true for none of the genes reaching pca",
@@ -170,6 +220,15 @@ class Atlas:
         return replace(self, steps=self.steps + (
             _Neighbors(embedding, n_neighbors, n_components, metric, search, seed, into, precision),))
 
+    def _source(self, name):
+        """The scanned layer a layer is made from and the maps in order."""
+        maps = []
+        while name not in self.layers:
+            step = next(s for s in reversed(self.steps) if isinstance(s, _Map) and s.into == name)
+            maps.append(step)
+            name = step.layer
+        return name, maps[::-1]
+
     def on(self, device="auto") -> "Atlas":
         """atlas.rs:1622-1636: 'auto', 'gpu', 'gpu:N', an int N, or 'cpu'."""
         if isinstance(device, (int, np.integer)) and not isinstance(device, bool):
@@ -185,7 +244,10 @@ class Atlas:
     def explain(self) -> str:
         lines = []
         for s in reversed(self.steps):
-            if isinstance(s, _Pca):
+            if isinstance(s, _Map):
+                extra = f" target_sum={s.target_sum:g}" if s.kind == "normalize_total" else ""
+                lines.append(f'{s.kind.upper()} "{s.layer}" -> "{s.into}"{extra} [device: {self.device}]')
+            elif isinstance(s, _Pca):
                 lines.append(f'PCA "{s.layer}" -> "{s.into}" n_components={s.n_components} seed={s.seed} '
                              f"[device: {self.device}]")
             else:
@@ -210,10 +272,13 @@ class Atlas:
         resident: dict[str, DenseMatrix] = {}
         try:
             for step in self.steps:
+                if isinstance(step, _Map):
+                    continue  # applied on the device by the step that reads the layer
                 if isinstance(step, _Pca):
-                    layer = self.layers[step.layer]
+                    source, maps = self._source(step.layer)
+                    layer = self.layers[source]
                     _fits(layer, step, limit)
-                    res, emb = _run_pca(dev, layer, step)
+                    res, emb = _run_pca(dev, layer, step, _device_maps(maps))
                     resident[step.into] = emb
                     out[step.into] = res.embedding
                     out[f"{step.into}_loadings"] = res.loadings
@@ -247,14 +312,40 @@ def _fits(layer: _Layer, step: _Pca, limit):
                                  f"memory_limit={limit}; pass a larger memory_limit")
 
 
-def _run_pca(dev: Device, layer: _Layer, step: _Pca):
+def _device_maps(maps):
+    """The maps before a step as the device producer takes them: an optional
+    normalize_total then an optional log1p (ky_csr_upload_counts), or None."""
+    kinds = [m.kind for m in maps]
+    if not kinds:
+        return None
+    if kinds not in (["normalize_total"], ["log1p"], ["normalize_total", "log1p"]):
+        raise KytoError(KY_E501, f"the device makes a layer by normalize_total then log1p, not {' then '.join(kinds)}")
+    target = maps[0].target_sum if kinds[0] == "normalize_total" else None
+    return target, kinds[-1] == "log1p"
+
+
+def _upload(dev: Device, layer: _Layer, maps):
+    if maps is None:
+        return dev.upload_csr(layer.indptr, layer.indices, layer.values, layer.n_genes)
+    target, log1p = maps
+    return dev.upload_counts(layer.indptr, layer.indices, layer.values, layer.n_genes, target_sum=target, log1p=log1p)
+
+
+def _run_pca(dev: Device, layer: _Layer, step: _Pca, maps=None):
     if step.genes is None:
-        csr = dev.upload_csr(layer.indptr, layer.indices, layer.values, layer.n_genes)
+        csr = _upload(dev, layer, maps)
         try:
             res = dev.pca(csr, step.n_components, step.seed, PCA_POWER_ITERATIONS)
         finally:
             csr.free()
         return res, res.device_embedding
+    if maps is not None:
+        # The maps see every gene of a cell (its total), then the gene set is taken.
+        made = _upload(dev, layer, maps)
+        try:
+            layer = _Layer(layer.indptr, layer.indices, made.values(), layer.n_genes)
+        finally:
+            made.free()
     # A gene set: the layer's columns of the set, in order (pca.rs: the sums run in
     # the order a filter_genes before the node gives them), zero loadings elsewhere.
     flags = step.genes

@@ -84,6 +84,8 @@ SIGNATURES = {
     "ky_pca_project": [c_void_p, c_void_p, f64p, c_uint32, f64p, f64p, f32p, POINTER(c_void_p), f64p, u64p],
     "ky_pca": [c_void_p, c_void_p, c_uint32, c_uint64, c_uint32, f32p, POINTER(c_void_p), f32p, f64p, f64p,
                f64p, u32p, POINTER(PcaTiming)],
+    "ky_csr_upload_counts": [c_void_p, u64p, u32p, f32p, c_uint64, c_uint32, c_double, c_int, POINTER(c_void_p)],
+    "ky_csr_values": [c_void_p, f32p],
     "ky_neighbors": [c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_int, u32p, f32p, POINTER(KnnInfo)],
     "ky_nearest_keys": [c_void_p, c_void_p, c_uint32, c_uint64, c_uint64, c_uint32, c_int, f32p, u32p],
     "ky_ivf_build": [c_void_p, c_void_p, POINTER(IvfParams), POINTER(c_void_p), POINTER(IvfInfo)],

@@ -2,6 +2,7 @@
 // thread-local messages, no exception crossing the ABI.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -151,6 +152,46 @@ int ky_csr_upload(ky_ctx *ctx, const uint64_t *indptr, const uint32_t *indices, 
             throw;
         }
         *out = csr;
+    });
+}
+
+int ky_csr_upload_counts(ky_ctx *ctx, const uint64_t *indptr, const uint32_t *indices, const float *counts,
+                         uint64_t n_rows, uint32_t n_cols, double target_sum, int log1p, ky_csr **out) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(indptr, "indptr");
+        need(out, "out");
+        KY_CHECK(target_sum >= 0.0 && std::isfinite(target_sum), KY_E215,
+                 "normalize_total: target_sum must be a finite number above 0 (or 0: no normalize_total)");
+        use_device(*ctx);
+        if (indptr[n_rows] > 0) {
+            need(indices, "indices");
+            need(counts, "counts");
+        }
+        ky_csr *csr = new ky_csr();
+        try {
+            csr->ctx = ctx;
+            csr->n_rows = n_rows;
+            csr->n_cols = n_cols;
+            LayerMaps maps;
+            maps.normalize = target_sum > 0.0;
+            maps.target_sum = target_sum;
+            maps.log1p = log1p != 0;
+            csr_build(*csr, indptr, indices, counts, maps);
+        } catch (...) {
+            delete csr;
+            throw;
+        }
+        *out = csr;
+    });
+}
+
+int ky_csr_values(const ky_csr *csr, float *values) {
+    return guarded([&] {
+        need(csr, "csr");
+        need(values, "values");
+        use_device(*csr->ctx);
+        if (csr->nnz) download(*csr->ctx, values, csr->values.ptr, sizeof(float) * csr->nnz);
     });
 }
 

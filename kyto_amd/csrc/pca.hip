@@ -26,6 +26,7 @@
 #include "common.hpp"
 #include "dense.hpp"
 #include "device.hpp"
+#include "layer.hpp"
 #include "pca.hpp"
 
 namespace ky {
@@ -393,7 +394,8 @@ static unsigned lds_limit(const Context &ctx) {
     return (unsigned)std::min<size_t>(ctx.props.sharedMemPerBlock, 64 * 1024);
 }
 
-void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const float *values) {
+void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const float *values,
+               const LayerMaps &maps) {
     Context &ctx = *csr.ctx;
     const uint64_t n = csr.n_rows;
     KY_CHECK(n >= 1, KY_E215, "csr: a layer without cells");
@@ -413,6 +415,8 @@ void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const 
         KY_HIP(hipMemcpyAsync(csr.indices.ptr, indices, sizeof(uint32_t) * csr.nnz, hipMemcpyHostToDevice, st));
         KY_HIP(hipMemcpyAsync(csr.values.ptr, values, sizeof(float) * csr.nnz, hipMemcpyHostToDevice, st));
     }
+    // The input layer's maps on the uploaded values (normalize_total, log1p).
+    normalize_log1p(st, csr.indptr.ptr, n, csr.values.ptr, maps.target_sum, maps.normalize, maps.log1p);
     DevBuf<unsigned> flags(1);
     KY_HIP(hipMemsetAsync(flags.ptr, 0, sizeof(unsigned), st));
     check_rows<<<div_up(n, 256), 256, 0, st>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, n, csr.n_cols,

@@ -11,7 +11,15 @@ constexpr uint32_t CHUNK_ROWS = 8192;
 constexpr uint32_t PCA_OVERSAMPLES = 10;
 
 void ordered_sum(hipStream_t st, const double *partial, uint32_t n_parts, uint64_t stride, double *tmp, double *out);
-void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const float *values);
+// Maps applied on the device to the uploaded values before the layer is checked and
+// its columns built: kyto's normalize_total(target_sum) and log1p.
+struct LayerMaps {
+    bool normalize = false;
+    double target_sum = 0.0;
+    bool log1p = false;
+};
+void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const float *values,
+               const LayerMaps &maps = LayerMaps());
 void pca_sketch(Csr &csr, uint64_t seed, uint64_t cell_offset, uint32_t l, double *sketch, double *gene_sums,
                 double *omega_sums);
 void pca_product(Csr &csr, const double *q, uint32_t w, const double *block_means, double *product, double *sums);

@@ -94,6 +94,11 @@ class Device:
     def upload_csr(self, indptr, indices, values, n_genes: int) -> "CsrLayer":
         return CsrLayer(self, indptr, indices, values, n_genes)
 
+    def upload_counts(self, indptr, indices, counts, n_genes: int, target_sum=1e4, log1p=True) -> "CsrLayer":
+        """A counts CSR made into the layer on the device: kyto's normalize_total
+        (target_sum; None skips it) then log1p (ky_csr_upload_counts)."""
+        return CsrLayer(self, indptr, indices, counts, n_genes, target_sum=target_sum, log1p=log1p)
+
     def upload_dense(self, x) -> "DenseMatrix":
         return DenseMatrix.upload(self, x)
 
@@ -201,7 +206,10 @@ def _info(info) -> dict:
 class CsrLayer:
     """A CSR layer resident on the device (ky_csr_upload)."""
 
-    def __init__(self, dev: Device, indptr, indices, values, n_genes: int):
+    def __init__(self, dev: Device, indptr, indices, values, n_genes: int, target_sum=None, log1p=False):
+        """values as given (ky_csr_upload); or counts made into the layer on the device
+        (ky_csr_upload_counts): normalize_total(target_sum) when target_sum is set,
+        then log1p when `log1p`."""
         self.dev = dev
         indptr = np.ascontiguousarray(indptr, np.uint64)
         indices = np.ascontiguousarray(indices, np.uint32)
@@ -210,13 +218,24 @@ class CsrLayer:
         self.n_genes = int(n_genes)
         self.nnz = int(indptr[-1]) if len(indptr) else 0
         h = c_void_p()
-        N.check(dev._lib.ky_csr_upload(dev.handle, N.ptr(indptr, N.u64p), N.ptr(indices, N.u32p),
-                                       N.ptr(values, N.f32p), self.n_rows, self.n_genes, byref(h)))
+        if target_sum is None and not log1p:
+            N.check(dev._lib.ky_csr_upload(dev.handle, N.ptr(indptr, N.u64p), N.ptr(indices, N.u32p),
+                                           N.ptr(values, N.f32p), self.n_rows, self.n_genes, byref(h)))
+        else:
+            N.check(dev._lib.ky_csr_upload_counts(dev.handle, N.ptr(indptr, N.u64p), N.ptr(indices, N.u32p),
+                                                  N.ptr(values, N.f32p), self.n_rows, self.n_genes,
+                                                  float(target_sum or 0.0), 1 if log1p else 0, byref(h)))
         self._h = h
 
     @property
     def handle(self):
         return self._h
+
+    def values(self) -> np.ndarray:
+        """The resident values (nnz floats, row order): ky_csr_values."""
+        out = np.empty(max(self.nnz, 1), np.float32)
+        N.check(self.dev._lib.ky_csr_values(self._h, N.ptr(out, N.f32p)))
+        return out[: self.nnz]
 
     def free(self):
         if self._h:

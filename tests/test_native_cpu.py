@@ -132,3 +132,22 @@ def test_synthetic_generator_is_deterministic_and_hits_density():
     norm, log = oracle.normalize_log1p(a.indptr, a.values)
     S.normalize_log1p(a)
     assert np.array_equal(a.values.view(np.uint32), log.view(np.uint32))
+
+
+def test_plan_layer_maps_are_checked():
+    layer = (np.array([0, 1, 2], np.uint64), np.array([0, 1], np.uint32), np.array([1.0, 2.0], np.float32), 3)
+    a = kyto_amd.scan(layer)
+    with pytest.raises(kyto_amd.KytoError, match="KY-E501|501"):
+        a.normalize_total()
+    with pytest.raises(kyto_amd.KytoError, match="KY-E215|215"):
+        a.normalize_total(-1.0)
+    b = a.normalize_total(1e4).log1p()
+    assert b.explain().splitlines()[:2] == ['LOG1P "normalized" -> "log1p" [device: auto]',
+                                            'NORMALIZE_TOTAL "X" -> "normalized" target_sum=10000 [device: auto]']
+    with pytest.raises(kyto_amd.KytoError, match="220"):
+        b.log1p(layer="X")
+    # pca reads the latest written layer.
+    assert b.pca(seed=0).steps[-1].layer == "log1p"
+    assert b.pca(seed=0, layer="X").steps[-1].layer == "X"
+    with pytest.raises(kyto_amd.KytoError):
+        b.pca(seed=0, layer="nope")
