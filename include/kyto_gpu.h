@@ -78,6 +78,33 @@ int ky_csr_upload_counts(ky_ctx *ctx, const uint64_t *indptr, const uint32_t *in
 /* The resident layer's values (nnz floats, in row order): the layer a
  * ky_csr_upload_counts made, for a sink of the layer or a check of its bits. */
 int ky_csr_values(const ky_csr *csr, float *values);
+/* A store's row blocks: fill rows [first_row, first_row + n_rows) of the layer as a
+ * CSR of their own — indptr (n_rows + 1 entries, from 0), then, when the block's
+ * values fit `capacity`, its indices and values — and return the block's number of
+ * values (a number above capacity asks for the block again with larger buffers; a
+ * negative number is a failure). The buffers are pinned host memory of the library. */
+typedef int64_t (*ky_block_reader)(void *user, uint64_t first_row, uint64_t n_rows, uint64_t *indptr,
+                                   uint32_t *indices, float *values, uint64_t capacity);
+
+typedef struct ky_stream_info {
+    uint64_t blocks;  /* blocks read */
+    uint64_t nnz;     /* values of the layer */
+    double read_ms;   /* time spent in the reader (host) */
+    double total_ms;  /* the whole upload, the layer's maps and column copy included */
+    double bytes;     /* bytes copied to the device */
+} ky_stream_info;
+
+/* The layer read block by block (kyto-exec's block stream over a store, crates/
+ * kyto-exec/src/pipeline.rs): `block_rows` cells a block through two pinned buffer
+ * sets, the reader filling one while the other is copied to the device on a stream of
+ * its own and the previous block's maps (normalize_total(target_sum) when
+ * target_sum > 0, then log1p when `log1p` is 1, as ky_csr_upload_counts) run, so
+ * reading, transfer and device work overlap and the host holds at most two blocks
+ * (memory_limit). nnz_hint (0: estimated from the first block) sizes the resident
+ * arrays. KY_E215 as ky_csr_upload; KY_E505 when the reader fails. */
+int ky_csr_stream(ky_ctx *ctx, uint64_t n_rows, uint32_t n_cols, uint64_t block_rows, ky_block_reader reader,
+                  void *user, uint64_t nnz_hint, double target_sum, int log1p, ky_csr **out,
+                  ky_stream_info *info);
 int ky_csr_free(ky_csr *csr);
 int ky_csr_shape(const ky_csr *csr, uint64_t *n_rows, uint32_t *n_cols, uint64_t *nnz);
 

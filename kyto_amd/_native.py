@@ -50,6 +50,16 @@ class KnnInfo(ctypes.Structure):
                 ("total_ms", c_double), ("scan_kernel_ms", c_double)]
 
 
+class StreamInfo(ctypes.Structure):
+    _fields_ = [("blocks", c_uint64), ("nnz", c_uint64), ("read_ms", c_double), ("total_ms", c_double),
+                ("bytes", c_double)]
+
+
+# ky_block_reader: (user, first_row, n_rows, indptr, indices, values, capacity) -> nnz
+BlockReader = ctypes.CFUNCTYPE(ctypes.c_int64, c_void_p, c_uint64, c_uint64, POINTER(c_uint64), POINTER(c_uint32),
+                               POINTER(c_float), c_uint64)
+
+
 class IvfParams(ctypes.Structure):
     _fields_ = [("n_lists", c_uint32), ("kmeans_iters", c_uint32), ("seed", c_uint64),
                 ("hidden", c_uint32), ("train_queries", c_uint32), ("batch", c_uint32),
@@ -86,6 +96,8 @@ SIGNATURES = {
                f64p, u32p, POINTER(PcaTiming)],
     "ky_csr_upload_counts": [c_void_p, u64p, u32p, f32p, c_uint64, c_uint32, c_double, c_int, POINTER(c_void_p)],
     "ky_csr_values": [c_void_p, f32p],
+    "ky_csr_stream": [c_void_p, c_uint64, c_uint32, c_uint64, BlockReader, c_void_p, c_uint64, c_double, c_int,
+                      POINTER(c_void_p), POINTER(StreamInfo)],
     "ky_neighbors": [c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_int, u32p, f32p, POINTER(KnnInfo)],
     "ky_nearest_keys": [c_void_p, c_void_p, c_uint32, c_uint64, c_uint64, c_uint32, c_int, f32p, u32p],
     "ky_ivf_build": [c_void_p, c_void_p, POINTER(IvfParams), POINTER(c_void_p), POINTER(IvfInfo)],
@@ -144,6 +156,8 @@ def synth_lib():
         lib.ks_count.argtypes = [POINTER(SynthSpec), u64p]
         lib.ks_fill.restype = None
         lib.ks_fill.argtypes = [POINTER(SynthSpec), u64p, u32p, f32p]
+        lib.ks_block.restype = c_uint64
+        lib.ks_block.argtypes = [POINTER(SynthSpec), c_uint64, c_uint64, c_void_p, c_void_p, c_void_p, c_uint64]
         lib.ks_cell_types.restype = None
         lib.ks_cell_types.argtypes = [POINTER(SynthSpec), u32p]
         lib.ks_normalize_log1p.restype = None

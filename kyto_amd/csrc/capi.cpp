@@ -186,6 +186,33 @@ int ky_csr_upload_counts(ky_ctx *ctx, const uint64_t *indptr, const uint32_t *in
     });
 }
 
+int ky_csr_stream(ky_ctx *ctx, uint64_t n_rows, uint32_t n_cols, uint64_t block_rows, ky_block_reader reader,
+                  void *user, uint64_t nnz_hint, double target_sum, int log1p, ky_csr **out,
+                  ky_stream_info *info) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(out, "out");
+        KY_CHECK(target_sum >= 0.0 && std::isfinite(target_sum), KY_E215,
+                 "normalize_total: target_sum must be a finite number above 0 (or 0: no normalize_total)");
+        use_device(*ctx);
+        ky_csr *csr = new ky_csr();
+        try {
+            csr->ctx = ctx;
+            csr->n_rows = n_rows;
+            csr->n_cols = n_cols;
+            LayerMaps maps;
+            maps.normalize = target_sum > 0.0;
+            maps.target_sum = target_sum;
+            maps.log1p = log1p != 0;
+            csr_stream(*csr, block_rows, reader, user, nnz_hint, maps, info);
+        } catch (...) {
+            delete csr;
+            throw;
+        }
+        *out = csr;
+    });
+}
+
 int ky_csr_values(const ky_csr *csr, float *values) {
     return guarded([&] {
         need(csr, "csr");

@@ -394,27 +394,12 @@ static unsigned lds_limit(const Context &ctx) {
     return (unsigned)std::min<size_t>(ctx.props.sharedMemPerBlock, 64 * 1024);
 }
 
-void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const float *values,
-               const LayerMaps &maps) {
+// The layer's maps (when given), the row checks and the chunked column copy of a CSR
+// whose rows are resident (indptr, indices, values at csr.nnz).
+static void csr_finish(Csr &csr, const LayerMaps &maps) {
     Context &ctx = *csr.ctx;
     const uint64_t n = csr.n_rows;
-    KY_CHECK(n >= 1, KY_E215, "csr: a layer without cells");
-    KY_CHECK(indptr[0] == 0, KY_E215, "csr: indptr does not start at 0");
-    for (uint64_t r = 0; r < n; r++)
-        KY_CHECK(indptr[r + 1] >= indptr[r], KY_E215, "csr: indptr decreases at row " + std::to_string(r));
-    csr.nnz = indptr[n];
-    KY_CHECK(csr.nnz < (1ull << 32), KY_E505,
-             "csr: a layer holds 2^32 or more values on one device; pass a smaller memory_limit");
-    KY_CHECK(csr.n_cols >= 1, KY_E215, "csr: a layer without genes");
-    csr.indptr.alloc(n + 1);
-    csr.indices.alloc(std::max<uint64_t>(csr.nnz, 1));
-    csr.values.alloc(std::max<uint64_t>(csr.nnz, 1));
     hipStream_t st = ctx.stream;
-    KY_HIP(hipMemcpyAsync(csr.indptr.ptr, indptr, sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, st));
-    if (csr.nnz) {
-        KY_HIP(hipMemcpyAsync(csr.indices.ptr, indices, sizeof(uint32_t) * csr.nnz, hipMemcpyHostToDevice, st));
-        KY_HIP(hipMemcpyAsync(csr.values.ptr, values, sizeof(float) * csr.nnz, hipMemcpyHostToDevice, st));
-    }
     // The input layer's maps on the uploaded values (normalize_total, log1p).
     normalize_log1p(st, csr.indptr.ptr, n, csr.values.ptr, maps.target_sum, maps.normalize, maps.log1p);
     DevBuf<unsigned> flags(1);
@@ -445,6 +430,164 @@ void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const 
                                                          csr.col_row.ptr, csr.col_val.ptr);
     KY_HIP(hipGetLastError());
     KY_HIP(hipStreamSynchronize(st));
+}
+
+void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const float *values,
+               const LayerMaps &maps) {
+    Context &ctx = *csr.ctx;
+    const uint64_t n = csr.n_rows;
+    KY_CHECK(n >= 1, KY_E215, "csr: a layer without cells");
+    KY_CHECK(indptr[0] == 0, KY_E215, "csr: indptr does not start at 0");
+    for (uint64_t r = 0; r < n; r++)
+        KY_CHECK(indptr[r + 1] >= indptr[r], KY_E215, "csr: indptr decreases at row " + std::to_string(r));
+    csr.nnz = indptr[n];
+    KY_CHECK(csr.nnz < (1ull << 32), KY_E505,
+             "csr: a layer holds 2^32 or more values on one device; pass a smaller memory_limit");
+    KY_CHECK(csr.n_cols >= 1, KY_E215, "csr: a layer without genes");
+    csr.indptr.alloc(n + 1);
+    csr.indices.alloc(std::max<uint64_t>(csr.nnz, 1));
+    csr.values.alloc(std::max<uint64_t>(csr.nnz, 1));
+    hipStream_t st = ctx.stream;
+    KY_HIP(hipMemcpyAsync(csr.indptr.ptr, indptr, sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, st));
+    if (csr.nnz) {
+        KY_HIP(hipMemcpyAsync(csr.indices.ptr, indices, sizeof(uint32_t) * csr.nnz, hipMemcpyHostToDevice, st));
+        KY_HIP(hipMemcpyAsync(csr.values.ptr, values, sizeof(float) * csr.nnz, hipMemcpyHostToDevice, st));
+    }
+    csr_finish(csr, maps);
+}
+
+// A pinned host buffer (hipHostMalloc), grown on demand.
+template <typename T>
+struct Pinned {
+    T *ptr = nullptr;
+    size_t n = 0;
+    void reserve(size_t count) {
+        if (count <= n) return;
+        if (ptr) KY_HIP(hipHostFree(ptr));
+        ptr = nullptr;
+        n = 0;
+        KY_HIP(hipHostMalloc(reinterpret_cast<void **>(&ptr), sizeof(T) * std::max<size_t>(count, 1),
+                             hipHostMallocDefault));
+        n = count;
+    }
+    ~Pinned() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+};
+
+// A device buffer grown to `count` keeping its first `keep` elements (the caller has
+// synchronised every stream that uses it).
+template <typename T>
+static void grow_keep(DevBuf<T> &buf, size_t count, size_t keep, hipStream_t st) {
+    if (count <= buf.n) return;
+    DevBuf<T> bigger(count);
+    if (keep) KY_HIP(hipMemcpyAsync(bigger.ptr, buf.ptr, sizeof(T) * keep, hipMemcpyDeviceToDevice, st));
+    KY_HIP(hipStreamSynchronize(st));
+    buf = std::move(bigger);
+}
+
+// The layer read block by block (a store's row blocks): the reader fills block b into
+// one of two pinned buffer sets while the other set's copy to the device (its own
+// stream) and the previous block's maps (the context's stream) run; the blocks' rows
+// are appended to the resident CSR. The host never holds more than two blocks.
+void csr_stream(Csr &csr, uint64_t block_rows, ky_block_reader reader, void *user, uint64_t nnz_hint,
+                const LayerMaps &maps, ky_stream_info *info) {
+    Context &ctx = *csr.ctx;
+    const uint64_t n = csr.n_rows;
+    KY_CHECK(n >= 1, KY_E215, "csr: a layer without cells");
+    KY_CHECK(csr.n_cols >= 1, KY_E215, "csr: a layer without genes");
+    KY_CHECK(block_rows >= 1, KY_E215, "csr: block_rows must be at least 1");
+    KY_CHECK(reader != nullptr, KY_E215, "csr: reader is null");
+    const double t_start = now_ms();
+    double read_ms = 0.0;
+    hipStream_t st = ctx.stream, cp = nullptr;
+    KY_HIP(hipStreamCreateWithFlags(&cp, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~StreamGuard() {
+            if (s) {
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamDestroy(s);
+            }
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } guard{cp};
+    KY_HIP(hipEventCreateWithFlags(&guard.e[0], hipEventDisableTiming));
+    KY_HIP(hipEventCreateWithFlags(&guard.e[1], hipEventDisableTiming));
+    Pinned<uint64_t> pin_ptr[2];
+    Pinned<uint32_t> pin_idx[2];
+    Pinned<float> pin_val[2];
+    const uint64_t first_cap = std::max<uint64_t>(1024, std::min<uint64_t>(block_rows * csr.n_cols / 8, 1ull << 26));
+    for (int k = 0; k < 2; k++) {
+        pin_ptr[k].reserve(block_rows + 1);
+        pin_idx[k].reserve(first_cap);
+        pin_val[k].reserve(first_cap);
+    }
+    csr.indptr.alloc(n + 1);
+    uint64_t total = 0, blocks = 0;
+    bool used[2] = {false, false};
+    for (uint64_t first = 0; first < n; first += block_rows, blocks++) {
+        const int k = (int)(blocks % 2);
+        const uint64_t cnt = std::min<uint64_t>(block_rows, n - first);
+        if (used[k]) KY_HIP(hipEventSynchronize(guard.e[k]));  // the set's previous copy is done
+        const double t0 = now_ms();
+        int64_t got = reader(user, first, cnt, pin_ptr[k].ptr, pin_idx[k].ptr, pin_val[k].ptr, pin_idx[k].n);
+        KY_CHECK(got >= 0, KY_E505, "csr: the block reader failed at row " + std::to_string(first));
+        if ((uint64_t)got > pin_idx[k].n) {
+            pin_idx[k].reserve((uint64_t)got);
+            pin_val[k].reserve((uint64_t)got);
+            got = reader(user, first, cnt, pin_ptr[k].ptr, pin_idx[k].ptr, pin_val[k].ptr, pin_idx[k].n);
+            KY_CHECK(got >= 0 && (uint64_t)got <= pin_idx[k].n, KY_E505,
+                     "csr: the block reader failed at row " + std::to_string(first));
+        }
+        read_ms += now_ms() - t0;
+        const uint64_t nnz = (uint64_t)got;
+        uint64_t *ip = pin_ptr[k].ptr;
+        KY_CHECK(ip[0] == 0 && ip[cnt] == nnz, KY_E215,
+                 "csr: a block's indptr does not run from 0 to its number of values (rows " + std::to_string(first) +
+                     ")");
+        for (uint64_t r = 0; r < cnt; r++)
+            KY_CHECK(ip[r + 1] >= ip[r], KY_E215, "csr: indptr decreases at row " + std::to_string(first + r));
+        for (uint64_t r = 0; r <= cnt; r++) ip[r] += total;  // global offsets
+        KY_CHECK(total + nnz < (1ull << 32), KY_E505,
+                 "csr: a layer holds 2^32 or more values on one device; pass a smaller memory_limit");
+        if (total + nnz > csr.values.n) {
+            // Grow the resident arrays (first block: from the hint or an estimate).
+            KY_HIP(hipStreamSynchronize(cp));
+            KY_HIP(hipStreamSynchronize(st));
+            uint64_t want = nnz_hint ? nnz_hint : (uint64_t)((double)(total + nnz) / (double)(first + cnt) * n * 1.05);
+            want = std::max<uint64_t>(want, (total + nnz) + (total + nnz) / 4);
+            grow_keep(csr.indices, want, total, st);
+            grow_keep(csr.values, want, total, st);
+        }
+        // Copy the block (its indptr including the end offset) on the copy stream,
+        // then the maps of its rows on the context's stream once it has arrived.
+        KY_HIP(hipMemcpyAsync(csr.indptr.ptr + first, ip, sizeof(uint64_t) * (cnt + 1), hipMemcpyHostToDevice, cp));
+        if (nnz) {
+            KY_HIP(hipMemcpyAsync(csr.indices.ptr + total, pin_idx[k].ptr, sizeof(uint32_t) * nnz,
+                                  hipMemcpyHostToDevice, cp));
+            KY_HIP(hipMemcpyAsync(csr.values.ptr + total, pin_val[k].ptr, sizeof(float) * nnz,
+                                  hipMemcpyHostToDevice, cp));
+        }
+        KY_HIP(hipEventRecord(guard.e[k], cp));
+        used[k] = true;
+        KY_HIP(hipStreamWaitEvent(st, guard.e[k], 0));
+        normalize_log1p(st, csr.indptr.ptr + first, cnt, csr.values.ptr, maps.target_sum, maps.normalize, maps.log1p);
+        total += nnz;
+    }
+    KY_HIP(hipStreamSynchronize(cp));
+    KY_HIP(hipStreamSynchronize(st));
+    csr.nnz = total;
+    csr_finish(csr, LayerMaps());
+    if (info) {
+        info->blocks = blocks;
+        info->nnz = total;
+        info->read_ms = read_ms;
+        info->total_ms = now_ms() - t_start;
+        info->bytes = (double)total * 8.0 + (double)(n + blocks) * 8.0;
+    }
 }
 
 // The sum of n_parts arrays of `stride` values (part c at partial + c * stride) into

@@ -20,6 +20,9 @@ struct LayerMaps {
 };
 void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const float *values,
                const LayerMaps &maps = LayerMaps());
+// The layer read block by block through pinned double buffers (pca.hip).
+void csr_stream(Csr &csr, uint64_t block_rows, ky_block_reader reader, void *user, uint64_t nnz_hint,
+                const LayerMaps &maps, ky_stream_info *info);
 void pca_sketch(Csr &csr, uint64_t seed, uint64_t cell_offset, uint32_t l, double *sketch, double *gene_sums,
                 double *omega_sums);
 void pca_product(Csr &csr, const double *q, uint32_t w, const double *block_means, double *product, double *sums);

@@ -162,6 +162,55 @@ void ks_fill(const ks_spec *s, const uint64_t *indptr, uint32_t *indices, float 
     free(fold);
 }
 
+/* One block of rows [first, first + count) as a CSR of its own: indptr (count + 1,
+ * from 0), then, when its nnz fits `capacity`, the indices and values. Returns the
+ * block's nnz (a block larger than capacity is counted but not filled). The rows are
+ * those of the whole matrix: draws depend on (seed, cell, gene) only. */
+uint64_t ks_block(const ks_spec *s, uint64_t first, uint64_t count, uint64_t *indptr, uint32_t *indices,
+                  float *values, uint64_t capacity) {
+    double *mu = (double *)malloc(sizeof(double) * s->n_genes);
+    double *fold = (double *)malloc(sizeof(double) * s->n_genes * s->n_types);
+    gene_tables(s, mu, fold);
+    uint64_t *row_nnz = (uint64_t *)malloc(sizeof(uint64_t) * (count + 1));
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (uint64_t i = 0; i < count; i++) {
+        const uint64_t c = first + i;
+        double size = cell_size(s, c);
+        const double *f = fold + (uint64_t)cell_type(s, c) * s->n_genes;
+        uint64_t cnt = 0;
+        for (uint32_t g = 0; g < s->n_genes; g++) {
+            double u = uni(s->seed, S_COUNT, c * (uint64_t)s->n_genes + g);
+            if (nb_count(mu[g] * size * f[g], s->phi, u)) cnt++;
+        }
+        row_nnz[i] = cnt;
+    }
+    indptr[0] = 0;
+    for (uint64_t i = 0; i < count; i++) indptr[i + 1] = indptr[i] + row_nnz[i];
+    const uint64_t nnz = indptr[count];
+    if (nnz <= capacity) {
+        #pragma omp parallel for schedule(dynamic, 64)
+        for (uint64_t i = 0; i < count; i++) {
+            const uint64_t c = first + i;
+            double size = cell_size(s, c);
+            const double *f = fold + (uint64_t)cell_type(s, c) * s->n_genes;
+            uint64_t at = indptr[i];
+            for (uint32_t g = 0; g < s->n_genes; g++) {
+                double u = uni(s->seed, S_COUNT, c * (uint64_t)s->n_genes + g);
+                uint32_t k = nb_count(mu[g] * size * f[g], s->phi, u);
+                if (k) {
+                    indices[at] = g;
+                    values[at] = (float)k;
+                    at++;
+                }
+            }
+        }
+    }
+    free(row_nnz);
+    free(mu);
+    free(fold);
+    return nnz;
+}
+
 /* Cell types (for tests that look at clusters). */
 void ks_cell_types(const ks_spec *s, uint32_t *types) {
     for (uint64_t c = 0; c < s->n_cells; c++) types[c] = cell_type(s, c);

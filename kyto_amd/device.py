@@ -94,6 +94,27 @@ class Device:
     def upload_csr(self, indptr, indices, values, n_genes: int) -> "CsrLayer":
         return CsrLayer(self, indptr, indices, values, n_genes)
 
+    def stream_counts(self, n_rows: int, n_genes: int, read_block, block_rows: int = 32_768, nnz_hint: int = 0,
+                      target_sum=1e4, log1p=True) -> "CsrLayer":
+        """A counts layer read block by block through pinned double buffers
+        (ky_csr_stream), made into the layer on the device as upload_counts does.
+        read_block(first, n_rows, indptr, indices, values, capacity) fills the
+        library's pinned buffers (ctypes pointers) and returns the block's nnz."""
+        def trampoline(user, first, count, indptr, indices, values, capacity):
+            try:
+                return int(read_block(int(first), int(count), indptr, indices, values, int(capacity)))
+            except Exception:  # noqa: BLE001 - a failure is reported to the library as -1
+                return -1
+
+        cb = N.BlockReader(trampoline)
+        h = c_void_p()
+        info = N.StreamInfo()
+        N.check(self._lib.ky_csr_stream(self._h, int(n_rows), int(n_genes), int(block_rows), cb, None, int(nnz_hint),
+                                        float(target_sum or 0.0), 1 if log1p else 0, byref(h), byref(info)))
+        layer = CsrLayer._adopt(self, h, int(n_rows), int(n_genes), int(info.nnz))
+        layer.stream_info = _info(info)
+        return layer
+
     def upload_counts(self, indptr, indices, counts, n_genes: int, target_sum=1e4, log1p=True) -> "CsrLayer":
         """A counts CSR made into the layer on the device: kyto's normalize_total
         (target_sum; None skips it) then log1p (ky_csr_upload_counts)."""
@@ -226,6 +247,12 @@ class CsrLayer:
                                                   N.ptr(values, N.f32p), self.n_rows, self.n_genes,
                                                   float(target_sum or 0.0), 1 if log1p else 0, byref(h)))
         self._h = h
+
+    @classmethod
+    def _adopt(cls, dev, h, n_rows, n_genes, nnz) -> "CsrLayer":
+        self = cls.__new__(cls)
+        self.dev, self._h, self.n_rows, self.n_genes, self.nnz = dev, h, n_rows, n_genes, nnz
+        return self
 
     @property
     def handle(self):

@@ -110,3 +110,18 @@ def log1p_layer(n_cells: int, n_genes: int, n_types: int, density: float, seed: 
 def workload(name: str, with_types: bool = False) -> Layer:
     w = WORKLOADS[name]
     return log1p_layer(w.n_cells, w.n_genes, w.n_types, w.density, w.seed, with_types)
+
+
+def block_reader(n_cells: int, n_genes: int, n_types: int, density: float, seed: int = 0):
+    """A reader of the synthetic counts matrix by row blocks (for Device.stream_counts):
+    the same matrix as counts(), generated block by block into the caller's buffers."""
+    import ctypes
+
+    lib = N.synth_lib()
+    spec = _spec(n_cells, n_genes, n_types, seed, calibrate(n_genes, n_types, density, seed))
+
+    def read(first, count, indptr, indices, values, capacity):
+        return lib.ks_block(ctypes_ref(spec), first, count, ctypes.cast(indptr, ctypes.c_void_p),
+                            ctypes.cast(indices, ctypes.c_void_p), ctypes.cast(values, ctypes.c_void_p), capacity)
+
+    return read

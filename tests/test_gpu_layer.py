@@ -55,3 +55,35 @@ def test_plan_normalize_log1p_pca_equals_host_layer():
         .on("gpu").collect()
     assert np.array_equal(a["X_pca"], b["X_pca"])
     assert np.array_equal(a["X_pca_variance"], b["X_pca_variance"])
+
+
+def test_streamed_blocks_equal_one_upload(device):
+    # 70,000 cells in blocks of 32,768 (two full, one partial); dense enough that the
+    # first pinned capacity is too small, and a tiny nnz hint so the resident arrays
+    # grow while blocks arrive.
+    n, g = 70_000, 300
+    c = S.counts(n, g, 5, 0.2, seed=6)
+    read = S.block_reader(n, g, 5, 0.2, seed=6)
+    streamed = device.stream_counts(n, g, read, block_rows=32_768, nnz_hint=10, target_sum=1e4, log1p=True)
+    whole = device.upload_counts(c.indptr, c.indices, c.values, g, target_sum=1e4, log1p=True)
+    try:
+        assert streamed.stream_info["blocks"] == 3 and streamed.nnz == c.nnz
+        assert np.array_equal(_bits(streamed.values()), _bits(whole.values()))
+        a = device.pca(streamed, 20, seed=0, power_iterations=4)
+        b = device.pca(whole, 20, seed=0, power_iterations=4)
+        assert np.array_equal(a.embedding, b.embedding)
+        a.device_embedding.free()
+        b.device_embedding.free()
+    finally:
+        streamed.free()
+        whole.free()
+
+
+def test_stream_reader_failure_is_e505(device):
+    from kyto_amd import _native as N
+
+    def bad(first, count, indptr, indices, values, capacity):
+        raise RuntimeError("store unreadable")
+
+    with pytest.raises(N.KytoError, match="KY-E505|505"):
+        device.stream_counts(1000, 10, bad, block_rows=256)
