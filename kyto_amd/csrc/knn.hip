@@ -538,9 +538,9 @@ void knn_exact_rows(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint
     DevBuf<double> keys_tmp(count * k);
     const unsigned blocks = div_up(count, QB);
     // Segments of the points: enough blocks to fill the device (about 2048), each
-    // segment at least 16 tiles, n_seg * k within the merge's 1024 entries.
+    // segment at least 16 tiles, n_seg * k within the merge's 4096 entries.
     uint32_t n_seg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(2048, blocks), n / (16 * TP)));
-    n_seg = std::min<uint32_t>(n_seg, 1024 / k);
+    n_seg = std::min<uint32_t>(n_seg, 4096 / k);
     const double *sk = seeded ? d_keys : nullptr;
     const uint32_t *sr = seeded ? d_rows : nullptr;
     if (n_seg <= 1) {
@@ -551,8 +551,12 @@ void knn_exact_rows(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint
         DevBuf<double> seg_keys(count * n_seg * k);
         launch_scan<true, 128>(padded_width(d), st, blocks, pts, nullptr, n, d, d_list, 0, count, k, seg_rows.ptr,
                                seg_keys.ptr, seg_cnt.ptr, n_seg, sk, sr);
-        merge_segments<1024><<<(unsigned)count, 64, 0, st>>>(seg_rows.ptr, seg_keys.ptr, seg_cnt.ptr, count, n_seg,
-                                                              k, rows_tmp.ptr, keys_tmp.ptr, cnt_tmp.ptr);
+        if (n_seg * k <= 1024)
+            merge_segments<1024><<<(unsigned)count, 64, 0, st>>>(seg_rows.ptr, seg_keys.ptr, seg_cnt.ptr, count,
+                                                                  n_seg, k, rows_tmp.ptr, keys_tmp.ptr, cnt_tmp.ptr);
+        else
+            merge_segments<4096><<<(unsigned)count, 64, 0, st>>>(seg_rows.ptr, seg_keys.ptr, seg_cnt.ptr, count,
+                                                                  n_seg, k, rows_tmp.ptr, keys_tmp.ptr, cnt_tmp.ptr);
         KY_HIP(hipGetLastError());
     }
     scatter_exact<<<div_up(count * k, 256), 256, 0, st>>>(d_list, count, k, rows_tmp.ptr, keys_tmp.ptr, d_keys, d_rows);
