@@ -96,6 +96,28 @@ __device__ __forceinline__ void wave_bitonic(K *keys, uint32_t *rows, unsigned N
     }
 }
 
+// Ascending sort of 64 (key, row) pairs held one per lane, in registers: lane l ends
+// with the l-th smallest pair, ties broken by row. The bitonic network of the LDS sort
+// above, each compare-exchange done by a lane and its partner lane.
+__device__ __forceinline__ void wave_sort64(unsigned long long &key, uint32_t &row) {
+    const unsigned lane = lane_id();
+#pragma unroll
+    for (unsigned size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+        for (unsigned stride = size >> 1; stride > 0; stride >>= 1) {
+            const unsigned long long ok = __shfl_xor(key, stride);
+            const uint32_t orow = __shfl_xor(row, stride);
+            const bool other_less = ok < key || (ok == key && orow < row);
+            // The lower position of an ascending pair keeps the smaller, and so on.
+            const bool keep_small = ((lane & stride) == 0) == ((lane & size) == 0);
+            if (other_less == keep_small) {
+                key = ok;
+                row = orow;
+            }
+        }
+    }
+}
+
 // Compact query slot `ql`'s buffer (count entries) to its `keep` smallest (key, row)
 // pairs, sorted ascending; when it held at least `keep`, the keep-th pair becomes the
 // threshold. Called by the whole wavefront.

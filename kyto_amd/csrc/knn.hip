@@ -150,7 +150,9 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
                                                    uint32_t *__restrict__ out_count) {
     using K = typename std::conditional<EXACT, unsigned long long, uint32_t>::type;
     constexpr int QS = DP + 4;
-    __shared__ __attribute__((aligned(16))) float sq[QB * QS];
+    __shared__ __attribute__((aligned(16))) float sq[EXACT ? 1 : QB * QS];
+    // EXACT: the queries' rows widened to f64 (DP a row, zero past d).
+    __shared__ __attribute__((aligned(16))) double sqd[EXACT ? QB * DP : 1];
     __shared__ float sqo[QB];
     __shared__ K bkey[QB * CAP];
     __shared__ uint32_t brow[QB * CAP];
@@ -190,7 +192,8 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
         uint32_t qg = qrow[r];
         float v = 0.0f;
         if (qg != 0xFFFFFFFFu && c < stride) v = a[(uint64_t)qg * stride + c];
-        sq[r * QS + c] = v;
+        if constexpr (EXACT) sqd[r * DP + c] = (double)v;
+        else sq[r * QS + c] = v;
     }
     __syncthreads();
     unsigned nq = 0;  // this wave's live queries
@@ -199,21 +202,27 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
     for (uint64_t t0 = p_begin; t0 < p_end; t0 += TP) {
         const uint64_t p = t0 + lane;
         const bool valid = p < p_end;
-        float pr[DP];
-        if (valid) {
-            if constexpr (!EXACT) {
-                const float4 *src = reinterpret_cast<const float4 *>(a + p * DP);
+        float pr[EXACT ? 1 : DP];
+        double pd[EXACT ? DP : 1];  // EXACT: the point widened to f64, zero past d
+        if constexpr (EXACT) {
+            // Unconditional loads (a lane past the segment reads its first point,
+            // components past d read component d - 1, then zeroed), so no load waits
+            // on a branch.
+            const uint64_t pp = valid ? p : p_begin;
 #pragma unroll
-                for (int c = 0; c < DP / 4; c++) {
-                    float4 v = src[c];
-                    pr[4 * c] = v.x;
-                    pr[4 * c + 1] = v.y;
-                    pr[4 * c + 2] = v.z;
-                    pr[4 * c + 3] = v.w;
-                }
-            } else {
+            for (int c = 0; c < DP; c++) {
+                const float v = a[pp * d + min((unsigned)c, d - 1)];
+                pd[c] = c < (int)d ? (double)v : 0.0;
+            }
+        } else if (valid) {
+            const float4 *src = reinterpret_cast<const float4 *>(a + p * DP);
 #pragma unroll
-                for (int c = 0; c < DP; c++) pr[c] = c < (int)d ? a[p * d + c] : 0.0f;
+            for (int c = 0; c < DP / 4; c++) {
+                float4 v = src[c];
+                pr[4 * c] = v.x;
+                pr[4 * c + 1] = v.y;
+                pr[4 * c + 2] = v.z;
+                pr[4 * c + 3] = v.w;
             }
         } else {
 #pragma unroll
@@ -242,12 +251,20 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
                 key = order_f32(kf);
             } else {
                 double l[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                // The query's row: every LDS read first, then the arithmetic.
+                double qv[DP];
+                const double2 *qd = reinterpret_cast<const double2 *>(sqd + ql * DP);
+#pragma unroll
+                for (int c = 0; c < DP / 2; c++) {
+                    const double2 v = qd[c];
+                    qv[2 * c] = v.x;
+                    qv[2 * c + 1] = v.y;
+                }
 #pragma unroll
                 for (int c = 0; c < DP; c++) {
-                    if (c < (int)d) {
-                        double t = (double)sq[ql * QS + c] - (double)pr[c];
-                        l[c & 7] += t * t;
-                    }
+                    // Past d both sides are 0, and adding t * t = +0.0 leaves the sum as it is.
+                    const double t = qv[c] - pd[c];
+                    l[c & 7] += t * t;
                 }
                 double h0 = l[0] + l[4], h1 = l[1] + l[5], h2 = l[2] + l[6], h3 = l[3] + l[7];
                 double kd = (h0 + h2) + (h1 + h3);
@@ -343,8 +360,6 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
                                               double bf_err_max, double bf_acc_rel, int floor_mode,
                                               double *__restrict__ out_keys, uint32_t *__restrict__ out_rows,
                                               uint8_t *__restrict__ certified) {
-    __shared__ unsigned long long skey[4][64];
-    __shared__ uint32_t srow[4][64];
     const unsigned wave = threadIdx.x / 64, lane = lane_id();
     const uint64_t i = (uint64_t)blockIdx.x * 4 + wave;
     if (i >= q_count) return;
@@ -352,32 +367,62 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
     const unsigned cnt = cand_count[i];
     const unsigned t = lane & 7, g = lane >> 3;  // kyto lane t, candidate group g
     const float *qrow = pts + q * d;
-    for (unsigned c0 = 0; c0 < 64; c0 += 8) {
-        const unsigned c = c0 + g;
-        uint32_t row = 0xFFFFFFFFu;
-        double part = 0.0;
-        if (c < cnt) {
-            row = cand_rows[i * m + c];
-            const float *prow = pts + (uint64_t)row * d;
-            for (unsigned j = t; j < d; j += 8) {
-                double diff = (double)qrow[j] - (double)prow[j];
-                part += diff * diff;
+    // Lane t sums components t, t + 8, ... of each of its group's 8 candidates
+    // (8r + g, r < 8); each 64-component chunk issues all its loads first. Loads are
+    // unconditional (a missing candidate reads the query's row, components past d
+    // read component d - 1) so that none waits on a branch; those values are unused.
+    uint32_t rowv[8], lrow[8];
+    double part[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const unsigned c = 8 * r + g;
+        const uint32_t cr = cand_rows[i * m + (c < cnt ? c : 0)];
+        rowv[r] = c < cnt ? cr : 0xFFFFFFFFu;
+        lrow[r] = c < cnt ? cr : (uint32_t)q;
+        part[r] = 0.0;
+    }
+    for (unsigned j0 = 0; j0 < d; j0 += 64) {
+        float qv[8], pv[8][8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) qv[u] = qrow[min(j0 + t + 8 * u, d - 1)];
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+            for (int u = 0; u < 8; u++) pv[r][u] = pts[(uint64_t)lrow[r] * d + min(j0 + t + 8 * u, d - 1)];
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                // Past d the difference is 0 and adds +0.0 (no change to the sum).
+                const float p = j0 + t + 8 * u < d ? pv[r][u] : qv[u];
+                const double diff = (double)qv[u] - (double)p;
+                part[r] += diff * diff;
             }
-        }
+    }
+    // Lane 8g + t keeps the key of candidate 8t + g; every lane of a group ends the
+    // reduction with the same sum (the additions commute), so it can pick its own.
+    unsigned long long mine = ~0ull;
+    uint32_t mine_row = 0xFFFFFFFFu;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const unsigned c = 8 * r + g;
         // ((l0 + l4) + (l2 + l6)) + ((l1 + l5) + (l3 + l7)) over the group's lanes.
-        double h = part + __shfl_xor(part, 4, 8);       // t < 4: l_t + l_{t+4}
-        double qv = h + __shfl_xor(h, 2, 8);            // t in {0,1}: h_t + h_{t+2}
-        double key = qv + __shfl_xor(qv, 1, 8);         // t = 0: q0 + q1
-        if (t == 0) {
-            skey[wave][c] = c < cnt ? (unsigned long long)__double_as_longlong(key) : ~0ull;
-            srow[wave][c] = row;
+        const double h = part[r] + __shfl_xor(part[r], 4, 8);
+        const double hq = h + __shfl_xor(h, 2, 8);
+        const double key = hq + __shfl_xor(hq, 1, 8);
+        if (t == (unsigned)r) {
+            mine = c < cnt ? (unsigned long long)__double_as_longlong(key) : ~0ull;
+            mine_row = rowv[r];
         }
     }
-    wave_sync();
-    wave_bitonic<unsigned long long>(skey[wave], srow[wave], 64);
+    // Lane c takes candidate c (= 8r + g) from lane 8g + r.
+    const int src = 8 * (lane & 7) + (lane >> 3);
+    unsigned long long skey = __shfl(mine, src);
+    uint32_t srow = __shfl(mine_row, src);
+    wave_sort64(skey, srow);
     bool cert = false;
     if (cnt >= k) {
-        const double kth = __longlong_as_double((long long)skey[wave][k - 1]);
+        const double kth = __longlong_as_double((long long)__shfl(skey, (int)k - 1));
         if ((uint64_t)cnt >= n - 1) {
             cert = true;  // every other row is a candidate
         } else {
@@ -399,8 +444,8 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
         }
     }
     if (lane < k) {
-        out_keys[q * k + lane] = __longlong_as_double((long long)skey[wave][lane]);
-        out_rows[q * k + lane] = srow[wave][lane];
+        out_keys[q * k + lane] = __longlong_as_double((long long)skey);
+        out_rows[q * k + lane] = srow;
     }
     if (lane == 0) certified[q] = cert ? 1 : 0;
 }
