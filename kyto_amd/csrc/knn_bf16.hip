@@ -311,8 +311,9 @@ __device__ unsigned compact_buffer(Entry *__restrict__ buf, unsigned cnt, unsign
 constexpr int LEAD_TILES = 16;        // tiles scanned before a block's own position
 constexpr uint64_t EST_MIN_ROWS = 1 << 17;  // threshold estimates from a sample above this size
 // The sample: every 12th row of the scan order. With 32 subsets of n / 384 sampled
-// rows and the 9th smallest minimum, about -ln(1 - 9/32) * 384 = 127 rows (twice the
-// 64 candidates) fall at or below a query's estimate on average.
+// rows and the p-th smallest minimum as the estimate, about -ln(1 - p/32) * 384 rows
+// fall at or below it on average; p is the smallest giving 8 (k + 1) (p = 9 and 127
+// rows for k = 14, p = 15 and 242 rows for k = 29).
 constexpr uint32_t EST_STRIDE = 12;
 
 constexpr unsigned SCAP = 64;  // staged appends per wavefront
@@ -614,19 +615,19 @@ constexpr int EST_SPB = 16 * EST_MB;          // points per super-block
 constexpr int EST_QPW = 16 * EST_NB;          // queries per wavefront
 constexpr int EST_QPB = 4 * EST_QPW;          // queries per block
 constexpr int EST_SUBSETS = 4 * 4 * EST_MB;   // per query: 4 lanes x 4 EST_MB keys of a super-block
-constexpr int EST_PICK = 9;                   // the 9th smallest subset minimum
 
 // A starting threshold for each query: the points (every pstride-th row of the scan
 // order) fall into EST_SUBSETS disjoint subsets by their position within a
 // super-block (the lane and register that hold the key), each lane keeps the running
-// minimum of its keys per subset in registers, and the EST_PICK-th smallest of a
-// query's subset minima is its threshold: EST_PICK subsets hold a point at or below
-// it, so at least EST_PICK - 1 rows other than the query itself do. No appends and no
+// minimum of its keys per subset in registers, and the pick-th smallest of a query's
+// subset minima is its threshold: pick subsets hold a point at or below it, so at
+// least pick - 1 rows other than the query itself do. No appends and no
 // compactions: the scan is the MFMA keys and one v_min a key. A starting threshold
 // only steers how much the candidate scan keeps; the candidates do not depend on it.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void estimate_bf16(
     const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q, uint64_t np, uint32_t pstride,
-    const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq, float *__restrict__ out_thr) {
+    const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq, uint32_t pick,
+    float *__restrict__ out_thr) {
     __shared__ __attribute__((aligned(16))) unsigned char tile[3][TILE * ROW_BYTES];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
     const unsigned nblk = gridDim.x, xcd = blockIdx.x % 8, per = nblk / 8, extra = nblk % 8;
@@ -726,14 +727,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 #pragma unroll
         for (int e = 0; e < 4 * EST_MB; e++) mins[((lane / 16) * 4 * EST_MB + e) * (EST_QPW + 1) + 16 * b + (lane % 16)] = rmin[b][e];
     wave_sync();
-    // Lane q: the EST_PICK-th smallest of its query's minima (remove the smallest
-    // EST_PICK - 1 times).
+    // Lane q: the pick-th smallest of its query's minima (remove the smallest pick - 1
+    // times).
     float v[EST_SUBSETS];
 #pragma unroll
     for (int s = 0; s < EST_SUBSETS; s++) v[s] = mins[s * (EST_QPW + 1) + lane];
     float t = INFINITY;
-#pragma unroll
-    for (int pick = 0; pick < EST_PICK; pick++) {
+    for (uint32_t r = 0; r < pick; r++) {
         t = INFINITY;
         int at = 0;
 #pragma unroll
@@ -825,7 +825,12 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
     if (n >= EST_MIN_ROWS) {
         thr_est.alloc(nq);
         const uint64_t np = div_up(n, EST_STRIDE);
-        estimate_bf16<<<div_up(nq, EST_QPB), 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, thr_est.ptr);
+        uint32_t pick = 1;
+        while (pick < EST_SUBSETS - 1 &&
+               -std::log1p(-(double)pick / EST_SUBSETS) * EST_SUBSETS * EST_STRIDE < 8.0 * (k + 1))
+            pick++;
+        estimate_bf16<<<div_up(nq, EST_QPB), 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, pick,
+                                                            thr_est.ptr);
         KY_HIP(hipGetLastError());
     }
     hipEvent_t ev[2];
