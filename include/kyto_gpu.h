@@ -41,6 +41,7 @@ typedef struct ky_ctx ky_ctx;     /* one device, one stream */
 typedef struct ky_csr ky_csr;     /* a CSR layer resident in device memory */
 typedef struct ky_dense ky_dense; /* a row-major f32 matrix resident in device memory */
 typedef struct ky_ivf ky_ivf;     /* an IVF index (k-means lists + router) on the device */
+typedef struct ky_graph ky_graph; /* a weighted cell graph (CSR) resident in device memory */
 
 /* The calling thread's last error message ("" if none). Replaces the
  * KytoError::DeviceFailure payload (launch.rs:62-69). */
@@ -197,6 +198,36 @@ int ky_neighbors(ky_ctx *ctx, const ky_dense *points, uint32_t n_components,
  * count x m). */
 int ky_nearest_keys(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uint64_t first,
                     uint64_t count, uint32_t m, int precision, float *keys, uint32_t *rows);
+
+/* --------------------------------------------------------- connectivities ---- */
+
+typedef struct ky_conn_info {
+    uint64_t edges;          /* stored entries of the symmetric graph */
+    uint64_t zero_rho_rows;  /* rows whose distances are all 0 (sigma from the global floor) */
+    double upload_ms;        /* the neighbour graph to the device */
+    double scales_ms;        /* rho, sigma and the directed memberships */
+    double union_ms;         /* transpose and fuzzy union */
+    double total_ms;
+} ky_conn_info;
+
+/* scanpy's connectivities of a neighbour graph (UMAP's fuzzy simplicial set with
+ * set_op_mix_ratio 1 and local_connectivity 1): connectivities (connectivities.rs:76-123),
+ * smooth_knn_distance (:125-168), membership (:179-188), fuzzy_union (:190-256).
+ * Input as kyto's NeighborGraph (neighbors.rs:52-175), checked as from_csr checks it
+ * (:68-147, KY-E215): n_cells rows of n_neighbors - 1 entries (host arrays,
+ * n_cells x (n_neighbors - 1)), columns increasing in each row and below n_cells,
+ * distances finite and >= 0. Output: the symmetric weighted graph (columns increasing
+ * in each row, float32 weights in (0, 1], no stored zeros) resident on the device;
+ * each row's sigma and rho (float32, n_cells) when `sigma` / `rho` are not NULL. */
+int ky_connectivities(ky_ctx *ctx, uint64_t n_cells, uint32_t n_neighbors, const uint32_t *indices,
+                      const float *distances, ky_graph **out, float *sigma, float *rho,
+                      ky_conn_info *info);
+/* ConnectivityGraph's accessors (connectivities.rs:44-71): rows and stored edges, and
+ * a copy of the CSR into caller buffers (indptr n_cells + 1, indices and weights
+ * `edges` entries). */
+int ky_graph_shape(const ky_graph *g, uint64_t *n_cells, uint64_t *edges);
+int ky_graph_download(const ky_graph *g, uint64_t *indptr, uint32_t *indices, float *weights);
+int ky_graph_free(ky_graph *g);
 
 /* ------------------------------------------------------ learned IVF router ---- */
 /* No counterpart in the reference (its approximate search is an HNSW index,

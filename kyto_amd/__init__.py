@@ -8,7 +8,7 @@ kyto's plan API (crates/kyto-py/src/atlas.rs:822-966, 1622-1720).
              .neighbors(n_neighbors=15)         # atlas.rs:857-966
              .on(device="gpu")                  # atlas.rs:1622-1636
              .collect(memory_limit="32GB"))     # atlas.rs:1691-1720
-    out["X_pca"], out["distances"]
+    out["X_pca"], out["distances"], out["connectivities"]
 
 Chained methods only build a plan; collect() runs it on the device. There is no CPU
 path: on(device="cpu") is refused with KY-E501 and a missing native library raises.
@@ -22,9 +22,9 @@ from typing import Any
 import numpy as np
 
 from ._native import KytoError, KY_E215, KY_E501
-from .device import CsrLayer, DenseMatrix, Device, Graph, IvfIndex, PcaOutput, device_count
+from .device import ConnectivityGraph, CsrLayer, DenseMatrix, Device, Graph, IvfIndex, PcaOutput, device_count
 
-__all__ = ["scan", "Atlas", "Device", "KytoError", "Graph", "PcaOutput", "device_count", "CsrLayer", "DenseMatrix",
+__all__ = ["scan", "Atlas", "Device", "KytoError", "Graph", "ConnectivityGraph", "PcaOutput", "device_count", "CsrLayer", "DenseMatrix",
            "IvfIndex"]
 
 # crates/kyto-plan/src/physical_plan.rs:73
@@ -251,8 +251,9 @@ class Atlas:
                 lines.append(f'PCA "{s.layer}" -> "{s.into}" n_components={s.n_components} seed={s.seed} '
                              f"[device: {self.device}]")
             else:
-                lines.append(f'NEIGHBORS "{s.embedding}" n_neighbors={s.n_neighbors} metric={s.metric} '
-                             f"search={s.search} precision={s.precision} [device: {self.device}]")
+                dist, conn = _graph_names(s.into)
+                lines.append(f'NEIGHBORS "{s.embedding}" -> "{dist}", "{conn}" n_neighbors={s.n_neighbors} '
+                             f"metric={s.metric} search={s.search} precision={s.precision} [device: {self.device}]")
         lines.append(f"SCAN ({', '.join(self.layers)})")
         return "\n".join(lines)
 
@@ -290,8 +291,10 @@ class Atlas:
                 else:
                     emb = resident[step.embedding]
                     g = _run_neighbors(dev, emb, step)
-                    key = "distances" if step.into is None else f"{step.into}_distances"
+                    key, conn = _graph_names(step.into)
                     out[key] = g
+                    # The connectivities graph the step declares (atlas.rs:857-862).
+                    out[conn] = dev.connectivities(g)
                     if progress:
                         progress(("neighbors", key))
         finally:
@@ -299,6 +302,13 @@ class Atlas:
                 m.free()
             dev.close()
         return out
+
+
+def _graph_names(into):
+    """The distances and connectivities graphs a neighbors step writes (atlas.rs:857-862)."""
+    if into is None:
+        return "distances", "connectivities"
+    return f"{into}_distances", f"{into}_connectivities"
 
 
 def _fits(layer: _Layer, step: _Pca, limit):

@@ -60,6 +60,11 @@ BlockReader = ctypes.CFUNCTYPE(ctypes.c_int64, c_void_p, c_uint64, c_uint64, POI
                                POINTER(c_float), c_uint64)
 
 
+class ConnInfo(ctypes.Structure):
+    _fields_ = [("edges", c_uint64), ("zero_rho_rows", c_uint64), ("upload_ms", c_double), ("scales_ms", c_double),
+                ("union_ms", c_double), ("total_ms", c_double)]
+
+
 class IvfParams(ctypes.Structure):
     _fields_ = [("n_lists", c_uint32), ("kmeans_iters", c_uint32), ("seed", c_uint64),
                 ("hidden", c_uint32), ("train_queries", c_uint32), ("batch", c_uint32),
@@ -100,6 +105,11 @@ SIGNATURES = {
                       POINTER(c_void_p), POINTER(StreamInfo)],
     "ky_neighbors": [c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_int, u32p, f32p, POINTER(KnnInfo)],
     "ky_nearest_keys": [c_void_p, c_void_p, c_uint32, c_uint64, c_uint64, c_uint32, c_int, f32p, u32p],
+    "ky_connectivities": [c_void_p, c_uint64, c_uint32, u32p, f32p, POINTER(c_void_p), f32p, f32p,
+                          POINTER(ConnInfo)],
+    "ky_graph_shape": [c_void_p, u64p, u64p],
+    "ky_graph_download": [c_void_p, u64p, u32p, f32p],
+    "ky_graph_free": [c_void_p],
     "ky_ivf_build": [c_void_p, c_void_p, POINTER(IvfParams), POINTER(c_void_p), POINTER(IvfInfo)],
     "ky_ivf_free": [c_void_p],
     "ky_ivf_search": [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_float, u32p, f32p,

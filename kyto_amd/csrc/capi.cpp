@@ -4,10 +4,12 @@
 
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "common.hpp"
+#include "connectivities.hpp"
 #include "dense.hpp"
 #include "ivf.hpp"
 #include "knn.hpp"
@@ -357,6 +359,51 @@ int ky_neighbors(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uin
         need(distances, "distances");
         use_device(*ctx);
         neighbors_run(*ctx, *points, n_components, n_neighbors, metric, precision, indices, distances, info);
+    });
+}
+
+int ky_connectivities(ky_ctx *ctx, uint64_t n_cells, uint32_t n_neighbors, const uint32_t *indices,
+                      const float *distances, ky_graph **out, float *sigma, float *rho, ky_conn_info *info) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(indices, "indices");
+        need(distances, "distances");
+        need(out, "out");
+        use_device(*ctx);
+        auto g = std::make_unique<ky_graph>();
+        connectivities_run(*ctx, n_cells, n_neighbors, indices, distances, *g, sigma, rho, info);
+        *out = g.release();
+    });
+}
+
+int ky_graph_shape(const ky_graph *g, uint64_t *n_cells, uint64_t *edges) {
+    return guarded([&] {
+        need(g, "graph");
+        if (n_cells) *n_cells = g->n;
+        if (edges) *edges = g->nnz;
+    });
+}
+
+int ky_graph_download(const ky_graph *g, uint64_t *indptr, uint32_t *indices, float *weights) {
+    return guarded([&] {
+        need(g, "graph");
+        need(indptr, "indptr");
+        need(indices, "indices");
+        need(weights, "weights");
+        use_device(*g->ctx);
+        download(*g->ctx, indptr, g->indptr.ptr, sizeof(uint64_t) * (g->n + 1));
+        if (g->nnz) {
+            download(*g->ctx, indices, g->indices.ptr, sizeof(uint32_t) * g->nnz);
+            download(*g->ctx, weights, g->weights.ptr, sizeof(float) * g->nnz);
+        }
+    });
+}
+
+int ky_graph_free(ky_graph *g) {
+    return guarded([&] {
+        if (!g) return;
+        (void)hipSetDevice(g->ctx->device);
+        delete g;
     });
 }
 

@@ -126,6 +126,16 @@ struct Dense {
     DevBuf<float> data;
 };
 
+// A weighted cell graph in CSR: columns increasing in each row.
+struct Graph {
+    Context *ctx = nullptr;
+    uint64_t n = 0;
+    uint64_t nnz = 0;
+    DevBuf<uint64_t> indptr;  // n + 1
+    DevBuf<uint32_t> indices;  // nnz
+    DevBuf<float> weights;     // nnz
+};
+
 // Host wall clock in milliseconds.
 inline double now_ms() {
     using namespace std::chrono;
@@ -139,3 +149,4 @@ inline unsigned div_up(uint64_t a, uint64_t b) { return static_cast<unsigned>((a
 struct ky_ctx : ky::Context {};
 struct ky_csr : ky::Csr {};
 struct ky_dense : ky::Dense {};
+struct ky_graph : ky::Graph {};

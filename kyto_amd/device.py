@@ -52,6 +52,22 @@ class Graph:
         return self.indices.reshape(-1, self.k)
 
 
+@dataclass
+class ConnectivityGraph:
+    """kyto's ConnectivityGraph (connectivities.rs:33-72): symmetric CSR, columns
+    increasing in each row, float32 weights in (0, 1], no stored zeros."""
+    indptr: np.ndarray   # u64, n + 1
+    indices: np.ndarray  # u32
+    weights: np.ndarray  # f32
+    sigma: np.ndarray    # f32, each row's sigma
+    rho: np.ndarray      # f32, each row's rho
+    info: dict = field(default_factory=dict)
+
+    @property
+    def n_cells(self) -> int:
+        return len(self.indptr) - 1
+
+
 class Device:
     """One GPU context (ky_open)."""
 
@@ -200,6 +216,32 @@ class Device:
                                        N.ptr(dist, N.f32p), byref(info)))
         indptr = np.arange(0, n * k + 1, k, dtype=np.uint64)
         return Graph(int(n_neighbors), indptr, idx[: n * k], dist[: n * k], _info(info))
+
+    def connectivities(self, graph: Graph) -> ConnectivityGraph:
+        """scanpy's connectivities of a neighbour graph (ky_connectivities)."""
+        n = len(graph.indptr) - 1
+        k = graph.n_neighbors - 1
+        idx = np.ascontiguousarray(graph.indices, np.uint32)
+        dist = np.ascontiguousarray(graph.distances, np.float32)
+        if len(idx) != n * k or len(dist) != n * k:
+            raise N.KytoError(N.KY_E215, f"connectivities: {len(idx)} indices and {len(dist)} distances for "
+                                         f"{n} rows of {k}")
+        sigma = np.empty(max(n, 1), np.float32)
+        rho = np.empty(max(n, 1), np.float32)
+        h = c_void_p()
+        info = N.ConnInfo()
+        N.check(self._lib.ky_connectivities(self._h, n, int(graph.n_neighbors), N.ptr(idx, N.u32p),
+                                            N.ptr(dist, N.f32p), byref(h), N.ptr(sigma, N.f32p),
+                                            N.ptr(rho, N.f32p), byref(info)))
+        try:
+            edges = int(info.edges)
+            indptr = np.empty(n + 1, np.uint64)
+            cols = np.empty(max(edges, 1), np.uint32)
+            w = np.empty(max(edges, 1), np.float32)
+            N.check(self._lib.ky_graph_download(h, N.ptr(indptr, N.u64p), N.ptr(cols, N.u32p), N.ptr(w, N.f32p)))
+        finally:
+            N.check(self._lib.ky_graph_free(h))
+        return ConnectivityGraph(indptr, cols[:edges], w[:edges], sigma[:n], rho[:n], _info(info))
 
     def nearest_keys(self, points: "DenseMatrix", first: int, count: int, m: int,
                      n_components: int | None = None):

@@ -78,6 +78,10 @@ def lib():
                                          ctypes.c_size_t, ctypes.c_size_t, _f64p, _u32p, _u64p]
         L.ko_round_up.restype = ctypes.c_float
         L.ko_round_up.argtypes = [ctypes.c_double]
+        L.ko_smooth_knn.argtypes = [_f32p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_double, _f32p, _f32p]
+        L.ko_connectivities.restype = ctypes.c_size_t
+        L.ko_connectivities.argtypes = [ctypes.c_size_t, ctypes.c_size_t, _u32p, _f32p, _u64p, _u32p, _f32p,
+                                        _f32p, _f32p]
         L.ko_max_threads.restype = ctypes.c_int
         L.ko_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -396,3 +400,41 @@ def neighbors(points, n_neighbors=15, metric="euclidean", n_components=None, thr
         raise ValueError("KY-E215: embedding not finite")
     keys, rows, _ = knn_exact(pts, n_neighbors - 1, metric, threads)
     return graph(keys, rows, metric)
+
+
+@dataclass
+class Connectivities:
+    """kyto's ConnectivityGraph (connectivities.rs:33-72) and each row's scales."""
+    indptr: np.ndarray   # u64, n + 1
+    indices: np.ndarray  # u32, columns increasing in each row
+    weights: np.ndarray  # f32 in (0, 1]
+    sigma: np.ndarray    # f32, n
+    rho: np.ndarray      # f32, n
+
+
+def smooth_knn(ranked, n_neighbors):
+    """`smooth_knn_distance` (connectivities.rs:125-168) of one row of ranked distances
+    (the cell's own 0 left out): (sigma, rho) as f32."""
+    r = np.ascontiguousarray(ranked, np.float32)
+    sigma = np.zeros(1, np.float32)
+    rho = np.zeros(1, np.float32)
+    lib().ko_smooth_knn(_p(r, _f32p), len(r), n_neighbors, float(np.log2(n_neighbors)), _p(sigma, _f32p),
+                        _p(rho, _f32p))
+    return sigma[0], rho[0]
+
+
+def connectivities(indices, distances, n_neighbors) -> Connectivities:
+    """scanpy's connectivities of a neighbour graph (connectivities.rs:76-123 and the
+    fuzzy union :190-256): rows of n_neighbors - 1 entries, columns increasing."""
+    idx = np.ascontiguousarray(indices, np.uint32).ravel()
+    dist = np.ascontiguousarray(distances, np.float32).ravel()
+    k = n_neighbors - 1
+    n = len(idx) // k
+    indptr = np.zeros(n + 1, np.uint64)
+    out_idx = np.zeros(2 * n * k, np.uint32)
+    out_w = np.zeros(2 * n * k, np.float32)
+    sigma = np.zeros(n, np.float32)
+    rho = np.zeros(n, np.float32)
+    nnz = lib().ko_connectivities(n, n_neighbors, _p(idx, _u32p), _p(dist, _f32p), _p(indptr, _u64p),
+                                  _p(out_idx, _u32p), _p(out_w, _f32p), _p(sigma, _f32p), _p(rho, _f32p))
+    return Connectivities(indptr, out_idx[:nnz].copy(), out_w[:nnz].copy(), sigma, rho)

@@ -470,3 +470,171 @@ void ko_set_threads(int t) {
     (void)t;
 #endif
 }
+
+/* --------------------------------------------------------- connectivities ---- */
+
+/* umap's constants: crates/kyto-exec/src/connectivities.rs:25-30. */
+#define KO_SMOOTH_K_ITERATIONS 64
+#define KO_SMOOTH_K_TOLERANCE 1e-5
+#define KO_MIN_K_DIST_SCALE 1e-3
+
+/* Rust's f32::exp (connectivities.rs:186) is the platform's expf, whose last bit
+ * differs between platforms where exp(x) lies near the midpoint of two floats (glibc
+ * 2.35's expf differs from the correctly rounded value on about 8e-5 of arguments;
+ * the platform that made scanpy's fixtures differs from both on 0.6% of their
+ * weights, each by one unit in the last place). The oracle takes the correctly
+ * rounded value: exp in f64, rounded once to f32. */
+static inline float exp32(float x) { return (float)exp((double)x); }
+
+/* f32 total order (Rust's total_cmp) as an unsigned key. */
+static inline uint32_t total_key(float v) {
+    uint32_t b;
+    memcpy(&b, &v, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+/* numpy's mean of f32 values as umap's numba code gives it: a f32 sum in order over
+ * `len` values, divided in f64 by `count`: connectivities.rs:170-177. */
+static double mean32(const float *values, size_t len, size_t count) {
+    float sum = 0.0f;
+    for (size_t i = 0; i < len; i++) sum += values[i];
+    return (double)sum / (double)count;
+}
+
+/* smooth_knn_distance for one row of ranked distances (the cell's own 0 left out):
+ * connectivities.rs:125-168. */
+void ko_smooth_knn(const float *ranked, size_t k, size_t n_neighbors, double target, float *sigma_out,
+                   float *rho_out) {
+    float rho = 0.0f;
+    for (size_t j = 0; j < k; j++)
+        if (ranked[j] > 0.0f) {
+            rho = ranked[j];
+            break;
+        }
+    const double ceiling = (double)FLT_MAX;
+    double lo = 0.0, hi = ceiling, mid = 1.0;
+    for (int it = 0; it < KO_SMOOTH_K_ITERATIONS; it++) {
+        double sum = 0.0;
+        for (size_t j = 0; j < k; j++) {
+            const float diff = ranked[j] - rho; /* a f32 difference */
+            const double d = (double)diff;
+            sum += d > 0.0 ? exp(-(d / mid)) : 1.0;
+        }
+        if (fabs(sum - target) < KO_SMOOTH_K_TOLERANCE) break;
+        if (sum > target) {
+            hi = mid;
+            mid = (lo + hi) / 2.0;
+        } else {
+            lo = mid;
+            mid = hi >= ceiling ? mid * 2.0 : (lo + hi) / 2.0;
+        }
+    }
+    float sigma = (float)mid;
+    if (rho > 0.0f) {
+        const double floor = KO_MIN_K_DIST_SCALE * mean32(ranked, k, n_neighbors);
+        if ((double)sigma < floor) sigma = (float)floor;
+    }
+    *sigma_out = sigma;
+    *rho_out = rho;
+}
+
+/* compute_membership_strengths for one entry: connectivities.rs:179-188. */
+static float membership(size_t row, uint32_t column, float distance, float sigma, float rho) {
+    if ((size_t)column == row) return 0.0f;
+    const float diff = distance - rho;
+    if (diff <= 0.0f || sigma == 0.0f) return 1.0f;
+    return exp32(-(diff / sigma));
+}
+
+/* scanpy's connectivities of a neighbour graph (n rows of k = n_neighbors - 1 entries,
+ * columns increasing in a row): connectivities.rs:76-123, the fuzzy union
+ * :190-256. Writes the symmetric CSR (indptr n + 1; indices and weights up to 2 n k
+ * entries) and each row's sigma and rho; returns the number of edges. */
+size_t ko_connectivities(size_t n, size_t n_neighbors, const uint32_t *indices, const float *distances,
+                         uint64_t *indptr_out, uint32_t *indices_out, float *weights_out, float *sigma_out,
+                         float *rho_out) {
+    const size_t k = n_neighbors - 1;
+    float *ranked = malloc(sizeof(float) * n * k);
+    float *w = malloc(sizeof(float) * n * k);
+    const double target = log2((double)n_neighbors);
+    int any_zero = 0;
+    for (size_t r = 0; r < n; r++) {
+        /* The row ranked by (distance, column), insertion sort. */
+        float *dst = ranked + r * k;
+        uint32_t cols[64];
+        for (size_t j = 0; j < k; j++) {
+            const float d = distances[r * k + j];
+            const uint32_t c = indices[r * k + j];
+            size_t p = j;
+            while (p > 0 && (total_key(dst[p - 1]) > total_key(d) ||
+                             (total_key(dst[p - 1]) == total_key(d) && cols[p - 1] > c))) {
+                dst[p] = dst[p - 1];
+                cols[p] = cols[p - 1];
+                p--;
+            }
+            dst[p] = d;
+            cols[p] = c;
+        }
+        ko_smooth_knn(dst, k, n_neighbors, target, &sigma_out[r], &rho_out[r]);
+        if (rho_out[r] == 0.0f) any_zero = 1;
+    }
+    /* A row of zero distances takes its floor from the mean of every distance. */
+    if (any_zero) {
+        const double floor = KO_MIN_K_DIST_SCALE * mean32(ranked, n * k, n * n_neighbors);
+        for (size_t r = 0; r < n; r++)
+            if (rho_out[r] == 0.0f && (double)sigma_out[r] < floor) sigma_out[r] = (float)floor;
+    }
+    for (size_t r = 0; r < n; r++)
+        for (size_t j = 0; j < k; j++)
+            w[r * k + j] = membership(r, indices[r * k + j], distances[r * k + j], sigma_out[r], rho_out[r]);
+    /* The transpose by counting (rows come out increasing in each column). */
+    uint64_t *tptr = calloc(n + 1, sizeof(uint64_t));
+    for (size_t e = 0; e < n * k; e++) tptr[indices[e] + 1]++;
+    for (size_t r = 0; r < n; r++) tptr[r + 1] += tptr[r];
+    uint64_t *cursor = malloc(sizeof(uint64_t) * (n + 1));
+    memcpy(cursor, tptr, sizeof(uint64_t) * (n + 1));
+    uint32_t *trow = malloc(sizeof(uint32_t) * n * k);
+    float *tw = malloc(sizeof(float) * n * k);
+    for (size_t e = 0; e < n * k; e++) {
+        const uint64_t slot = cursor[indices[e]]++;
+        trow[slot] = (uint32_t)(e / k);
+        tw[slot] = w[e];
+    }
+    /* w + w^T - w o w^T, merged by column, zeros dropped. */
+    size_t nnz = 0;
+    indptr_out[0] = 0;
+    for (size_t r = 0; r < n; r++) {
+        size_t f = r * k, fe = (r + 1) * k, b = tptr[r], be = tptr[r + 1];
+        while (f < fe || b < be) {
+            uint32_t col;
+            float a = 0.0f, t = 0.0f;
+            if (b >= be || (f < fe && indices[f] < trow[b])) {
+                col = indices[f];
+                a = w[f++];
+            } else if (f >= fe || trow[b] < indices[f]) {
+                col = trow[b];
+                t = tw[b++];
+            } else {
+                col = indices[f];
+                a = w[f++];
+                t = tw[b++];
+            }
+            const float sum = a + t;
+            const float prod = a * t;
+            const float weight = sum - prod;
+            if (weight != 0.0f) {
+                indices_out[nnz] = col;
+                weights_out[nnz] = weight;
+                nnz++;
+            }
+        }
+        indptr_out[r + 1] = nnz;
+    }
+    free(ranked);
+    free(w);
+    free(tptr);
+    free(cursor);
+    free(trow);
+    free(tw);
+    return nnz;
+}

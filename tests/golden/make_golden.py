@@ -12,7 +12,9 @@ keeps the arrays the pca() + neighbors() parity tests need:
   variance, variance_ratio) on those HVGs (pca.npz);
 - synthetic_neighbors.npz / pbmc3k_neighbors.npz: scanpy's exact kNN distance graph
   (n_neighbors=15, euclidean) on the PCA embedding (neighbors.npz), as
-  int32 indices / float64 distances.
+  int32 indices / float64 distances, and its connectivities graph (UMAP's fuzzy
+  weights on that graph) as conn_indptr / conn_indices (int32) / conn_weights
+  (float32).
 
 Run from the repository root: python tests/golden/make_golden.py
 """
@@ -61,6 +63,9 @@ def main():
             indptr=nb["distances/indptr"].astype(np.int64),
             indices=nb["distances/indices"].astype(np.int32),
             distances=nb["distances/data"].astype(np.float64),
+            conn_indptr=nb["connectivities/indptr"].astype(np.int64),
+            conn_indices=nb["connectivities/indices"].astype(np.int32),
+            conn_weights=nb["connectivities/data"].astype(np.float32),
         )
 
 

@@ -58,3 +58,58 @@ def clustered(seed, n, width, clusters, offset):
             noise = np.float32(st.next() + st.next() + st.next() - np.float32(1.5))
             out[r, j] = np.float32(centres[c * width + j] + noise + np.float32(offset))
     return out
+
+
+def scanpy_graph(name):
+    """scanpy's neighbour graph as umap receives it (indices u32 and its float64
+    distances as float32, n x 14) and scanpy's connectivities (indptr, indices,
+    weights)."""
+    nb = load(f"{name}_neighbors.npz")
+    n = len(nb["indptr"]) - 1
+    idx = nb["indices"].astype(np.uint32).reshape(n, -1)
+    dist = nb["distances"].astype(np.float32).reshape(n, -1)
+    return idx, dist, (nb["conn_indptr"].astype(np.uint64), nb["conn_indices"].astype(np.uint32),
+                       nb["conn_weights"])
+
+
+# The neighbour graphs of connectivities.rs's unit tests (:266-397): rows of
+# (column, distance), n_neighbors = row length + 1.
+KNOWN_GRAPHS = {
+    "line": [[(1, 1.0), (2, 3.0)], [(0, 1.0), (2, 2.0)], [(0, 3.0), (1, 2.0)]],
+    "duplicates": [[(1, 0.0), (2, 1.0)], [(0, 0.0), (2, 1.0)], [(1, 1.0), (2, 0.0)]],
+    "same": [[(1, 0.0), (2, 0.0)], [(0, 0.0), (2, 0.0)], [(0, 0.0), (1, 0.0)]],
+    "far": [[(1, 1.0), (2, 1.0), (3, 1e6)], [(0, 1.0), (2, 1.0), (3, 1.0)], [(0, 1.0), (1, 1.0), (3, 1.0)],
+            [(0, 1e6), (1, 1.0), (2, 1.0)]],
+}
+
+
+def known_graph(name):
+    rows = KNOWN_GRAPHS[name]
+    idx = np.array([[c for c, _ in r] for r in rows], np.uint32)
+    dist = np.array([[d for _, d in r] for r in rows], np.float32)
+    return idx, dist, idx.shape[1] + 1
+
+
+def random_graph(n, k, seed, ties=False):
+    """A random neighbour graph (columns increasing, no self entries), distances drawn
+    so rows hold ties and zeros when `ties`."""
+    rng = np.random.default_rng(seed)
+    idx = np.empty((n, k), np.uint32)
+    for r in range(n):
+        c = rng.choice(n - 1, size=k, replace=False)
+        c[c >= r] += 1
+        idx[r] = np.sort(c)
+    dist = rng.gamma(2.0, 1.0, size=(n, k)).astype(np.float32)
+    if ties:
+        dist = np.round(dist * 4) / 4
+        dist[rng.random((n, k)) < 0.05] = 0.0
+        dist[rng.random(n) < 0.01] = 0.0  # rows of zero distances
+    return idx, dist.astype(np.float32)
+
+
+def dense_union(idx, w, n):
+    """w + w^T - w o w^T on a dense float32 matrix, as scipy orders it."""
+    W = np.zeros((n, n), np.float32)
+    rows = np.repeat(np.arange(n), idx.shape[1])
+    W[rows, idx.ravel()] = w.ravel()
+    return (W + W.T) - W * W.T

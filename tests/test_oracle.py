@@ -207,3 +207,96 @@ def test_round_up_never_rounds_down():
     assert lib.ko_round_up(1.0) == 1.0
     assert lib.ko_round_up(1.0 + 1e-12) >= 1.0 + 1e-12
     assert lib.ko_round_up(float("inf")) == float("inf")
+
+
+# ----------------------------------------------------------- connectivities ----
+
+from tests.helpers import dense_union, known_graph, random_graph, scanpy_graph  # noqa: E402
+
+
+@pytest.mark.parametrize("name", ["synthetic", "pbmc3k"])
+def test_connectivities_against_scanpy_fixture(name):
+    # connectivities_parity.rs: scanpy's own distances in, the same edges out, weights
+    # within connectivities.weights = 1e-5 (tolerances.toml:173-178). Measured: every
+    # edge equal; weights equal bit for bit but for 46 of 9,762 / 370 of 62,096, one or
+    # two units in the last place, each at an exp(x) within 0.01 units of a rounding
+    # midpoint (the fixture platform's float32 exp and the correctly rounded one part).
+    idx, dist, (ip, ix, w) = scanpy_graph(name)
+    c = oracle.connectivities(idx, dist, 15)
+    assert np.array_equal(c.indptr, ip)
+    assert np.array_equal(c.indices, ix)
+    diff = np.abs(c.weights.astype(np.float64) - w)
+    assert diff.max() <= 1e-5
+    assert diff.max() <= 2.5e-7  # two units in the last place at 1.0
+    assert (c.weights == w).mean() >= 0.99
+
+
+def test_sigma_known_answers():
+    # connectivities.rs:319-350: memberships sum to log2(k); the floor's rounding
+    # (umap-learn 0.5.12 gives sigma bits 983688805).
+    d = np.array([0.5, 0.9, 1.4, 2.0, 2.7], np.float32)
+    sigma, rho = oracle.smooth_knn(d, 6)
+    assert rho == np.float32(0.5)
+    m = np.exp(-(np.maximum(d - rho, 0) / np.float64(sigma)))
+    assert abs(m.sum() - np.log2(6)) < 1e-4
+    r = np.array([0, 0, 0, 0, 2.4006681, 2.417103, 2.4605973, 2.4799116, 2.5937085], np.float32)
+    sigma, rho = oracle.smooth_knn(r, 10)
+    assert rho == np.float32(2.4006681)
+    assert int(np.float32(sigma).view(np.uint32)) == 983688805
+
+
+def _weight(c, r, col):
+    s, e = int(c.indptr[r]), int(c.indptr[r + 1])
+    j = np.searchsorted(c.indices[s:e], col)
+    assert c.indices[s + j] == col
+    return c.weights[s + j]
+
+
+def test_connectivities_known_graphs():
+    # connectivities.rs:266-397.
+    c = oracle.connectivities(*known_graph("line"))
+    assert c.indptr.tolist() == [0, 2, 4, 6] and c.indices.tolist() == [1, 2, 0, 2, 0, 1]
+    for r in range(3):
+        for col in range(3):
+            if r != col:
+                assert _weight(c, r, col) == _weight(c, col, r)
+    assert _weight(c, 0, 1) == 1.0
+    assert _weight(c, 1, 2) >= 1.0 - np.finfo(np.float32).eps
+    assert 0.0 < _weight(c, 0, 2) < 1.0
+    c = oracle.connectivities(*known_graph("duplicates"))
+    assert c.indptr.tolist() == [0, 2, 4, 6] and c.indices.tolist() == [1, 2, 0, 2, 0, 1]
+    assert c.weights[0] == 1.0 and np.all((c.weights > 0) & (c.weights <= 1))
+    c = oracle.connectivities(*known_graph("same"))
+    assert c.weights.tolist() == [1.0] * 6
+    c = oracle.connectivities(*known_graph("far"))
+    assert c.indptr.tolist() == [0, 2, 5, 8, 10]
+    assert c.indices.tolist() == [1, 2, 0, 2, 3, 0, 1, 3, 1, 2]
+
+
+@pytest.mark.parametrize("ties", [False, True])
+def test_connectivities_union_equals_the_dense_formula(ties):
+    # The sparse merge against (W + W^T) - W o W^T on a dense float32 matrix, with the
+    # memberships restated in numpy from the oracle's sigma and rho (f32 differences
+    # and quotients, exp rounded once from f64).
+    n, k = 300, 9
+    idx, dist = random_graph(n, k, seed=5, ties=ties)
+    c = oracle.connectivities(idx, dist, k + 1)
+    sig, rho = c.sigma[:, None], c.rho[:, None]
+    diff = (dist - rho).astype(np.float32)
+    with np.errstate(over="ignore", divide="ignore", invalid="ignore"):
+        x = (-(diff / sig)).astype(np.float32)
+        w = np.where((diff <= 0) | (sig == 0), np.float32(1), np.exp(x.astype(np.float64)).astype(np.float32))
+    w = np.where(idx == np.arange(n)[:, None], np.float32(0), w).astype(np.float32)
+    U = dense_union(idx, w, n)
+    nz = np.nonzero(U)
+    assert np.array_equal(c.indptr, np.concatenate([[0], np.cumsum(np.bincount(nz[0], minlength=n))]))
+    assert np.array_equal(c.indices, nz[1].astype(np.uint32))
+    assert np.array_equal(c.weights.view(np.uint32), U[nz].view(np.uint32))
+    # Rows not floored: the memberships of the bisected sigma sum to log2(k + 1).
+    target = np.log2(k + 1)
+    with np.errstate(over="ignore", divide="ignore", invalid="ignore"):
+        s = np.where(diff > 0, np.exp(-(diff.astype(np.float64) / c.sigma[:, None].astype(np.float64))), 1.0).sum(1)
+    floor = 1e-3 * dist.sum(axis=1, dtype=np.float32).astype(np.float64) / (k + 1)
+    free = (c.rho > 0) & (c.sigma.astype(np.float64) > floor * (1 + 1e-6))
+    assert free.sum() > n // 2
+    assert np.abs(s[free] - target).max() < 1e-4
