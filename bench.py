@@ -282,6 +282,24 @@ def main():
         out["learned"] = learned_leg(dev, emb, k, exact, args)
         log(f"recall@15 {out['recall_at_15']:.5f} (graph equal to FP32: {same}); kNN {knn_ms:.1f} ms vs plain FP32 "
             f"{plain.info['total_ms']:.1f} ms ({out['knn_speedup_vs_plain_fp32']:.1f}x); learned {out['learned']}")
+    # Outside the timed region: the connectivities graph the neighbors step declares
+    # (UMAP's fuzzy weights on the last step's graph), the second of two calls timed.
+    dev.connectivities(g)
+    conn = dev.connectivities(g)
+    out["connectivities"] = {"edges": int(conn.info["edges"]), "ms": conn.info["total_ms"],
+                             "upload_ms": conn.info["upload_ms"], "scales_ms": conn.info["scales_ms"],
+                             "union_ms": conn.info["union_ms"], "zero_rho_rows": int(conn.info["zero_rho_rows"])}
+    log(f"connectivities: {conn.info['edges']} edges in {conn.info['total_ms']:.1f} ms "
+        f"(upload {conn.info['upload_ms']:.1f}, scales {conn.info['scales_ms']:.1f}, union {conn.info['union_ms']:.1f})")
+    if not args.no_cpu_baseline:
+        import oracle
+        t0 = time.perf_counter()
+        ref = oracle.connectivities(g.rows(), g.distances.reshape(-1, k - 1), k)
+        out["connectivities"]["cpu_oracle_ms"] = (time.perf_counter() - t0) * 1e3
+        out["connectivities"]["cpu_oracle_cores"] = 1
+        out["connectivities"]["equal_to_oracle"] = bool(
+            np.array_equal(ref.indices, conn.indices) and
+            np.array_equal(ref.weights.view(np.uint32), conn.weights.view(np.uint32)))
     embedding = emb.download()
     emb.free()
     if not args.no_cpu_baseline and layer is not None:

@@ -31,34 +31,32 @@
 
 namespace ky {
 
-namespace {
-
 constexpr int SMOOTH_K_ITERATIONS = 64;      // umap's n_iter (connectivities.rs:26)
 constexpr double SMOOTH_K_TOLERANCE = 1e-5;  // :28
 constexpr double MIN_K_DIST_SCALE = 1e-3;    // :30
 
 // Rust's total_cmp order of f32 values as an unsigned key, and back.
-__device__ __forceinline__ uint32_t total_key(float v) {
+static __device__ __forceinline__ uint32_t total_key(float v) {
     const uint32_t b = __float_as_uint(v);
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
-__device__ __forceinline__ float from_total_key(uint32_t u) {
+static __device__ __forceinline__ float from_total_key(uint32_t u) {
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
 
 // f32 exp, correctly rounded: exp in f64 rounded once to f32.
-__device__ __forceinline__ float exp32(float x) { return (float)exp((double)x); }
+static __device__ __forceinline__ float exp32(float x) { return (float)exp((double)x); }
 
 // compute_membership_strengths for one entry (connectivities.rs:179-188).
-__device__ __forceinline__ float membership(uint64_t row, uint32_t column, float distance, float sigma, float rho) {
+static __device__ __forceinline__ float membership(uint64_t row, uint32_t column, float distance, float sigma, float rho) {
     if ((uint64_t)column == row) return 0.0f;
     const float diff = distance - rho;
     if (diff <= 0.0f || sigma == 0.0f) return 1.0f;
     return exp32(-(diff / sigma));
 }
 
-// flags: bit 0 an invalid row (columns not increasing below n, or a distance that is
-// not finite and >= 0), bit 1 a row with rho = 0.
+// flags bit 0: an invalid row (columns not increasing below n, or a distance that is
+// not finite and >= 0); zero_rows counts the rows with rho = 0.
 template <int KMAX>
 __global__ __launch_bounds__(256) void fuzzy_scales(const uint32_t *__restrict__ indices,
                                                     const float *__restrict__ dist, uint64_t n, uint32_t k,
@@ -235,14 +233,12 @@ __global__ void union_rows(const uint32_t *__restrict__ indices, const float *__
 }
 
 template <int KMAX>
-void launch_scales(hipStream_t st, const uint32_t *idx, const float *dist, uint64_t n, uint32_t k, uint32_t nn,
+static void launch_scales(hipStream_t st, const uint32_t *idx, const float *dist, uint64_t n, uint32_t k, uint32_t nn,
                    double target, float *ranked, float *sigma, float *rho, float *w, unsigned *flags,
                    unsigned long long *zero_rows) {
     fuzzy_scales<KMAX><<<div_up(n, 256), 256, 0, st>>>(idx, dist, n, k, nn, target, ranked, sigma, rho, w, flags,
                                                        zero_rows);
 }
-
-}  // namespace
 
 void connectivities_run(Context &ctx, uint64_t n, uint32_t n_neighbors, const uint32_t *h_indices,
                         const float *h_dist, Graph &out, float *h_sigma, float *h_rho, ky_conn_info *info) {

@@ -476,21 +476,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                             else if (g == 2) acc[b].s89ab = r;
                             else acc[b].scdef = r;
                         }
-                    // One test for the super-block: the smallest of each lane's 16 keys of a
-                    // query against the query's threshold.
+                    // One test for the super-block: the smallest of each lane's 4 MB keys of
+                    // a query (a tree of 3-input mins) against the query's threshold; the
+                    // column blocks' ballots OR'ed in scalar registers, one scalar branch.
                     bool hit[NB];
-                    bool any = false;
+                    uint64_t hits = 0;
 #pragma unroll
                     for (int b = 0; b < NB; b++) {
-                        float lo = fminf(fminf(acc[b][0], acc[b][1]), fminf(acc[b][2], acc[b][3]));
+                        float part[(4 * MB + 2) / 3];
 #pragma unroll
-                        for (int g = 1; g < MB; g++)
-                            lo = fminf(lo, fminf(fminf(acc[b][4 * g], acc[b][4 * g + 1]),
-                                                 fminf(acc[b][4 * g + 2], acc[b][4 * g + 3])));
+                        for (int e = 0; e < 4 * MB; e += 3)
+                            part[e / 3] = e + 2 < 4 * MB ? fminf(fminf(acc[b][e], acc[b][e + 1]), acc[b][e + 2])
+                                                         : e + 1 < 4 * MB ? fminf(acc[b][e], acc[b][e + 1])
+                                                                          : acc[b][e];
+                        float lo = part[0];
+#pragma unroll
+                        for (int e = 1; e < (4 * MB + 2) / 3; e += 2)
+                            lo = e + 1 < (4 * MB + 2) / 3 ? fminf(fminf(lo, part[e]), part[e + 1]) : fminf(lo, part[e]);
                         hit[b] = lo <= thr[b];
-                        any = any || hit[b];
+                        hits |= __ballot(hit[b]);
                     }
-                    if (!__ballot(any)) continue;
+                    if (!hits) continue;
                     // Appends: the 4 lanes of a query take buffer slots in lane order after
                     // its count, and stage slots after the wave's staged entries. The keys of
                     // a column block are walked with a uniform index into a 16-register
