@@ -312,11 +312,13 @@ constexpr int LEAD_TILES = 16;        // tiles scanned before a block's own posi
 constexpr uint64_t EST_MIN_ROWS = 1 << 17;  // threshold estimates from a sample above this size
 // The sample: every 12th row of the scan order. With 32 subsets of n / 384 sampled
 // rows and the p-th smallest minimum as the estimate, about -ln(1 - p/32) * 384 rows
-// fall at or below it on average; p is the smallest giving 8 (k + 1) (p = 9 and 127
-// rows for k = 14, p = 15 and 242 rows for k = 29).
+// fall at or below it on average; p is the smallest giving 7 (k + 1) (p = 8 and 112
+// rows for k = 14, p = 14 and 221 rows for k = 29).
 constexpr uint32_t EST_STRIDE = 12;
 
 constexpr unsigned SCAP = 64;  // staged appends per wavefront
+constexpr unsigned MAX_TAIL_SEG = 16;                // point segments of a tail query block
+constexpr unsigned MERGE_CAP = MAX_TAIL_SEG * 64;    // a tail query's merged lists (m <= 64)
 
 // The candidate scan. Points: np rows, point p being row p * pstride of the scan
 // order (P'). Queries: i < nq, query i being row qpos[i] of the scan order (row i
@@ -343,7 +345,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq,
     const uint32_t *__restrict__ perm, uint32_t m, const float *__restrict__ thr_init, Entry *__restrict__ buf,
     uint32_t *__restrict__ out_rows, float *__restrict__ out_keys, uint32_t *__restrict__ out_count,
-    float *__restrict__ out_thr) {
+    float *__restrict__ out_thr, uint32_t main_blocks, uint32_t n_seg, uint32_t seg_tiles,
+    uint32_t *__restrict__ seg_rows, float *__restrict__ seg_keys, uint32_t *__restrict__ seg_count,
+    float *__restrict__ seg_thr) {
     __shared__ __attribute__((aligned(16))) unsigned char tile[3][TILE * ROW_BYTES];
     // Per wave: the append stage, which a compaction's sort scratch overlays.
     __shared__ __attribute__((aligned(16))) uint32_t stage_key[4][SCAP];
@@ -351,9 +355,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     __shared__ uint16_t stage_dst[4][SCAP];
     __shared__ uint32_t qrow_lds[QPB];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
-    // XCD-aware block order.
-    const unsigned nblk = gridDim.x, xcd = blockIdx.x % 8, per = nblk / 8, extra = nblk % 8;
-    const unsigned lblk = xcd * per + min(xcd, extra) + blockIdx.x / 8;
+    // Blocks below main_blocks scan every point for query block lblk (XCD-aware order);
+    // the others are the tail: query block main_blocks + t / n_seg over point segment
+    // t % n_seg (seg_tiles tiles), with its own buffers and outputs.
+    const bool tail = blockIdx.x >= main_blocks;
+    unsigned lblk, seg = 0;
+    if (!tail) {
+        const unsigned nblk = main_blocks, xcd = blockIdx.x % 8, per = nblk / 8, extra = nblk % 8;
+        lblk = xcd * per + min(xcd, extra) + blockIdx.x / 8;
+    } else {
+        const unsigned t = blockIdx.x - main_blocks;
+        lblk = main_blocks + t / n_seg;
+        seg = t % n_seg;
+    }
+    const uint64_t main_q = (uint64_t)main_blocks * QPB;
+    // Buffer (and tail output) slot of query i.
+    auto slot = [&](uint64_t i) { return tail ? main_q + (i - main_q) * n_seg + seg : i; };
     const uint64_t qbase = (uint64_t)lblk * QPB + (uint64_t)wave * QPW;  // this wave's first query
     uint32_t *qrow = qrow_lds + wave * QPW;  // scan-order row of each of the wave's queries
     // B fragments (the wave's queries, loaded once), and per column block b the
@@ -381,7 +398,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         wave_sync();
         if (lane < staged) {
             const unsigned dst = sd[lane];
-            buf[(qbase + (dst >> 9)) * BCAP + (dst & (BCAP - 1))] = make_uint2(sk[lane], sr[lane]);
+            buf[slot(qbase + (dst >> 9)) * BCAP + (dst & (BCAP - 1))] = make_uint2(sk[lane], sr[lane]);
         }
         staged = 0;
         wave_sync();
@@ -390,7 +407,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     auto compact_query = [&](int b, unsigned l, bool sorted, uint32_t *sk, uint32_t *sr) {
         const unsigned c = __builtin_amdgcn_readlane(cb[b], l);
         float t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr[b]), l));
-        const unsigned kept = compact_buffer(buf + (qbase + 16 * b + l) * BCAP, c, m, &t, sorted, sk, sr);
+        const unsigned kept = compact_buffer(buf + slot(qbase + 16 * b + l) * BCAP, c, m, &t, sorted, sk, sr);
         if (lane % 16 == l) {
             cb[b] = kept;
             thr[b] = t;
@@ -405,7 +422,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     // position (their near neighbours first tighten the thresholds early); with one,
     // every block starts at tile 0, so the blocks resident on an XCD read the same
     // tiles at about the same time through its L2.
-    const uint64_t first_tile = thr_init ? 0 : (min<uint64_t>(own, n_tiles - 1) + n_tiles - lead) % n_tiles;
+    // A tail block scans its segment only.
+    const uint64_t first_tile =
+        tail ? (uint64_t)seg * seg_tiles : thr_init ? 0 : (min<uint64_t>(own, n_tiles - 1) + n_tiles - lead) % n_tiles;
+    const uint64_t n_scan = tail ? min<uint64_t>(seg_tiles, n_tiles - first_tile) : n_tiles;
     // Tiles go straight from global memory to LDS (global_load_lds_dwordx4) into three
     // buffers: tile t + 2 is requested once tile t's keys are done, so it has the
     // whole of tile t + 1 to arrive. A tile is 128 rows of 128 bytes with the 16-byte
@@ -429,12 +449,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     auto wrap_next = [&](uint32_t t) { return t + 1 == (uint32_t)n_tiles ? 0u : t + 1; };
     uint32_t tcur = (uint32_t)first_tile, tfetch = wrap_next(tcur);
     fetch((uint64_t)tcur * TILE, tile[0]);
-    if (n_tiles > 1) fetch((uint64_t)tfetch * TILE, tile[1]);
+    if (n_scan > 1) fetch((uint64_t)tfetch * TILE, tile[1]);
     tfetch = wrap_next(tfetch);
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
     int cur = 0;
-    for (uint64_t it = 0; it < n_tiles; it++) {
+    for (uint64_t it = 0; it < n_scan; it++) {
         const uint64_t t0 = (uint64_t)tcur * TILE;
         const int nxt = cur == 2 ? 0 : cur + 1, after = nxt == 2 ? 0 : nxt + 1;
         // tb: this tile, nx: the buffer tile t + 2 goes to; sk, sr, sd: the stage. The
@@ -563,7 +583,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     }
                 }
             }
-            if (it + 2 < n_tiles) {
+            if (it + 2 < n_scan) {
                 fetch((uint64_t)tfetch * TILE, nx);
                 __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4): all but tile t + 2 (tile t + 1 is in LDS)
             } else {
@@ -597,19 +617,77 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 kept = compact_query(b, ql % 16, true, stage_key[wave], stage_row[wave]);
                 t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr[b]), ql % 16));
             }
-        const uint64_t o = qout[i];
-        if (out_rows) {
+        const uint64_t o = tail ? (i - main_q) * n_seg + seg : qout[i];
+        uint32_t *const o_rows = tail ? seg_rows : out_rows;
+        float *const o_keys = tail ? seg_keys : out_keys;
+        if (o_rows) {
             for (unsigned e = lane; e < m; e += 64) {
                 const uint32_t row = stage_row[wave][e];
-                out_rows[o * m + e] = e < kept && row < 0xFFFFFFFFu ? perm[row] : 0xFFFFFFFFu;
-                out_keys[o * m + e] = e < kept ? unorder_f32(stage_key[wave][e]) : INFINITY;
+                o_rows[o * m + e] = e < kept && row < 0xFFFFFFFFu ? perm[row] : 0xFFFFFFFFu;
+                o_keys[o * m + e] = e < kept ? unorder_f32(stage_key[wave][e]) : INFINITY;
             }
         }
         if (lane == 0) {
-            out_count[o] = kept;
-            out_thr[o] = t;
+            (tail ? seg_count : out_count)[o] = kept;
+            (tail ? seg_thr : out_thr)[o] = t;
         }
         wave_sync();
+    }
+}
+
+// The tail's segment lists merged (one wavefront a query): every segment kept all its
+// points at or below its threshold (its m-th key, or the starting one), so below the
+// smallest of them, T, the union is whole; its m smallest pairs at or below T are the
+// query's candidates, as a scan of every point would leave them (the threshold the
+// m-th key when m were kept, else T). Output at slot qout[j].
+__global__ __launch_bounds__(64) void merge_tail(const uint32_t *__restrict__ seg_rows,
+                                                 const float *__restrict__ seg_keys,
+                                                 const uint32_t *__restrict__ seg_count,
+                                                 const float *__restrict__ seg_thr, uint64_t nq, uint32_t n_seg,
+                                                 uint32_t m, const uint32_t *__restrict__ qout,
+                                                 uint32_t *__restrict__ out_rows, float *__restrict__ out_keys,
+                                                 uint32_t *__restrict__ out_count, float *__restrict__ out_thr) {
+    __shared__ uint32_t skey[MERGE_CAP];
+    __shared__ uint32_t srow[MERGE_CAP];
+    const uint64_t j = blockIdx.x;
+    if (j >= nq) return;
+    const unsigned lane = lane_id();
+    float T = INFINITY;
+    for (uint32_t s = 0; s < n_seg; s++) T = fminf(T, seg_thr[j * n_seg + s]);
+    unsigned total = 0;  // wave-uniform
+    for (uint32_t s = 0; s < n_seg; s++) {
+        const uint64_t sl = j * n_seg + s;
+        const unsigned cnt = seg_count[sl];
+        for (unsigned e0 = 0; e0 < cnt; e0 += 64) {
+            const unsigned e = e0 + lane;
+            const float key = e < cnt ? seg_keys[sl * m + e] : INFINITY;
+            const bool keep = e < cnt && key <= T;
+            const uint64_t mask = __ballot(keep);
+            if (keep) {
+                const unsigned at = total + mbcnt(mask);
+                skey[at] = order_f32(key);
+                srow[at] = seg_rows[sl * m + e];
+            }
+            total += (unsigned)__popcll(mask);
+        }
+    }
+    unsigned N = 64;
+    while (N < total) N <<= 1;
+    for (unsigned e = total + lane; e < N; e += 64) {
+        skey[e] = 0xFFFFFFFFu;
+        srow[e] = 0xFFFFFFFFu;
+    }
+    wave_sync();
+    wave_bitonic<uint32_t>(skey, srow, N);
+    const unsigned kept = min(total, m);
+    const uint64_t o = qout[j];
+    for (unsigned e = lane; e < m; e += 64) {
+        out_rows[o * m + e] = e < kept ? srow[e] : 0xFFFFFFFFu;
+        out_keys[o * m + e] = e < kept ? unorder_f32(skey[e]) : INFINITY;
+    }
+    if (lane == 0) {
+        out_count[o] = kept;
+        out_thr[o] = total >= m ? unorder_f32(skey[m - 1]) : T;
     }
 }
 
@@ -825,15 +903,36 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
     // every query at that threshold, so its buffer takes only the rows at or below it
     // (any starting threshold gives the same candidates; a low one only leaves fewer
     // than m).
-    DevBuf<Entry> buf(nq * BCAP);
     DevBuf<float> thr_est;
     const unsigned blocks = div_up(nq, QPB);
+    // The last round of blocks: when the query blocks fill the device's resident
+    // blocks r whole times and a remainder of at most half of them, the remainder
+    // would run almost alone at the end. Its query blocks are split instead over n_seg point segments (blocks
+    // of their own in the same launch, filling the free slots), and merge_tail joins
+    // their lists.
+    int per_cu = 0;
+    KY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, search_bf16, 256, 0));
+    const unsigned resident = (unsigned)std::max(per_cu, 1) * (unsigned)std::max(ctx.n_cu, 1);
+    unsigned main_blocks = blocks, n_seg = 1, seg_tiles = 0;
+    const uint64_t n_tiles = div_up(n, TILE);
+    if (blocks > resident && blocks % resident != 0 && 2 * (blocks % resident) <= resident) {
+        const unsigned rem = blocks % resident;
+        main_blocks = blocks - rem;
+        n_seg = (unsigned)std::min<uint64_t>(std::min<unsigned>(MAX_TAIL_SEG, resident / rem), n_tiles);
+        seg_tiles = div_up(n_tiles, n_seg);
+        n_seg = div_up(n_tiles, seg_tiles);  // no empty segment
+    }
+    const uint64_t main_q = std::min<uint64_t>(nq, (uint64_t)main_blocks * QPB);
+    const uint64_t tail_q = nq - main_q;
+    DevBuf<Entry> buf((main_q + tail_q * n_seg) * BCAP);
+    DevBuf<uint32_t> seg_rows(tail_q * n_seg * m), seg_count(tail_q * n_seg);
+    DevBuf<float> seg_keys(tail_q * n_seg * m), seg_thr(tail_q * n_seg);
     if (n >= EST_MIN_ROWS) {
         thr_est.alloc(nq);
         const uint64_t np = div_up(n, EST_STRIDE);
         uint32_t pick = 1;
         while (pick < EST_SUBSETS - 1 &&
-               -std::log1p(-(double)pick / EST_SUBSETS) * EST_SUBSETS * EST_STRIDE < 8.0 * (k + 1))
+               -std::log1p(-(double)pick / EST_SUBSETS) * EST_SUBSETS * EST_STRIDE < 7.0 * (k + 1))
             pick++;
         estimate_bf16<<<div_up(nq, EST_QPB), 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, pick,
                                                             thr_est.ptr);
@@ -843,10 +942,17 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
     KY_HIP(hipEventCreate(&ev[0]));
     KY_HIP(hipEventCreate(&ev[1]));
     KY_HIP(hipEventRecord(ev[0], st));
-    search_bf16<<<blocks, 256, 0, st>>>(P.ptr, Q.ptr, n, 1, qpos, qout, nq, perm.ptr, m, thr_est.ptr, buf.ptr,
-                                               d_rows, d_keys, d_count, d_thr);
+    const unsigned grid = main_blocks + (blocks - main_blocks) * n_seg;
+    search_bf16<<<grid, 256, 0, st>>>(P.ptr, Q.ptr, n, 1, qpos, qout, nq, perm.ptr, m, thr_est.ptr, buf.ptr, d_rows,
+                                      d_keys, d_count, d_thr, main_blocks, n_seg, seg_tiles, seg_rows.ptr,
+                                      seg_keys.ptr, seg_count.ptr, seg_thr.ptr);
     KY_HIP(hipGetLastError());
     KY_HIP(hipEventRecord(ev[1], st));
+    if (tail_q) {
+        merge_tail<<<(unsigned)tail_q, 64, 0, st>>>(seg_rows.ptr, seg_keys.ptr, seg_count.ptr, seg_thr.ptr, tail_q,
+                                                    n_seg, m, qout + main_q, d_rows, d_keys, d_count, d_thr);
+        KY_HIP(hipGetLastError());
+    }
     // A query whose starting threshold left fewer than k rows at or below it ends with
     // fewer than k candidates: the re-rank leaves it uncertified and the exact search
     // takes it (knn.hip).

@@ -140,3 +140,19 @@ def test_large_input_threshold_estimate(device):
     got = g16.rows()[first:first + count]
     assert np.array_equal(got, want_rows)
     assert np.array_equal(g16.distances.reshape(-1, 14)[first:first + count], want_dist)
+
+
+def test_scan_tail_split_over_point_segments(device):
+    # 200,000 rows: 782 query blocks against 768 resident ones, so the last 14 blocks'
+    # queries are scanned as 16 point segments each and their lists merged. The graph
+    # must still be the FP32 path's on every row, and the oracle's on sampled rows.
+    pts = _large_clustered(200_000, 50, 16, seed=23)
+    g16 = _exact(device, pts, 15, "bf16")
+    g32 = _exact(device, pts, 15, "fp32")
+    assert np.array_equal(g16.rows(), g32.rows())
+    assert np.array_equal(g16.distances.view(np.uint32), g32.distances.view(np.uint32))
+    assert g16.info["fallback"] < 1000
+    first, count = 199_000, 400
+    keys, rows, _ = oracle.knn_exact_range(pts, 14, first, count)
+    o = np.argsort(rows, axis=1)
+    assert np.array_equal(g16.rows()[first:first + count], np.take_along_axis(rows, o, 1))
