@@ -347,6 +347,95 @@ static double min_singular_bound(const Mat &r) {
     return 1.0 / std::sqrt(ss);
 }
 
+// Gram matrix a^T a (n x n) of a tall a (m x n): partial Grams of 64-row chunks in
+// parallel, added in chunk order (the same bits on any thread count).
+static Mat gram(const Mat &a) {
+    const size_t m = a.rows, n = a.cols, CH = 64, nch = (m + CH - 1) / CH;
+    std::vector<double> part(nch * n * n, 0.0);
+    parallel_if(nch, CH * n * n, [&](size_t c0, size_t c1) {
+        for (size_t c = c0; c < c1; c++) {
+            double *g = &part[c * n * n];
+            for (size_t r = c * CH; r < std::min(m, (c + 1) * CH); r++) {
+                const double *x = a.row(r);
+                for (size_t i = 0; i < n; i++) {
+                    const double xi = x[i];
+                    double *gi = g + i * n;
+                    for (size_t j = i; j < n; j++) gi[j] += xi * x[j];
+                }
+            }
+        }
+    });
+    Mat g(n, n);
+    for (size_t c = 0; c < nch; c++)
+        for (size_t i = 0; i < n; i++)
+            for (size_t j = i; j < n; j++) g.at(i, j) += part[c * n * n + i * n + j];
+    for (size_t i = 0; i < n; i++)
+        for (size_t j = 0; j < i; j++) g.at(i, j) = g.at(j, i);
+    return g;
+}
+
+// g = r^T r with r upper triangular; false when a pivot is not positive.
+static bool cholesky_upper(const Mat &g, Mat &r) {
+    const size_t n = g.rows;
+    r = Mat(n, n);
+    for (size_t j = 0; j < n; j++) {
+        double d = g.at(j, j);
+        for (size_t k = 0; k < j; k++) d -= r.at(k, j) * r.at(k, j);
+        if (!(d > 0.0) || !std::isfinite(d)) return false;
+        const double rjj = std::sqrt(d);
+        r.at(j, j) = rjj;
+        for (size_t i = j + 1; i < n; i++) {
+            double v = g.at(j, i);
+            for (size_t k = 0; k < j; k++) v -= r.at(k, j) * r.at(k, i);
+            r.at(j, i) = v / rjj;
+        }
+    }
+    return true;
+}
+
+// a <- a r^-1 (r upper triangular), row by row.
+static void solve_right_upper(Mat &a, const Mat &r) {
+    const size_t n = r.rows;
+    parallel_if(a.rows, n * n, [&](size_t r0, size_t r1) {
+        for (size_t i = r0; i < r1; i++) {
+            double *x = a.row(i);
+            for (size_t j = 0; j < n; j++) {
+                double v = x[j];
+                for (size_t k = 0; k < j; k++) v -= x[k] * r.at(k, j);
+                x[j] = v / r.at(j, j);
+            }
+        }
+    });
+}
+
+// |r|_F |r^-1|_F, an upper bound of the 2-norm condition number of r.
+static double condition_bound(const Mat &r) {
+    const double lo = min_singular_bound(r);
+    return lo > 0.0 ? frobenius(r) / lo : INFINITY;
+}
+
+// Thin QR by Cholesky QR twice: a = q r when a is well conditioned (the first
+// factor's condition bound at most 1e6, where two passes leave q orthonormal to
+// working precision); false otherwise. Four parallel sweeps over a in place of a
+// Householder QR's two per column.
+static bool cholesky_qr2(const Mat &a, Mat &q, Mat &r) {
+    Mat r1, r2;
+    if (!cholesky_upper(gram(a), r1) || condition_bound(r1) > 1e6) return false;
+    q = a;
+    solve_right_upper(q, r1);
+    if (!cholesky_upper(gram(q), r2)) return false;
+    solve_right_upper(q, r2);
+    const size_t n = r1.rows;
+    r = Mat(n, n);
+    for (size_t i = 0; i < n; i++)
+        for (size_t j = i; j < n; j++) {
+            double v = 0.0;
+            for (size_t k = i; k <= j; k++) v += r2.at(i, k) * r1.at(k, j);
+            r.at(i, j) = v;
+        }
+    return true;
+}
+
 Mat extend_basis(const Mat &basis, const Mat &block) {
     size_t rows = block.rows;
     double scale = frobenius(block);
@@ -365,7 +454,9 @@ Mat extend_basis(const Mat &basis, const Mat &block) {
     Mat q, r, ur;
     std::vector<double> s;
     KY_CHECK(rows >= n, KY_E505, "pca: a Krylov block wider than the genes (internal)");
-    householder_qr(rest, q, r);
+    // A well-conditioned block (every Krylov block of a PCA here) by Cholesky QR twice,
+    // any other by Householder.
+    if (!cholesky_qr2(rest, q, r)) householder_qr(rest, q, r);
     // Full rank (every singular value of R above the drop tolerance) keeps the whole
     // block, and the new columns may then be Q itself: span(Q U) = span(Q) for a square
     // orthogonal U, and the passes and the Rayleigh-Ritz step depend on the basis only
