@@ -61,6 +61,24 @@ def test_extend_basis_against_oracle():
     assert D.dense_extend_basis(np.zeros((3, 0)), blk).shape[1] == 2
 
 
+@pytest.mark.parametrize("spread", [1.0, 1e-8])
+def test_extend_basis_well_and_ill_conditioned(spread):
+    # A full-rank block: orthonormalised by Cholesky QR twice when well conditioned,
+    # by Householder when a column is 1e-8 of the others (condition ~1e8, still above
+    # the 1e-10 drop tolerance, so every column is kept).
+    rng = np.random.default_rng(7)
+    basis = np.linalg.qr(rng.standard_normal((500, 40)))[0]
+    block = rng.standard_normal((500, 30))
+    block[:, 7] *= spread
+    mine = D.dense_extend_basis(basis, block)
+    ref = oracle.extend_basis(basis, block)
+    assert mine.shape == ref.shape == (500, 30)
+    assert np.abs(mine.T @ mine - np.eye(30)).max() < 1e-12
+    assert np.abs(basis.T @ mine).max() < 1e-12
+    s = np.linalg.svd(mine.T @ ref, compute_uv=False)
+    assert np.abs(s - 1.0).max() < 1e-10
+
+
 def test_rayleigh_ritz_against_oracle():
     rng = np.random.default_rng(1)
     a = rng.standard_normal((200, 200))
