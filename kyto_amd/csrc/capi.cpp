@@ -391,11 +391,9 @@ int ky_graph_download(const ky_graph *g, uint64_t *indptr, uint32_t *indices, fl
         need(indices, "indices");
         need(weights, "weights");
         use_device(*g->ctx);
-        download(*g->ctx, indptr, g->indptr.ptr, sizeof(uint64_t) * (g->n + 1));
-        if (g->nnz) {
-            download(*g->ctx, indices, g->indices.ptr, sizeof(uint32_t) * g->nnz);
-            download(*g->ctx, weights, g->weights.ptr, sizeof(float) * g->nnz);
-        }
+        download_parts(*g->ctx, {{indptr, g->indptr.ptr, sizeof(uint64_t) * (g->n + 1)},
+                                 {indices, g->indices.ptr, sizeof(uint32_t) * g->nnz},
+                                 {weights, g->weights.ptr, sizeof(float) * g->nnz}});
     });
 }
 

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -99,6 +100,17 @@ struct Context {
 // Device-to-host copy of `bytes` into pageable memory through the pinned staging
 // buffer; returns once the data is in dst.
 void download(Context &ctx, void *dst, const void *src, size_t bytes);
+
+// One device-to-host copy of several arrays, pipelined as one stream of chunks.
+struct CopyPart {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+void download_parts(Context &ctx, const std::vector<CopyPart> &parts);
+
+// f(t) for t in [0, tasks) on the persistent host worker pool (dense.cpp).
+void host_parallel(size_t tasks, const std::function<void(size_t)> &f);
 
 struct Csr {
     Context *ctx = nullptr;

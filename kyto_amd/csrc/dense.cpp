@@ -100,6 +100,8 @@ class Pool {
     std::condition_variable sleep_cv_;
 };
 
+void host_parallel(size_t tasks, const std::function<void(size_t)> &f) { Pool::get().run(tasks, f); }
+
 // f(begin, end) over [0, n) in contiguous ranges, one a worker (serially below 64).
 template <typename F>
 static void parallel_for(size_t n, F &&f) {

@@ -779,8 +779,7 @@ void neighbors_run(Context &ctx, const Dense &points, uint32_t n_components, uin
     DevBuf<uint32_t> g_idx(n * k);
     DevBuf<float> g_dist(n * k);
     knn_graph(ctx, keys.ptr, rows.ptr, n, k, g_idx.ptr, g_dist.ptr);
-    download(ctx, h_idx, g_idx.ptr, sizeof(uint32_t) * n * k);
-    download(ctx, h_dist, g_dist.ptr, sizeof(float) * n * k);
+    download_parts(ctx, {{h_idx, g_idx.ptr, sizeof(uint32_t) * n * k}, {h_dist, g_dist.ptr, sizeof(float) * n * k}});
     if (info) info->total_ms = now_ms() - t_start;
 }
 

@@ -7,9 +7,11 @@
 //   are reused only by the thread that freed them: a context is used by one thread at
 //   a time, every call runs on its context's one stream and synchronises it before it
 //   returns, so a reused block's earlier work is ordered before its new work.
-// - Downloads through a pinned staging buffer of the context: the device-to-host copy
-//   of one chunk overlaps the host copy of the chunk before it (several threads), in
-//   place of a copy into pageable memory, which the runtime stages at a few GB/s.
+// - Downloads through a pinned staging buffer of the context: the arrays of one
+//   download form one stream of 8 MB chunks, and the device-to-host copy of a chunk
+//   overlaps the host copy of the chunk before it (8 slices on the persistent host
+//   pool), in place of a copy into pageable memory, which the runtime stages at a few
+//   GB/s.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -122,11 +124,16 @@ void device_cache_trim() {
 // Copies `bytes` from device memory to pageable host memory through the context's
 // pinned staging buffer, CHUNK bytes at a time: the device copy of chunk i + 1 runs
 // while host threads copy chunk i out of the other half of the buffer.
-void download(Context &ctx, void *dst, const void *src, size_t bytes) {
-    constexpr size_t SMALL = 1ull << 20, CHUNK = 32ull << 20;
+void download(Context &ctx, void *dst, const void *src, size_t bytes) { download_parts(ctx, {{dst, src, bytes}}); }
+
+void download_parts(Context &ctx, const std::vector<CopyPart> &parts) {
+    constexpr size_t SMALL = 1ull << 20, CHUNK = 8ull << 20, SLICES = 8;
     hipStream_t st = ctx.stream;
-    if (bytes <= SMALL) {
-        KY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    size_t total = 0;
+    for (const CopyPart &p : parts) total += p.bytes;
+    if (total <= SMALL) {
+        for (const CopyPart &p : parts)
+            if (p.bytes) KY_HIP(hipMemcpyAsync(p.dst, p.src, p.bytes, hipMemcpyDeviceToHost, st));
         KY_HIP(hipStreamSynchronize(st));
         return;
     }
@@ -135,34 +142,35 @@ void download(Context &ctx, void *dst, const void *src, size_t bytes) {
         KY_HIP(hipEventCreateWithFlags(&ctx.staged[0], hipEventDisableTiming));
         KY_HIP(hipEventCreateWithFlags(&ctx.staged[1], hipEventDisableTiming));
     }
-    auto *out = static_cast<unsigned char *>(dst);
-    const auto *in = static_cast<const unsigned char *>(src);
+    // The chunks of every part in order: chunk i goes to staging half i % 2; its host
+    // copy (SLICES slices on the host pool) runs while chunk i + 1 is in flight.
+    struct Chunk {
+        unsigned char *dst;
+        const unsigned char *src;
+        size_t len;
+    };
+    std::vector<Chunk> chunks;
+    for (const CopyPart &p : parts)
+        for (size_t off = 0; off < p.bytes; off += CHUNK)
+            chunks.push_back({static_cast<unsigned char *>(p.dst) + off, static_cast<const unsigned char *>(p.src) + off,
+                              std::min(CHUNK, p.bytes - off)});
     auto *stage = static_cast<unsigned char *>(ctx.staging);
-    const size_t chunks = (bytes + CHUNK - 1) / CHUNK;
     auto copy_out = [&](size_t i) {
-        const size_t off = i * CHUNK, len = std::min(CHUNK, bytes - off);
+        const Chunk &c = chunks[i];
         KY_HIP(hipEventSynchronize(ctx.staged[i % 2]));
         const unsigned char *from = stage + (i % 2) * CHUNK;
-        const unsigned workers = std::max(1u, std::min(8u, (unsigned)(len >> 22)));
-        if (workers == 1) {
-            std::memcpy(out + off, from, len);
-            return;
-        }
-        std::vector<std::thread> th;
-        const size_t per = (len + workers - 1) / workers;
-        for (unsigned w = 0; w < workers; w++) {
-            const size_t a = w * per, b = std::min(len, a + per);
-            if (a < b) th.emplace_back([=] { std::memcpy(out + off + a, from + a, b - a); });
-        }
-        for (auto &t : th) t.join();
+        const size_t per = (c.len + SLICES - 1) / SLICES;
+        host_parallel(SLICES, [&](size_t t) {
+            const size_t a = t * per, b = std::min(c.len, a + per);
+            if (a < b) std::memcpy(c.dst + a, from + a, b - a);
+        });
     };
-    for (size_t i = 0; i < chunks; i++) {
-        const size_t off = i * CHUNK, len = std::min(CHUNK, bytes - off);
-        KY_HIP(hipMemcpyAsync(stage + (i % 2) * CHUNK, in + off, len, hipMemcpyDeviceToHost, st));
+    for (size_t i = 0; i < chunks.size(); i++) {
+        KY_HIP(hipMemcpyAsync(stage + (i % 2) * CHUNK, chunks[i].src, chunks[i].len, hipMemcpyDeviceToHost, st));
         KY_HIP(hipEventRecord(ctx.staged[i % 2], st));
         if (i > 0) copy_out(i - 1);
     }
-    copy_out(chunks - 1);
+    copy_out(chunks.size() - 1);
 }
 
 }  // namespace ky

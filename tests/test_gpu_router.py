@@ -112,3 +112,26 @@ def test_learned_ivf_recall_and_gain(device):
             index.free()
     finally:
         m.free()
+
+
+def test_exact_fallback_of_a_row_list_above_a_device_round(device):
+    # 250,000 rows all sent to the exact search as a row list (confidence above 1): 977
+    # query blocks against 768 resident, so the listed rows' last round is split over
+    # point segments; the graph must be the full search's, bit for bit.
+    rng = np.random.default_rng(31)
+    centres = rng.standard_normal((24, 50)).astype(np.float32) * 3
+    pts = (centres[rng.integers(0, 24, 250_000)] + rng.standard_normal((250_000, 50)).astype(np.float32))
+    m = device.upload_dense(pts.astype(np.float32))
+    try:
+        full = device.neighbors(m, 15, precision="bf16")
+        index = device.ivf_build(m, n_lists=256, kmeans_iters=3, seed=0, hidden=64, train_queries=8192,
+                                 batch=1024, epochs=1, lr=1e-3, label_k=14)
+        try:
+            g = index.search(m, 15, n_probe=8, use_router=True, confidence=1.01)
+            assert g.info["fallback"] == 250_000
+            assert np.array_equal(g.rows(), full.rows())
+            assert np.array_equal(g.distances.view(np.uint32), full.distances.view(np.uint32))
+        finally:
+            index.free()
+    finally:
+        m.free()
