@@ -253,8 +253,7 @@ int ky_dense_upload(ky_ctx *ctx, const float *host, uint64_t rows, uint32_t cols
             m->rows = rows;
             m->cols = cols;
             m->data.alloc(rows * cols);
-            KY_HIP(hipMemcpyAsync(m->data.ptr, host, sizeof(float) * rows * cols, hipMemcpyHostToDevice, ctx->stream));
-            KY_HIP(hipStreamSynchronize(ctx->stream));
+            upload_parts(*ctx, {{m->data.ptr, host, sizeof(float) * rows * cols}});
         } catch (...) {
             delete m;
             throw;

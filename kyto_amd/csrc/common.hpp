@@ -108,6 +108,9 @@ struct CopyPart {
     size_t bytes;
 };
 void download_parts(Context &ctx, const std::vector<CopyPart> &parts);
+// Host-to-device copies of several arrays (dst on the device, src in pageable host
+// memory) through the same staging buffer; returns once the data is on the device.
+void upload_parts(Context &ctx, const std::vector<CopyPart> &parts);
 
 // f(t) for t in [0, tasks) on the persistent host worker pool (dense.cpp).
 void host_parallel(size_t tasks, const std::function<void(size_t)> &f);

@@ -254,9 +254,7 @@ void connectivities_run(Context &ctx, uint64_t n, uint32_t n_neighbors, const ui
     hipStream_t st = ctx.stream;
     DevBuf<uint32_t> idx(ne);
     DevBuf<float> dist(ne);
-    KY_HIP(hipMemcpyAsync(idx.ptr, h_indices, sizeof(uint32_t) * ne, hipMemcpyHostToDevice, st));
-    KY_HIP(hipMemcpyAsync(dist.ptr, h_dist, sizeof(float) * ne, hipMemcpyHostToDevice, st));
-    KY_HIP(hipStreamSynchronize(st));
+    upload_parts(ctx, {{idx.ptr, h_indices, sizeof(uint32_t) * ne}, {dist.ptr, h_dist, sizeof(float) * ne}});
     const double t_up = now_ms();
 
     DevBuf<float> ranked(ne), w(ne), sigma(n), rho(n);

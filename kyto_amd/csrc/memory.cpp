@@ -126,8 +126,54 @@ void device_cache_trim() {
 // while host threads copy chunk i out of the other half of the buffer.
 void download(Context &ctx, void *dst, const void *src, size_t bytes) { download_parts(ctx, {{dst, src, bytes}}); }
 
+namespace {
+constexpr size_t STAGE_CHUNK = 8ull << 20;
+constexpr size_t STAGE_SLICES = 8;
+
+void ensure_staging(Context &ctx) {
+    if (ctx.staging) return;
+    KY_HIP(hipHostMalloc(&ctx.staging, 2 * STAGE_CHUNK, hipHostMallocDefault));
+    KY_HIP(hipEventCreateWithFlags(&ctx.staged[0], hipEventDisableTiming));
+    KY_HIP(hipEventCreateWithFlags(&ctx.staged[1], hipEventDisableTiming));
+}
+}  // namespace
+
+void upload_parts(Context &ctx, const std::vector<CopyPart> &parts) {
+    constexpr size_t SMALL = 1ull << 20;
+    hipStream_t st = ctx.stream;
+    size_t total = 0;
+    for (const CopyPart &p : parts) total += p.bytes;
+    if (total <= SMALL) {
+        for (const CopyPart &p : parts)
+            if (p.bytes) KY_HIP(hipMemcpyAsync(p.dst, p.src, p.bytes, hipMemcpyHostToDevice, st));
+        KY_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    ensure_staging(ctx);
+    auto *stage = static_cast<unsigned char *>(ctx.staging);
+    // Chunk i is copied into staging half i % 2 (once that half's previous transfer has
+    // ended) while chunk i - 1 is in flight.
+    size_t i = 0;
+    for (const CopyPart &p : parts) {
+        for (size_t off = 0; off < p.bytes; off += STAGE_CHUNK, i++) {
+            const size_t len = std::min(STAGE_CHUNK, p.bytes - off);
+            if (i >= 2) KY_HIP(hipEventSynchronize(ctx.staged[i % 2]));
+            unsigned char *to = stage + (i % 2) * STAGE_CHUNK;
+            const unsigned char *from = static_cast<const unsigned char *>(p.src) + off;
+            const size_t per = (len + STAGE_SLICES - 1) / STAGE_SLICES;
+            host_parallel(STAGE_SLICES, [&](size_t t) {
+                const size_t a = t * per, b = std::min(len, a + per);
+                if (a < b) std::memcpy(to + a, from + a, b - a);
+            });
+            KY_HIP(hipMemcpyAsync(static_cast<unsigned char *>(p.dst) + off, to, len, hipMemcpyHostToDevice, st));
+            KY_HIP(hipEventRecord(ctx.staged[i % 2], st));
+        }
+    }
+    KY_HIP(hipStreamSynchronize(st));
+}
+
 void download_parts(Context &ctx, const std::vector<CopyPart> &parts) {
-    constexpr size_t SMALL = 1ull << 20, CHUNK = 8ull << 20, SLICES = 8;
+    constexpr size_t SMALL = 1ull << 20, CHUNK = STAGE_CHUNK, SLICES = STAGE_SLICES;
     hipStream_t st = ctx.stream;
     size_t total = 0;
     for (const CopyPart &p : parts) total += p.bytes;
@@ -137,11 +183,7 @@ void download_parts(Context &ctx, const std::vector<CopyPart> &parts) {
         KY_HIP(hipStreamSynchronize(st));
         return;
     }
-    if (!ctx.staging) {
-        KY_HIP(hipHostMalloc(&ctx.staging, 2 * CHUNK, hipHostMallocDefault));
-        KY_HIP(hipEventCreateWithFlags(&ctx.staged[0], hipEventDisableTiming));
-        KY_HIP(hipEventCreateWithFlags(&ctx.staged[1], hipEventDisableTiming));
-    }
+    ensure_staging(ctx);
     // The chunks of every part in order: chunk i goes to staging half i % 2; its host
     // copy (SLICES slices on the host pool) runs while chunk i + 1 is in flight.
     struct Chunk {

@@ -447,12 +447,9 @@ void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const 
     csr.indptr.alloc(n + 1);
     csr.indices.alloc(std::max<uint64_t>(csr.nnz, 1));
     csr.values.alloc(std::max<uint64_t>(csr.nnz, 1));
-    hipStream_t st = ctx.stream;
-    KY_HIP(hipMemcpyAsync(csr.indptr.ptr, indptr, sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, st));
-    if (csr.nnz) {
-        KY_HIP(hipMemcpyAsync(csr.indices.ptr, indices, sizeof(uint32_t) * csr.nnz, hipMemcpyHostToDevice, st));
-        KY_HIP(hipMemcpyAsync(csr.values.ptr, values, sizeof(float) * csr.nnz, hipMemcpyHostToDevice, st));
-    }
+    upload_parts(ctx, {{csr.indptr.ptr, indptr, sizeof(uint64_t) * (n + 1)},
+                       {csr.indices.ptr, indices, sizeof(uint32_t) * csr.nnz},
+                       {csr.values.ptr, values, sizeof(float) * csr.nnz}});
     csr_finish(csr, maps);
 }
 
