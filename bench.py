@@ -15,7 +15,7 @@ the learned router's own recall and time, the nearest-centroid probe it is compa
 with, and the plain FP32 kNN stage on the same embedding.
 
     python bench.py [--steps K] [--warmup W] [--precision bf16|fp32] [--q 4]
-                    [--workload 1m|4m|100k] [--no-compare] [--no-cpu-baseline] [--copy-calibration]
+                    [--workload 1m|4m|100k] [--compare] [--no-compare] [--no-cpu-baseline] [--copy-calibration]
 
 --workload 4m is the streamed form: 4,000,000 x 3,000 counts (40 types, ~7%) read in
 32,768-cell blocks through the pinned double-buffered reader, the layer made on the
@@ -119,10 +119,10 @@ def learned_leg(dev, emb, k, ref_rows, args):
         "build_ms": build_ms, "kmeans_ms": index.info["kmeans_ms"], "labels_ms": index.info["labels_ms"],
         "train_ms": index.info["train_ms"], "final_loss": index.info["final_loss"],
         "label_mass_in_top_8": index.info["label_recall_top"],
-        "router_search_ms": g.info["total_ms"], "router_recall_at_15": recall_of(g.rows(), ref_rows),
-        "centroid_search_ms": c.info["total_ms"], "centroid_recall_at_15": recall_of(c.rows(), ref_rows),
+        "router_search_ms": g.info["total_ms"], f"router_recall_at_{k}": recall_of(g.rows(), ref_rows),
+        "centroid_search_ms": c.info["total_ms"], f"centroid_recall_at_{k}": recall_of(c.rows(), ref_rows),
         "confidence_gate": args.confidence, "gated_rows_to_exact": int(gated.info["fallback"]),
-        "gated_recall_at_15": recall_of(gated.rows(), ref_rows), "gated_search_ms": gated.info["total_ms"],
+        f"gated_recall_at_{k}": recall_of(gated.rows(), ref_rows), "gated_search_ms": gated.info["total_ms"],
     }
 
 
@@ -140,6 +140,8 @@ def main():
     ap.add_argument("--train-queries", type=int, default=200_000)
     ap.add_argument("--confidence", type=float, default=0.99,
                     help="router confidence (softmax mass of the probed lists) below which a row is searched exactly")
+    ap.add_argument("--compare", action="store_true",
+                    help="run the comparison legs at 4m too (plain FP32 kNN stage, learned leg: minutes)")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the learned leg, the recall check and the plain FP32 kNN stage")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -186,7 +188,7 @@ def main():
     k = 30 if args.workload == "4m" else 15
     if args.workload == "4m":
         args.n_lists = 8192
-        args.no_compare = True
+        args.no_compare = not args.compare
 
     def step():
         res = dev.pca(csr, 50, seed=0, power_iterations=args.q, download_embedding=False)
@@ -279,14 +281,14 @@ def main():
         exact = plain.rows()
         same = bool(np.array_equal(g.rows(), exact) and np.array_equal(g.distances, plain.distances))
         knn_ms = float(np.mean([x["total_ms"] for x in knns]))
-        out["recall_at_15"] = recall_of(g.rows(), exact)
-        out["recall_definition"] = "share of the exact graph's 14 stored neighbours (self excluded) found"
+        out[f"recall_at_{k}"] = recall_of(g.rows(), exact)
+        out["recall_definition"] = f"share of the exact graph's {k - 1} stored neighbours (self excluded) found"
         out["graph_equal_to_plain_fp32"] = same
         out["plain_fp32_knn_ms"] = plain.info["total_ms"]
         out["knn_ms"] = knn_ms
         out["knn_speedup_vs_plain_fp32"] = plain.info["total_ms"] / knn_ms
         out["learned"] = learned_leg(dev, emb, k, exact, args)
-        log(f"recall@15 {out['recall_at_15']:.5f} (graph equal to FP32: {same}); kNN {knn_ms:.1f} ms vs plain FP32 "
+        log(f"recall@{k} {out[f'recall_at_{k}']:.5f} (graph equal to FP32: {same}); kNN {knn_ms:.1f} ms vs plain FP32 "
             f"{plain.info['total_ms']:.1f} ms ({out['knn_speedup_vs_plain_fp32']:.1f}x); learned {out['learned']}")
     # Outside the timed region: the connectivities graph the neighbors step declares
     # (UMAP's fuzzy weights on the last step's graph), the second of two calls timed.
