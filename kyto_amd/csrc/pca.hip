@@ -612,7 +612,9 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
     const size_t stride = (size_t)csr.n_cols * w;
     ctx.partials.reserve(stride * csr.n_chunks * 2);
     ctx.reduced.reserve(stride + w);
-    const uint32_t gpb = 16;
+    // 8 genes a block: the blocks resident on an XCD work on one or two chunks at a
+    // time, whose W rows (1 MB a chunk) stay in its L2.
+    const uint32_t gpb = 8;
     const unsigned groups = div_up(csr.n_cols, gpb);
     const unsigned slots = div_up(csr.n_chunks, 8);
     const unsigned blocks = 8 * slots * groups;
