@@ -16,6 +16,10 @@ with, and the plain FP32 kNN stage on the same embedding.
 
     python bench.py [--steps K] [--warmup W] [--precision bf16|fp32] [--q 4]
                     [--workload 1m|4m|100k] [--compare] [--no-compare] [--no-cpu-baseline] [--copy-calibration]
+                    [--no-store]
+
+Outside the timed region (1m, 100k) the resident layer is also written as a kyto store
+under /tmp and read back onto the device through the store reader (the "store" field).
 
 --workload 4m is the streamed form: 4,000,000 x 3,000 counts (40 types, ~7%) read in
 32,768-cell blocks through the pinned double-buffered reader, the layer made on the
@@ -126,6 +130,47 @@ def learned_leg(dev, emb, k, ref_rows, args):
     }
 
 
+def store_leg(dev, layer, csr, n_genes):
+    """Outside the timed region: the resident layer written as a kyto store (tiles of
+    4,096 cells x 1,024 genes, inner chunks of 65,536 values, shards of 2^20, zstd 3)
+    under /tmp, then read back tile by tile into the pinned buffers and onto the
+    device (ky_csr_stream_layer), its values compared with the resident layer's. The
+    files are in the page cache when read: this times decoding and transfer, not the
+    disk."""
+    import shutil
+    import tempfile
+    from kyto_amd import store as ks
+    tmp = tempfile.mkdtemp(prefix="kyto-store-", dir="/tmp")
+    try:
+        if shutil.disk_usage(tmp).free < 4 * layer.values.nbytes:
+            return {"skipped": "not enough free space under /tmp"}
+        path = os.path.join(tmp, "layer.kyto")
+        t0 = time.perf_counter()
+        ks.create_store(path, n_cells=len(layer.indptr) - 1, n_genes=n_genes, cell_block=4096, gene_block=1024,
+                        layers={"log1p": ks.LayerData(layer.indptr, layer.indices, layer.values)})
+        write_ms = (time.perf_counter() - t0) * 1e3
+        disk = sum(os.path.getsize(os.path.join(r, f)) for r, _, fs in os.walk(os.path.join(path, "layers")) for f in fs)
+        infos, equal = [], False
+        for _ in range(2):  # the second read is reported (the first warms the reader's buffers)
+            with ks.StoreLayer(path, "log1p") as opened:
+                t0 = time.perf_counter()
+                made = dev.stream_layer(opened)
+                wall = (time.perf_counter() - t0) * 1e3
+            if infos:
+                equal = bool(np.array_equal(made.values().view(np.uint32), csr.values().view(np.uint32)))
+            infos.append(dict(made.stream_info, wall_ms=wall))
+            made.free()
+        info = infos[-1]
+        decoded = float(info["bytes"])
+        return {"write_ms": write_ms, "bytes_on_disk": disk, "decoded_bytes": decoded, "blocks": int(info["blocks"]),
+                "read_ms": info["read_ms"], "stream_ms": info["total_ms"], "wall_ms": info["wall_ms"],
+                "decode_GBps": decoded / (info["read_ms"] / 1e3) / 1e9,
+                "stream_GBps": decoded / (info["total_ms"] / 1e3) / 1e9, "equal_to_resident_layer": equal,
+                "tiling": "4096 cells x 1024 genes, K 65536, S 1048576, zstd level 3"}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +190,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the learned leg, the recall check and the plain FP32 kNN stage")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-store", action="store_true", help="skip the store leg (write the layer as a kyto store, "
+                                                            "read it back onto the device)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one device-to-device copy of 256 MiB (the HBM counters' reference kernel)")
     args = ap.parse_args()
@@ -299,6 +346,9 @@ def main():
                              "union_ms": conn.info["union_ms"], "zero_rho_rows": int(conn.info["zero_rho_rows"])}
     log(f"connectivities: {conn.info['edges']} edges in {conn.info['total_ms']:.1f} ms "
         f"(upload {conn.info['upload_ms']:.1f}, scales {conn.info['scales_ms']:.1f}, union {conn.info['union_ms']:.1f})")
+    if layer is not None and not args.no_store:
+        out["store"] = store_leg(dev, layer, csr, w.n_genes)
+        log(f"store: {out['store']}")
     if not args.no_cpu_baseline:
         import oracle
         t0 = time.perf_counter()

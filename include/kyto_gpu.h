@@ -24,6 +24,18 @@ extern "C" {
 #endif
 
 #define KY_OK 0
+/* Storage errors of the store reader and writers (errors.toml:108-445). */
+#define KY_E102 102 /* a member name breaks format.md section 2.1 */
+#define KY_E103 103 /* a member of that name (case-insensitive) already exists */
+#define KY_E104 104 /* a writer parameter (chunk, shard, zstd level) is out of range */
+#define KY_E105 105 /* a size is above a limit of the format (section 4.5) */
+#define KY_E106 106 /* a CSR given to the layer writer is not valid (or stores a zero) */
+#define KY_E107 107 /* a stored layer breaks the tiled sparse format (section 4) */
+#define KY_E109 109 /* no layer, embedding or graph of that name in the store */
+#define KY_E110 110 /* the file system failed, or a Zarr node cannot be read */
+#define KY_E115 115 /* the store's format_version is not one this library reads */
+#define KY_E116 116 /* the path is not a kyto store */
+#define KY_E126 126 /* a stored embedding or graph breaks sections 6 and 7 */
 #define KY_E215 215 /* invalid argument or data (KytoError::InvalidArgument) */
 #define KY_E501 501 /* no kernel for the placed step */
 #define KY_E503 503 /* cancelled */
@@ -42,6 +54,7 @@ typedef struct ky_csr ky_csr;     /* a CSR layer resident in device memory */
 typedef struct ky_dense ky_dense; /* a row-major f32 matrix resident in device memory */
 typedef struct ky_ivf ky_ivf;     /* an IVF index (k-means lists + router) on the device */
 typedef struct ky_graph ky_graph; /* a weighted cell graph (CSR) resident in device memory */
+typedef struct ky_layer ky_layer; /* a tiled layer of a kyto store on disk, open for reading */
 
 /* The calling thread's last error message ("" if none). Replaces the
  * KytoError::DeviceFailure payload (launch.rs:62-69). */
@@ -108,6 +121,76 @@ int ky_csr_stream(ky_ctx *ctx, uint64_t n_rows, uint32_t n_cols, uint64_t block_
                   ky_stream_info *info);
 int ky_csr_free(ky_csr *csr);
 int ky_csr_shape(const ky_csr *csr, uint64_t *n_rows, uint32_t *n_cols, uint64_t *nnz);
+
+/* ---- kyto stores on disk (the reference's docs/format.md, store format 0.1) ---- */
+
+/* Open layer `layer` of the kyto store at directory `store` for reading (replaces
+ * kyto_store's TileReader::open, crates/kyto-store/src/tile_reader.rs:94-108, and
+ * layer_open.rs): the root's format_version (KY_E116 not a store, KY_E115 another
+ * version), schema.json, the four Zarr v3 arrays of the layer checked against it
+ * (shapes, dtypes, sharding; KY_E107), tile_offsets read and checked (from 0,
+ * non-decreasing, ending at nnz), the shard indexes of indices and data read (crc32c
+ * checked). KY_E109 if the store has no such layer. */
+int ky_store_layer_open(const char *store, const char *layer, ky_layer **out);
+/* The layer's axes, stored values and tiling; `counts` is 1 for uint32 values. */
+int ky_store_layer_shape(const ky_layer *layer, uint64_t *n_cells, uint32_t *n_genes, uint64_t *nnz,
+                         uint64_t *cell_block, uint64_t *gene_block, int *counts);
+/* A ky_block_reader over the layer (user = the ky_layer): rows [first_row,
+ * first_row + n_rows) as a CSR of their own, decoded from the zstd inner chunks of
+ * their cell blocks' tiles, gene blocks in increasing order, global gene indices
+ * (format.md sections 4.7 and 4.8; tile_decode.rs:74-130), values as f32 (uint32
+ * counts converted), each tile's indptr and local indices checked (KY_E107). With one
+ * gene block and whole cell blocks the chunks decode straight into the caller's
+ * (pinned) buffers. Returns the number of values, or -1 with ky_last_error() set. */
+int64_t ky_store_layer_read(void *layer, uint64_t first_row, uint64_t n_rows, uint64_t *indptr, uint32_t *indices,
+                            float *values, uint64_t capacity);
+/* The same read with a status (the store's error codes kept): *nnz is the rows'
+ * number of values; above `capacity` only indptr is written. */
+int ky_store_layer_rows(ky_layer *layer, uint64_t first_row, uint64_t n_rows, uint64_t *indptr, uint32_t *indices,
+                        float *values, uint64_t capacity, uint64_t *nnz);
+int ky_store_layer_close(ky_layer *layer);
+/* ky_csr_stream over a store layer (kyto-exec's block stream over a store,
+ * crates/kyto-exec/src/pipeline.rs with layer_stream.rs): block_rows cells a block
+ * (0: 8 cell blocks), the store's own error codes kept. */
+int ky_csr_stream_layer(ky_ctx *ctx, ky_layer *layer, uint64_t block_rows, double target_sum, int log1p,
+                        ky_csr **out, ky_stream_info *info);
+
+/* Write layer `name` (a CSR of n_cells x n_genes: indptr n_cells + 1, global gene
+ * indices strictly increasing in a row, no stored zeros; values float32, or uint32
+ * counts when `counts` is 1) into `<store>/layers/` as tiles of cell_block x
+ * gene_block (format.md section 4; kyto_store's tile_encode.rs and the layer arrays of
+ * zarr_arrays.rs:97-140): per-tile indptr sharded a cell block, tile_offsets, and
+ * indices / data sharded (nnz_shard entries a shard, nnz_chunk an inner chunk; zstd,
+ * crc32c index). KY_E104 / KY_E105 for sizes out of range, KY_E106 for an invalid CSR
+ * or a stored zero. */
+int ky_store_write_layer(const char *store, const char *name, uint64_t n_cells, uint64_t n_genes, const uint64_t *indptr,
+                         const uint32_t *indices, const void *values, int counts, uint64_t cell_block,
+                         uint64_t gene_block, uint64_t nnz_chunk, uint64_t nnz_shard, int zstd_level);
+/* Write embedding `name` (rows x width float32, row-major; axis 0 = cells, 1 = genes)
+ * into `<store>/embeddings/` as format.md section 6 gives it (embedding_io.rs:194-240):
+ * chunks of [chunk_rows, width], bytes + zstd(zstd_level), fill value 0, chunks of
+ * nothing but zeros not stored. KY_E102 for a bad name, KY_E103 if the name exists,
+ * KY_E104 for a level outside 1..22, KY_E105 for a chunk above 2^31 bytes. The caller
+ * lists the member in schema.json (a sink writes schema.json and the root last). */
+int ky_store_write_embedding(const char *store, const char *name, int axis, const float *values, uint64_t rows,
+                             uint32_t width, uint64_t chunk_rows, int zstd_level);
+/* Read a float32 embedding whole (embedding_io.rs read_embedding): its shape from
+ * schema.json, checked against the array (KY_E126); values may be null to ask the
+ * shape. */
+int ky_store_read_embedding(const char *store, const char *name, uint64_t *rows, uint32_t *width, float *values,
+                            uint64_t capacity);
+/* Write cell graph `name` (CSR: indptr n_cells + 1, indices strictly increasing in a
+ * row and below n_cells, float32 weights) into `<store>/graphs/` as format.md section 7
+ * gives it (graph_io.rs:189-243): indptr uint64 in unsharded chunks of 65,536, indices
+ * and data sharded (nnz_shard entries a shard, nnz_chunk an inner chunk, zstd, crc32c
+ * index at the end). KY_E215 for an invalid CSR (Graph::check, graph_io.rs:71-105). */
+int ky_store_write_graph(const char *store, const char *name, uint64_t n_cells, const uint64_t *indptr,
+                         const uint32_t *indices, const float *weights, uint64_t nnz_chunk, uint64_t nnz_shard,
+                         int zstd_level);
+/* Read a float32 cell graph whole (graph_io.rs read_graph, :119-187): indptr may be
+ * null to ask the shape; KY_E126 if it breaks section 7 or disagrees with schema.json. */
+int ky_store_read_graph(const char *store, const char *name, uint64_t *n_cells, uint64_t *edges, uint64_t *indptr,
+                        uint32_t *indices, float *weights);
 
 /* Device-resident dense matrices (the PCA embedding, X_pca). */
 int ky_dense_upload(ky_ctx *ctx, const float *host, uint64_t rows, uint32_t cols, ky_dense **out);

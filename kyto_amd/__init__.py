@@ -10,11 +10,17 @@ kyto's plan API (crates/kyto-py/src/atlas.rs:822-966, 1622-1720).
              .collect(memory_limit="32GB"))     # atlas.rs:1691-1720
     out["X_pca"], out["distances"], out["connectivities"]
 
+A kyto store on disk works the same way: scan("pbmc.kyto") reads its schema, pca()
+streams the layer's tiles through the pinned buffers (kyto_amd.store), and
+sink_zarr(path) writes the outputs as a new store (X_pca, X_pca_loadings, the
+distances and connectivities graphs, the layers).
+
 Chained methods only build a plan; collect() runs it on the device. There is no CPU
 path: on(device="cpu") is refused with KY-E501 and a missing native library raises.
 """
 from __future__ import annotations
 
+import os
 import re
 from dataclasses import dataclass, field, replace
 from typing import Any
@@ -22,6 +28,7 @@ from typing import Any
 import numpy as np
 
 from ._native import KytoError, KY_E215, KY_E501
+from . import store as _store
 from .device import ConnectivityGraph, CsrLayer, DenseMatrix, Device, Graph, IvfIndex, PcaOutput, device_count
 
 __all__ = ["scan", "Atlas", "Device", "KytoError", "Graph", "ConnectivityGraph", "PcaOutput", "device_count", "CsrLayer", "DenseMatrix",
@@ -75,6 +82,24 @@ class _Layer:
     @property
     def nnz(self):
         return int(self.indptr[-1])
+
+
+@dataclass(frozen=True)
+class _StoreRef:
+    """A layer of a kyto store on disk, read when a step needs it."""
+    store: str
+    name: str
+    n_cells: int
+    n_genes: int
+    nnz: int
+    dtype: str
+    cell_block: int
+    gene_block: int
+
+    def host(self) -> _Layer:
+        with _store.StoreLayer(self.store, self.name) as layer:
+            ip, ix, v = layer.read_rows()
+        return _Layer(ip, ix, v, self.n_genes)
 
 
 def _as_layer(x) -> _Layer:
@@ -262,6 +287,55 @@ class Atlas:
     def collect(self, *, memory_limit=None, threads=None, progress=None) -> dict:
         """Run the plan on the device; returns a dict of the outputs (embeddings,
         loadings, variances, graphs)."""
+        return self._run(memory_limit, threads, progress, keep_layers=False)
+
+    def sink_zarr(self, path, /, *, overwrite=False, memory_limit=None, threads=None, progress=None,
+                  cell_block=None, gene_block=None, zstd_level=_store.ZSTD_LEVEL) -> None:
+        """Run the plan and write its output as a new kyto store at `path` (the
+        reference's Atlas.sink_zarr, python/kyto/_kyto.pyi:1323; format.md §10): the
+        scanned layers (copied from a scanned store with its tables), the layer a
+        pca read when the maps made it (the last map's), each pca's embedding and `<into>_loadings`, each neighbors step's
+        distances and connectivities graphs. The store is written next to `path` and
+        renamed into place; KY-E118 if `path` exists unless overwrite=True (and then
+        only a kyto store is replaced)."""
+        out = self._run(memory_limit, threads, progress, keep_layers=True)
+        made = out.pop("__made_layers__")
+        first = next(iter(self.layers.values()))
+        n_cells, n_genes = first.n_cells, first.n_genes
+        from_store = isinstance(first, _StoreRef)
+        cb = cell_block or (first.cell_block if from_store else 4096)
+        gb = gene_block or (first.gene_block if from_store else 1024)
+        layers = {}
+        if not from_store:
+            for name, layer in self.layers.items():
+                layers[name] = _store.LayerData(layer.indptr, layer.indices, layer.values)
+        for name, (source, values) in made.items():
+            src = self.layers[source]
+            ip, ix = (src.indptr, src.indices) if not isinstance(src, _StoreRef) else src.host()[:2]
+            keep = values != 0  # a map can make a zero (format.md §4.6: never stored)
+            if not keep.all():
+                rows = np.repeat(np.arange(n_cells), np.diff(np.asarray(ip, np.int64)))[keep]
+                ip = np.zeros(n_cells + 1, np.uint64)
+                np.cumsum(np.bincount(rows, minlength=n_cells), out=ip[1:])
+                ix, values = ix[keep], values[keep]
+            layers[name] = _store.LayerData(ip, ix, values)
+        embeddings, graphs = {}, {}
+        for step in self.steps:
+            if isinstance(step, _Pca):
+                embeddings[step.into] = ("cells", out[step.into])
+                embeddings[f"{step.into}_loadings"] = ("genes", out[f"{step.into}_loadings"])
+            elif isinstance(step, _Neighbors):
+                key, conn = _graph_names(step.into)
+                g, c = out[key], out[conn]
+                graphs[key] = (g.indptr, g.indices, g.distances)
+                graphs[conn] = (c.indptr, c.indices, c.weights)
+        _store.create_store(path, n_cells=n_cells, n_genes=n_genes, cell_block=cb, gene_block=gb, layers=layers,
+                            embeddings=embeddings, graphs=graphs,
+                            tables_from=first.store if from_store else None,
+                            copy_layers=list(self.layers) if from_store else None, overwrite=overwrite,
+                            zstd_level=zstd_level)
+
+    def _run(self, memory_limit, threads, progress, keep_layers) -> dict:
         limit = parse_memory(memory_limit)
         if threads is not None:
             _integer("collect", "threads", threads, 1)
@@ -269,6 +343,7 @@ class Atlas:
             raise KytoError(KY_E501, "this build runs pca and neighbors on the GPU only: on(device='gpu')")
         ordinal = 0 if self.device in ("auto", "gpu", "cuda") else int(self.device)
         out: dict = {}
+        made: dict = {}
         dev = Device(ordinal)
         resident: dict[str, DenseMatrix] = {}
         try:
@@ -279,7 +354,9 @@ class Atlas:
                     source, maps = self._source(step.layer)
                     layer = self.layers[source]
                     _fits(layer, step, limit)
-                    res, emb = _run_pca(dev, layer, step, _device_maps(maps))
+                    res, emb = _run_pca(dev, layer, step, _device_maps(maps), keep_layer=keep_layers)
+                    if keep_layers and res.made_values is not None:
+                        made[step.layer] = (source, res.made_values)
                     resident[step.into] = emb
                     out[step.into] = res.embedding
                     out[f"{step.into}_loadings"] = res.loadings
@@ -301,6 +378,8 @@ class Atlas:
             for m in resident.values():
                 m.free()
             dev.close()
+        if keep_layers:
+            out["__made_layers__"] = made
         return out
 
 
@@ -334,26 +413,37 @@ def _device_maps(maps):
     return target, kinds[-1] == "log1p"
 
 
-def _upload(dev: Device, layer: _Layer, maps):
+def _upload(dev: Device, layer, maps):
+    if isinstance(layer, _StoreRef):
+        target, log1p = maps if maps is not None else (None, False)
+        with _store.StoreLayer(layer.store, layer.name) as opened:
+            return dev.stream_layer(opened, target_sum=target, log1p=log1p)
     if maps is None:
         return dev.upload_csr(layer.indptr, layer.indices, layer.values, layer.n_genes)
     target, log1p = maps
     return dev.upload_counts(layer.indptr, layer.indices, layer.values, layer.n_genes, target_sum=target, log1p=log1p)
 
 
-def _run_pca(dev: Device, layer: _Layer, step: _Pca, maps=None):
+def _run_pca(dev: Device, layer, step: _Pca, maps=None, keep_layer=False):
+    """The PCA of a step; with keep_layer, the made layer's values (the maps' output)
+    come back as res.made_values."""
     if step.genes is None:
         csr = _upload(dev, layer, maps)
         try:
             res = dev.pca(csr, step.n_components, step.seed, PCA_POWER_ITERATIONS)
+            res.made_values = csr.values() if keep_layer and maps is not None else None
         finally:
             csr.free()
         return res, res.device_embedding
+    if isinstance(layer, _StoreRef):
+        layer = layer.host()
+    made_values = None
     if maps is not None:
         # The maps see every gene of a cell (its total), then the gene set is taken.
         made = _upload(dev, layer, maps)
         try:
-            layer = _Layer(layer.indptr, layer.indices, made.values(), layer.n_genes)
+            made_values = made.values()
+            layer = _Layer(layer.indptr, layer.indices, made_values, layer.n_genes)
         finally:
             made.free()
     # A gene set: the layer's columns of the set, in order (pca.rs: the sums run in
@@ -377,6 +467,7 @@ def _run_pca(dev: Device, layer: _Layer, step: _Pca, maps=None):
     full = np.zeros((layer.n_genes, step.n_components), np.float32)
     full[flags] = res.loadings
     res.loadings = full
+    res.made_values = made_values if keep_layer else None
     return res, res.device_embedding
 
 
@@ -393,8 +484,31 @@ def _run_neighbors(dev: Device, emb: DenseMatrix, step: _Neighbors) -> Graph:
         index.free()
 
 
-def scan(layers, default_layer: str = "X") -> Atlas:
-    """A plan over one CSR layer (named `default_layer`) or a dict of them."""
+def _scan_store(path, default_layer) -> Atlas:
+    """A plan over the layers of a kyto store (the reference's scan(store),
+    python/kyto/_kyto.pyi:3820): its schema read now, no tile read until a step runs."""
+    path = os.fspath(path)
+    schema = _store.read_schema(path)
+    tiling = schema["tiling"]
+    refs = {m["name"]: _StoreRef(path, m["name"], int(schema["n_cells"]), int(schema["n_genes"]), int(m["nnz"]),
+                                 m["dtype"], int(tiling["cell_block_size"]), int(tiling["gene_block_size"]))
+            for m in schema.get("layers", [])}
+    if not refs:
+        raise KytoError(109, f"scan: the store at '{path}' has no layer")
+    if default_layer is None:
+        default_layer = "counts" if "counts" in refs else sorted(refs)[0]
+    if default_layer not in refs:
+        raise KytoError(109, f"scan: no layer named '{default_layer}' in the store at '{path}' (it has: "
+                             f"{', '.join(sorted(refs))})")
+    return Atlas(refs, default_layer)
+
+
+def scan(layers, default_layer: str | None = None) -> Atlas:
+    """A plan over a kyto store (a path), one CSR layer (named `default_layer`, "X" by
+    default) or a dict of them."""
+    if isinstance(layers, (str, os.PathLike)):
+        return _scan_store(layers, default_layer)
+    default_layer = "X" if default_layer is None else default_layer
     if isinstance(layers, dict):
         parsed = {k: _as_layer(v) for k, v in layers.items()}
     else:

@@ -119,6 +119,17 @@ SIGNATURES = {
     "ky_adam_step": [c_void_p, f32p, f32p, f32p, f32p, c_uint64, c_float, c_uint32],
     "ky_dense_extend_basis": [f64p, c_uint32, c_uint32, f64p, c_uint32, f64p, u32p],
     "ky_dense_rayleigh_ritz": [f64p, f64p, c_uint32, c_uint32, c_uint32, f64p, f64p],
+    "ky_store_layer_open": [c_char_p, c_char_p, POINTER(c_void_p)],
+    "ky_store_layer_shape": [c_void_p, u64p, u32p, u64p, u64p, u64p, POINTER(c_int)],
+    "ky_store_layer_close": [c_void_p],
+    "ky_store_layer_rows": [c_void_p, c_uint64, c_uint64, u64p, u32p, f32p, c_uint64, u64p],
+    "ky_csr_stream_layer": [c_void_p, c_void_p, c_uint64, c_double, c_int, POINTER(c_void_p), POINTER(StreamInfo)],
+    "ky_store_write_layer": [c_char_p, c_char_p, c_uint64, c_uint64, u64p, u32p, c_void_p, c_int, c_uint64, c_uint64,
+                             c_uint64, c_uint64, c_int],
+    "ky_store_write_embedding": [c_char_p, c_char_p, c_int, f32p, c_uint64, c_uint32, c_uint64, c_int],
+    "ky_store_read_embedding": [c_char_p, c_char_p, u64p, u32p, f32p, c_uint64],
+    "ky_store_write_graph": [c_char_p, c_char_p, c_uint64, u64p, u32p, f32p, c_uint64, c_uint64, c_int],
+    "ky_store_read_graph": [c_char_p, c_char_p, u64p, u64p, u64p, u32p, f32p],
 }
 
 _gpu = None
@@ -139,6 +150,9 @@ def gpu_lib():
             fn.argtypes = args
         lib.ky_last_error.restype = c_char_p
         lib.ky_last_error.argtypes = []
+        # A ky_block_reader: returns the block's number of values, -1 on failure.
+        lib.ky_store_layer_read.restype = ctypes.c_int64
+        lib.ky_store_layer_read.argtypes = [c_void_p, c_uint64, c_uint64, u64p, u32p, f32p, c_uint64]
         _gpu = lib
     return _gpu
 

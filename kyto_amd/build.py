@@ -37,7 +37,7 @@ MFMA_FLAGS = {
     "knn_bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1", "-fno-honor-nans", "-mno-amdgpu-ieee"],
     "router.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
 }
-GPU_SOURCES = ["pca.hip", "knn.hip", "knn_bf16.hip", "router.hip", "ivf.hip", "sort.hip", "capi.cpp", "dense.cpp", "memory.cpp", "layer.hip", "connectivities.hip"]
+GPU_SOURCES = ["pca.hip", "knn.hip", "knn_bf16.hip", "router.hip", "ivf.hip", "sort.hip", "capi.cpp", "dense.cpp", "memory.cpp", "layer.hip", "connectivities.hip", "store.cpp"]
 
 
 def _run(cmd):
@@ -61,7 +61,8 @@ def build_gpu() -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(GPU_SOURCES))) as pool:
         objs = list(pool.map(_compile, GPU_SOURCES))
     tmp = GPU_LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
+    # zstd (store chunks) is the system's libzstd.so.1, declared in store.cpp.
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread", "-l:libzstd.so.1"])
     os.replace(tmp, GPU_LIB)
     return GPU_LIB
 

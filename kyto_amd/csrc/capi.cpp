@@ -15,6 +15,7 @@
 #include "knn.hpp"
 #include "pca.hpp"
 #include "router.hpp"
+#include "store.hpp"
 
 namespace ky {
 
@@ -25,6 +26,7 @@ static thread_local std::string g_last_error;
 static thread_local hipStream_t t_call_stream = nullptr;
 
 void set_last_error(const std::string &message) { g_last_error = message; }
+std::string last_error_text() { return g_last_error; }
 
 template <typename F>
 static int run_guarded(F &&f) {
@@ -540,6 +542,159 @@ int ky_dense_rayleigh_ritz(const double *basis, const double *products, uint32_t
         rayleigh_ritz(b, p, k, ev, l);
         for (uint32_t j = 0; j < k; j++) eigenvalues[j] = ev[j];
         std::memcpy(loadings, l.v.data(), sizeof(double) * rows * k);
+    });
+}
+
+// ---- kyto stores on disk ----
+
+int ky_store_layer_open(const char *store, const char *layer, ky_layer **out) {
+    return guarded([&] {
+        need(store, "store");
+        need(layer, "layer");
+        need(out, "out");
+        *out = layer_open(store, layer);
+    });
+}
+
+int ky_store_layer_shape(const ky_layer *layer, uint64_t *n_cells, uint32_t *n_genes, uint64_t *nnz,
+                         uint64_t *cell_block, uint64_t *gene_block, int *counts) {
+    return guarded([&] {
+        need(layer, "layer");
+        uint64_t n, g, z, c, gb;
+        bool u32;
+        layer_shape(*layer, &n, &g, &z, &c, &gb, &u32);
+        KY_CHECK(g < (1ull << 32), KY_E215, "store: the layer has 2^32 or more genes");
+        if (n_cells) *n_cells = n;
+        if (n_genes) *n_genes = (uint32_t)g;
+        if (nnz) *nnz = z;
+        if (cell_block) *cell_block = c;
+        if (gene_block) *gene_block = gb;
+        if (counts) *counts = u32 ? 1 : 0;
+    });
+}
+
+int64_t ky_store_layer_read(void *layer, uint64_t first_row, uint64_t n_rows, uint64_t *indptr, uint32_t *indices,
+                            float *values, uint64_t capacity) {
+    int64_t got = -1;
+    int status = run_guarded([&] {
+        need(layer, "layer");
+        need(indptr, "indptr");
+        got = layer_read_rows(*static_cast<ky_layer *>(layer), first_row, n_rows, indptr, indices, values, capacity);
+    });
+    return status == KY_OK ? got : -1;
+}
+
+int ky_store_layer_rows(ky_layer *layer, uint64_t first_row, uint64_t n_rows, uint64_t *indptr, uint32_t *indices,
+                        float *values, uint64_t capacity, uint64_t *nnz) {
+    return guarded([&] {
+        need(layer, "layer");
+        need(indptr, "indptr");
+        need(nnz, "nnz");
+        *nnz = (uint64_t)layer_read_rows(*layer, first_row, n_rows, indptr, indices, values, capacity);
+    });
+}
+
+int ky_store_layer_close(ky_layer *layer) {
+    return guarded([&] { layer_close(layer); });
+}
+
+int ky_csr_stream_layer(ky_ctx *ctx, ky_layer *layer, uint64_t block_rows, double target_sum, int log1p,
+                        ky_csr **out, ky_stream_info *info) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(layer, "layer");
+        need(out, "out");
+        KY_CHECK(target_sum >= 0.0 && std::isfinite(target_sum), KY_E215,
+                 "normalize_total: target_sum must be a finite number above 0 (or 0: no normalize_total)");
+        uint64_t n, g, nnz, cb, gb;
+        bool u32;
+        layer_shape(*layer, &n, &g, &nnz, &cb, &gb, &u32);
+        KY_CHECK(g >= 1 && g < (1ull << 32), KY_E215, "store: the layer must have 1 to 2^32 - 1 genes");
+        if (block_rows == 0) block_rows = 8 * cb;
+        use_device(*ctx);
+        // The store's failure is kept and rethrown with its own code.
+        struct Reader {
+            ky_layer *layer;
+            std::exception_ptr failure;
+        } reader{layer, nullptr};
+        ky_block_reader read = [](void *user, uint64_t first, uint64_t count, uint64_t *ip, uint32_t *ix, float *v,
+                                  uint64_t cap) -> int64_t {
+            Reader *r = static_cast<Reader *>(user);
+            try {
+                return layer_read_rows(*r->layer, first, count, ip, ix, v, cap);
+            } catch (...) {
+                r->failure = std::current_exception();
+                return -1;
+            }
+        };
+        ky_csr *csr = new ky_csr();
+        try {
+            csr->ctx = ctx;
+            csr->n_rows = n;
+            csr->n_cols = (uint32_t)g;
+            LayerMaps maps;
+            maps.normalize = target_sum > 0.0;
+            maps.target_sum = target_sum;
+            maps.log1p = log1p != 0;
+            csr_stream(*csr, block_rows, read, &reader, nnz, maps, info);
+        } catch (...) {
+            delete csr;
+            if (reader.failure) std::rethrow_exception(reader.failure);
+            throw;
+        }
+        *out = csr;
+    });
+}
+
+int ky_store_write_layer(const char *store, const char *name, uint64_t n_cells, uint64_t n_genes, const uint64_t *indptr,
+                         const uint32_t *indices, const void *values, int counts, uint64_t cell_block,
+                         uint64_t gene_block, uint64_t nnz_chunk, uint64_t nnz_shard, int zstd_level) {
+    return guarded([&] {
+        need(store, "store");
+        need(name, "name");
+        write_layer(store, name, indptr, indices, values, counts != 0, n_cells, n_genes, cell_block, gene_block, nnz_chunk,
+                    nnz_shard, zstd_level);
+    });
+}
+
+int ky_store_write_embedding(const char *store, const char *name, int axis, const float *values, uint64_t rows,
+                             uint32_t width, uint64_t chunk_rows, int zstd_level) {
+    return guarded([&] {
+        need(store, "store");
+        need(name, "name");
+        write_embedding(store, name, axis, values, rows, width, chunk_rows, zstd_level);
+    });
+}
+
+int ky_store_read_embedding(const char *store, const char *name, uint64_t *rows, uint32_t *width, float *values,
+                            uint64_t capacity) {
+    return guarded([&] {
+        need(store, "store");
+        need(name, "name");
+        need(rows, "rows");
+        need(width, "width");
+        read_embedding(store, name, rows, width, values, capacity);
+    });
+}
+
+int ky_store_write_graph(const char *store, const char *name, uint64_t n_cells, const uint64_t *indptr,
+                         const uint32_t *indices, const float *weights, uint64_t nnz_chunk, uint64_t nnz_shard,
+                         int zstd_level) {
+    return guarded([&] {
+        need(store, "store");
+        need(name, "name");
+        write_graph(store, name, indptr, indices, weights, n_cells, nnz_chunk, nnz_shard, zstd_level);
+    });
+}
+
+int ky_store_read_graph(const char *store, const char *name, uint64_t *n_cells, uint64_t *edges, uint64_t *indptr,
+                        uint32_t *indices, float *weights) {
+    return guarded([&] {
+        need(store, "store");
+        need(name, "name");
+        need(n_cells, "n_cells");
+        need(edges, "edges");
+        read_graph(store, name, n_cells, edges, indptr, indices, weights);
     });
 }
 

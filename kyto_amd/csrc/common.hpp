@@ -24,6 +24,8 @@ struct Error : std::runtime_error {
 };
 
 void set_last_error(const std::string &message);
+// The calling thread's last error message (set by a failed C-ABI call).
+std::string last_error_text();
 
 #define KY_HIP(call)                                                                      \
     do {                                                                                  \

@@ -531,7 +531,9 @@ void csr_stream(Csr &csr, uint64_t block_rows, ky_block_reader reader, void *use
         if (used[k]) KY_HIP(hipEventSynchronize(guard.e[k]));  // the set's previous copy is done
         const double t0 = now_ms();
         int64_t got = reader(user, first, cnt, pin_ptr[k].ptr, pin_idx[k].ptr, pin_val[k].ptr, pin_idx[k].n);
-        KY_CHECK(got >= 0, KY_E505, "csr: the block reader failed at row " + std::to_string(first));
+        KY_CHECK(got >= 0, KY_E505,
+                 "csr: the block reader failed at row " + std::to_string(first) +
+                     (last_error_text().empty() ? std::string() : ": " + last_error_text()));
         if ((uint64_t)got > pin_idx[k].n) {
             pin_idx[k].reserve((uint64_t)got);
             pin_val[k].reserve((uint64_t)got);

@@ -33,6 +33,7 @@ class PcaOutput:
     passes: int
     timing: dict = field(default_factory=dict)
     device_embedding: "DenseMatrix | None" = None
+    made_values: "np.ndarray | None" = None  # the layer's values when the device made it (kept for a sink)
 
 
 @dataclass
@@ -130,6 +131,19 @@ class Device:
         layer = CsrLayer._adopt(self, h, int(n_rows), int(n_genes), int(info.nnz))
         layer.stream_info = _info(info)
         return layer
+
+    def stream_layer(self, layer, block_rows: int = 0, target_sum=None, log1p=False) -> "CsrLayer":
+        """A kyto store's layer (kyto_amd.store.StoreLayer) read cell block by cell block
+        through the pinned double buffers (ky_csr_stream_layer: the tiles decode
+        straight into them), with normalize_total(target_sum) / log1p made on the
+        device as upload_counts does. block_rows 0: eight cell blocks."""
+        h = c_void_p()
+        info = N.StreamInfo()
+        N.check(self._lib.ky_csr_stream_layer(self._h, layer.handle, int(block_rows), float(target_sum or 0.0),
+                                              1 if log1p else 0, byref(h), byref(info)))
+        out = CsrLayer._adopt(self, h, layer.n_cells, layer.n_genes, int(info.nnz))
+        out.stream_info = _info(info)
+        return out
 
     def upload_counts(self, indptr, indices, counts, n_genes: int, target_sum=1e4, log1p=True) -> "CsrLayer":
         """A counts CSR made into the layer on the device: kyto's normalize_total

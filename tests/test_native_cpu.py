@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 20
     for name in names:
         assert hasattr(lib, name), name
-    assert set(N.SIGNATURES) | {"ky_last_error"} >= set(names)
+    assert set(N.SIGNATURES) | {"ky_last_error", "ky_store_layer_read"} >= set(names)
 
 
 def test_missing_device_is_e504():
