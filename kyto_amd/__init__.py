@@ -293,11 +293,11 @@ class Atlas:
                   cell_block=None, gene_block=None, zstd_level=_store.ZSTD_LEVEL) -> None:
         """Run the plan and write its output as a new kyto store at `path` (the
         reference's Atlas.sink_zarr, python/kyto/_kyto.pyi:1323; format.md §10): the
-        scanned layers (copied from a scanned store with its tables), the layer a
-        pca read when the maps made it (the last map's), each pca's embedding and `<into>_loadings`, each neighbors step's
-        distances and connectivities graphs. The store is written next to `path` and
-        renamed into place; KY-E118 if `path` exists unless overwrite=True (and then
-        only a kyto store is replaced)."""
+        scanned layers (copied from a scanned store with its tables), the layer a pca
+        read when the maps made it (the last map's), each pca's embedding and
+        `<into>_loadings`, each neighbors step's distances and connectivities graphs.
+        The store is written next to `path` and renamed into place; KY-E118 if `path`
+        exists unless overwrite=True (and then only a kyto store is replaced)."""
         out = self._run(memory_limit, threads, progress, keep_layers=True)
         made = out.pop("__made_layers__")
         first = next(iter(self.layers.values()))
@@ -311,7 +311,8 @@ class Atlas:
                 layers[name] = _store.LayerData(layer.indptr, layer.indices, layer.values)
         for name, (source, values) in made.items():
             src = self.layers[source]
-            ip, ix = (src.indptr, src.indices) if not isinstance(src, _StoreRef) else src.host()[:2]
+            host = src.host() if isinstance(src, _StoreRef) else src
+            ip, ix = host.indptr, host.indices
             keep = values != 0  # a map can make a zero (format.md §4.6: never stored)
             if not keep.all():
                 rows = np.repeat(np.arange(n_cells), np.diff(np.asarray(ip, np.int64)))[keep]
@@ -505,10 +506,11 @@ def _scan_store(path, default_layer) -> Atlas:
 
 def scan(layers, default_layer: str | None = None) -> Atlas:
     """A plan over a kyto store (a path), one CSR layer (named `default_layer`, "X" by
-    default) or a dict of them."""
+    default) or a dict of them (the default layer: the only one, else "X")."""
     if isinstance(layers, (str, os.PathLike)):
         return _scan_store(layers, default_layer)
-    default_layer = "X" if default_layer is None else default_layer
+    if default_layer is None:
+        default_layer = next(iter(layers)) if isinstance(layers, dict) and len(layers) == 1 else "X"
     if isinstance(layers, dict):
         parsed = {k: _as_layer(v) for k, v in layers.items()}
     else:

@@ -43,7 +43,6 @@ size_t ZSTD_compressCCtx(ZSTD_CCtx *cctx, void *dst, size_t dst_capacity, const 
 size_t ZSTD_compressBound(size_t src_size);
 unsigned ZSTD_isError(size_t code);
 const char *ZSTD_getErrorName(size_t code);
-unsigned ZSTD_versionNumber(void);
 }
 
 namespace ky {
@@ -1412,7 +1411,5 @@ void read_graph(const std::string &store, const std::string &name, uint64_t *n_c
             if (e > ip[r] && indices[e] <= indices[e - 1]) b.fail("row " + std::to_string(r) + ": indices do not increase");
         }
 }
-
-unsigned zstd_version() { return ZSTD_versionNumber(); }
 
 }  // namespace ky

@@ -299,3 +299,46 @@ def test_scan_of_a_store(tmp_path):
     _expect("KY-E109", lambda: atlas.pca(layer="x", seed=0))
     ref = atlas.layers["counts"].host()
     _same((ref.indptr, ref.indices, ref.values), counts, True)
+
+
+def test_sink_writes_the_plan_outputs(tmp_path, monkeypatch):
+    # The sink's assembly on the CPU: collect's outputs stood in for (the device run
+    # is tests/test_gpu_store.py), the store written and read back.
+    import kyto_amd
+    from kyto_amd.device import ConnectivityGraph, Graph
+    n, g = 300, 40
+    counts = _csr(n, g, 0.1, seed=6, counts=True)
+    src = tmp_path / "in.kyto"
+    ks.create_store(src, n_cells=n, n_genes=g, cell_block=128, gene_block=32, layers={"counts": ks.LayerData(*counts)})
+    plan = kyto_amd.scan(src).normalize_total(1e4).log1p().pca(n_components=5, seed=0).neighbors(n_neighbors=4)
+    rng = np.random.default_rng(0)
+    made = counts[2].astype(np.float32) * 0.5
+    made[3] = 0.0  # a made zero is not stored
+    emb, load = rng.normal(size=(n, 5)).astype(np.float32), rng.normal(size=(g, 5)).astype(np.float32)
+    idx = np.sort(np.stack([rng.choice(n, 3, replace=False) for _ in range(n)]), axis=1).astype(np.uint32).ravel()
+    ip = np.arange(0, 3 * n + 1, 3, dtype=np.uint64)
+    dist, w = rng.random(3 * n).astype(np.float32), rng.random(3 * n).astype(np.float32)
+
+    def fake_run(self, memory_limit, threads, progress, keep_layers):
+        assert keep_layers
+        return {"X_pca": emb, "X_pca_loadings": load, "distances": Graph(4, ip, idx, dist),
+                "connectivities": ConnectivityGraph(ip, idx, w, None, None), "__made_layers__": {"log1p": ("counts", made)}}
+
+    monkeypatch.setattr(kyto_amd.Atlas, "_run", fake_run)
+    out = tmp_path / "out.kyto"
+    plan.sink_zarr(out)
+    schema = ks.read_schema(out)
+    assert [m["name"] for m in schema["layers"]] == ["counts", "log1p"]
+    assert schema["layers"][1] == {"name": "log1p", "dtype": "float32", "nnz": int(counts[0][-1]) - 1}
+    assert schema["tiling"] == {"cell_block_size": 128, "gene_block_size": 32}
+    with ks.StoreLayer(out, "counts") as layer:
+        _same(layer.read_rows(), counts, True)
+    with ks.StoreLayer(out, "log1p") as layer:
+        lip, lix, lv = layer.read_rows()
+    keep = made != 0
+    assert np.array_equal(lix, counts[1][keep]) and np.array_equal(lv, made[keep]) and int(lip[-1]) == keep.sum()
+    assert np.array_equal(ks.read_embedding(out, "X_pca"), emb)
+    assert np.array_equal(ks.read_embedding(out, "X_pca_loadings"), load)
+    assert np.array_equal(ks.read_graph(out, "distances")[2], dist)
+    assert np.array_equal(ks.read_graph(out, "connectivities")[2], w)
+    assert (out / "cells.parquet").read_bytes() == (src / "cells.parquet").read_bytes()
