@@ -130,7 +130,8 @@ int ky_csr_shape(const ky_csr *csr, uint64_t *n_rows, uint32_t *n_cols, uint64_t
  * version), schema.json, the four Zarr v3 arrays of the layer checked against it
  * (shapes, dtypes, sharding; KY_E107), tile_offsets read and checked (from 0,
  * non-decreasing, ending at nnz), the shard indexes of indices and data read (crc32c
- * checked). KY_E109 if the store has no such layer. */
+ * checked). KY_E109 if the store has no such layer. A ky_layer is read from one
+ * thread at a time (its decode buffers are reused between reads). */
 int ky_store_layer_open(const char *store, const char *layer, ky_layer **out);
 /* The layer's axes, stored values and tiling; `counts` is 1 for uint32 values. */
 int ky_store_layer_shape(const ky_layer *layer, uint64_t *n_cells, uint32_t *n_genes, uint64_t *nnz,
