@@ -18,14 +18,15 @@
 //
 // Block: 4 wavefronts, each owning 16*NB queries (NB MFMA column blocks, lane = query
 // l % 16 of its block); the block streams the points in tiles of TILE rows through
-// two LDS buffers (padded rows, conflict-free 16-byte fragment reads; the next tile
-// loaded into registers during the current one, one barrier a tile). The M-blocks of
-// a tile are software-pipelined: the next block's MFMAs are issued before this one's
-// epilogue, a min of each lane's 4 keys against its query's threshold, one branch an
-// M-block. A query keeps every point whose key is at most its threshold in a buffer
-// in device memory (BCAP entries): the first INIT points of its scan directly, then
-// appends; radix selection compacts a buffer back to its m smallest (key, row) pairs
-// when the next tile could overflow it, and sets the threshold to the m-th key.
+// three LDS buffers (global_load_lds two tiles ahead, XOR-swizzled rows for
+// conflict-free 16-byte fragment reads, one barrier a tile). Within a tile, super-
+// blocks of SPB points: the next super-block's fragments are read from LDS while this
+// one's MFMAs and hit test run; the hit test is a min tree of each lane's keys of a
+// query against its threshold, one scalar branch a super-block. A query keeps every
+// point whose key is at most its threshold in a buffer in device memory (BCAP
+// entries), appended through a per-wave LDS stage; radix selection compacts a buffer
+// back to its m smallest (key, row) pairs when the next tile could overflow it, and
+// sets the threshold to the m-th key.
 //
 // The certificate (knn.hip, rerank, floor_mode 1) uses, per row, err = |b - a| (f64,
 // a = x - mean exactly) and the accumulation bound delta_q = c (|b_q| + max|b|)^2:
@@ -463,14 +464,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         auto body = [&](const unsigned char *__restrict__ tb, unsigned char *__restrict__ nx,
                         uint32_t *__restrict__ sk, uint32_t *__restrict__ sr, uint16_t *__restrict__ sd) {
             if (wave_live) {
-#pragma nounroll
-                for (int sb = 0; sb < TILE / SPB; sb++) {
-                    // Keys of the super-block: acc[b][4 g + j] is point 16 g + 4 (lane / 16) + j
-                    // of the super-block, query 16 b + lane % 16 of the wave.
-                    f32x16 acc[NB];
-                    // All fragments first, then the first halves of K of every (g, b), then
-                    // the second halves: each accumulator's two MFMAs are MB * NB apart.
-                    bf16x8 a0[MB], a1[MB];
+                // The point fragments of super-block sb (row blocks g of 16 points, the two
+                // halves of K); those of the next super-block are read from LDS while this
+                // one's MFMAs and hit test run.
+                bf16x8 a0[MB], a1[MB];
+                auto fragments = [&](int sb) {
 #pragma unroll
                     for (int g = 0; g < MB; g++) {
                         const unsigned prow = sb * SPB + g * 16 + (lane % 16);
@@ -478,6 +476,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                         a0[g] = *reinterpret_cast<const bf16x8 *>(rp + (((lane / 16) ^ (prow % 8)) * 16));
                         a1[g] = *reinterpret_cast<const bf16x8 *>(rp + (((4 + lane / 16) ^ (prow % 8)) * 16));
                     }
+                };
+                fragments(0);
+#pragma nounroll
+                for (int sb = 0; sb < TILE / SPB; sb++) {
+                    // Keys of the super-block: acc[b][4 g + j] is point 16 g + 4 (lane / 16) + j
+                    // of the super-block, query 16 b + lane % 16 of the wave.
+                    f32x16 acc[NB];
+                    // The first halves of K of every (g, b), then the second halves: each
+                    // accumulator's two MFMAs are MB * NB apart.
                     f32x4 z[MB][NB];
 #pragma unroll
                     for (int g = 0; g < MB; g++)
@@ -496,6 +503,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                             else if (g == 2) acc[b].s89ab = r;
                             else acc[b].scdef = r;
                         }
+                    if (sb + 1 < TILE / SPB) fragments(sb + 1);
                     // One test for the super-block: the smallest of each lane's 4 MB keys of
                     // a query (a tree of 3-input mins) against the query's threshold; the
                     // column blocks' ballots OR'ed in scalar registers, one scalar branch.
@@ -525,17 +533,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 #pragma unroll
                     for (int b = 0; b < NB; b++) {
                         if (!__ballot(hit[b])) continue;
-                        // The keys (g, j) with a hit in some lane, found group by group.
+                        // The keys with a hit in some lane, one ballot a key. (No partial mins
+                        // here: the compiler would share them with the test above and build
+                        // that tree from 2-input mins, a VALU instruction more a column block
+                        // on every super-block.)
                         uint32_t sites = 0;
 #pragma unroll
-                        for (int g = 0; g < MB; g++) {
-                            const float gm = fminf(fminf(acc[b][4 * g], acc[b][4 * g + 1]),
-                                                   fminf(acc[b][4 * g + 2], acc[b][4 * g + 3]));
-                            if (!__ballot(gm <= thr[b])) continue;
-#pragma unroll
-                            for (int j = 0; j < 4; j++)
-                                if (__ballot(acc[b][4 * g + j] <= thr[b])) sites |= 1u << (4 * g + j);
-                        }
+                        for (int e = 0; e < 4 * MB; e++)
+                            if (__ballot(acc[b][e] <= thr[b])) sites |= 1u << e;
                         const uint32_t self = qrow[16 * b + (lane % 16)];
                         const uint64_t same = 0x0001000100010001ull << (lane % 16);
                         while (sites) {
