@@ -16,7 +16,7 @@ with, and the plain FP32 kNN stage on the same embedding.
 
     python bench.py [--steps K] [--warmup W] [--precision bf16|fp32] [--q 4]
                     [--workload 1m|4m|100k] [--compare] [--no-compare] [--no-cpu-baseline] [--copy-calibration]
-                    [--no-store]
+                    [--no-store] [--from-store]
 
 Outside the timed region (1m, 100k) the resident layer is also written as a kyto store
 under /tmp and read back onto the device through the store reader (the "store" field).
@@ -190,6 +190,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the learned leg, the recall check and the plain FP32 kNN stage")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--from-store", action="store_true",
+                    help="4m: write the counts as a kyto store first and read the layer from it")
     ap.add_argument("--no-store", action="store_true", help="skip the store leg (write the layer as a kyto store, "
                                                             "read it back onto the device)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -207,7 +209,38 @@ def main():
     dev = kyto_amd.Device(0)
     n = w.n_cells
     ingest = None
-    if args.workload == "4m":
+    if args.workload == "4m" and args.from_store:
+        # The counts written as a kyto store under /tmp first (outside any timing but
+        # its own), then read tile by tile into the pinned buffers and made into the
+        # log1p layer on the device.
+        import shutil
+        import tempfile
+        from kyto_amd import store as ks
+        c = S.counts(w.n_cells, w.n_genes, w.n_types, w.density, w.seed)
+        tmp = tempfile.mkdtemp(prefix="kyto-store-", dir="/tmp")
+        try:
+            path = os.path.join(tmp, "counts.kyto")
+            t0 = time.perf_counter()
+            ks.create_store(path, n_cells=w.n_cells, n_genes=w.n_genes, cell_block=4096, gene_block=1024,
+                            layers={"counts": ks.LayerData(c.indptr, c.indices, c.values.astype(np.uint32))})
+            write_ms = (time.perf_counter() - t0) * 1e3
+            del c
+            disk = sum(os.path.getsize(os.path.join(r, f)) for r, _, fs in os.walk(os.path.join(path, "layers"))
+                       for f in fs)
+            with ks.StoreLayer(path, "counts") as opened:
+                t0 = time.perf_counter()
+                csr = dev.stream_layer(opened, block_rows=32_768, target_sum=1e4, log1p=True)
+                wall = (time.perf_counter() - t0) * 1e3
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        ingest = dict(csr.stream_info, wall_ms=wall, store_write_ms=write_ms, store_bytes_on_disk=disk,
+                      source="kyto store (tiles 4,096 x 1,024, zstd 3)")
+        layer = None
+        nnz = csr.nnz
+        log(f"workload {w.name}: {w.n_cells} x {w.n_genes}, nnz {nnz}, store written in {write_ms:.0f} ms "
+            f"({disk / 1e9:.2f} GB on disk), read onto the device in {ingest['blocks']} blocks, "
+            f"{ingest['total_ms']:.0f} ms (reader {ingest['read_ms']:.0f} ms)")
+    elif args.workload == "4m":
         # Read in 32,768-cell row blocks through the pinned double-buffered reader; the
         # counts become the log1p layer on the device; the host holds two blocks.
         reader = S.block_reader(w.n_cells, w.n_genes, w.n_types, w.density, w.seed)
@@ -318,8 +351,9 @@ def main():
     if ingest is not None:
         out["ingest"] = ingest
     if args.workload == "4m":
-        out["config"]["ingest"] = "counts read in 32,768-cell blocks through pinned double buffers, " \
-                                  "normalize_total(1e4) + log1p on the device; host holds two blocks"
+        out["config"]["ingest"] = ("counts read from a kyto store" if args.from_store else "counts read") + \
+            " in 32,768-cell blocks through pinned double buffers, normalize_total(1e4) + log1p on the device; " \
+            "host holds two blocks"
     if not args.no_compare:
         # Outside the timed region: the plain FP32 exact path (kyto's exact graph, the
         # kNN stage the metric compares with), the BF16 graph's equality with it, and
