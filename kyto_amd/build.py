@@ -37,7 +37,7 @@ MFMA_FLAGS = {
     "knn_bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1", "-fno-honor-nans", "-mno-amdgpu-ieee"],
     "router.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
 }
-GPU_SOURCES = ["pca.hip", "knn.hip", "knn_bf16.hip", "router.hip", "ivf.hip", "sort.hip", "capi.cpp", "dense.cpp", "memory.cpp", "layer.hip", "connectivities.hip", "store.cpp"]
+GPU_SOURCES = ["pca.hip", "knn.hip", "knn_bf16.hip", "router.hip", "ivf.hip", "sort.hip", "capi.cpp", "dense.cpp", "memory.cpp", "layer.hip", "connectivities.hip", "store.cpp", "dense_dev.hip"]
 
 
 def _run(cmd):

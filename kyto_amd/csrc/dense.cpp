@@ -438,14 +438,17 @@ static bool cholesky_qr2(const Mat &a, Mat &q, Mat &r) {
     return true;
 }
 
-Mat extend_basis(const Mat &basis, const Mat &block) {
+Mat extend_basis(const Mat &basis, const Mat &block, DeviceBasis *dev) {
     size_t rows = block.rows;
     double scale = frobenius(block);
     if (scale == 0.0 || block.cols == 0) return Mat(rows, 0);
     if (!std::isfinite(scale))
         throw Error(KY_E215, "pca: orthonormalising a Krylov block did not converge (a value that is not finite)");
     Mat rest = block;
-    if (basis.cols > 0) {
+    if (basis.cols > 0 && dev) {
+        KY_CHECK(dev->cols == basis.cols, KY_E505, "pca: the device basis is out of step (internal)");
+        dev->project_out(rest);
+    } else if (basis.cols > 0) {
         for (int pass = 0; pass < 2; pass++) {
             Mat proj = mul_tn(basis, rest);   // bc x bw
             Mat back = mul_nn(basis, proj);   // rows x bw
@@ -683,11 +686,12 @@ static void flip_signs(Mat &m) {
 }
 
 void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<double> &eigenvalues,
-                   Mat &loadings) {
+                   Mat &loadings, DeviceBasis *dev) {
     loadings = Mat(basis.rows, 0);
     eigenvalues.clear();
+    if (dev) KY_CHECK(dev->cols == basis.cols, KY_E505, "pca: the device basis is out of step (internal)");
     if (basis.cols > 0) {
-        Mat t = mul_tn(basis, products);
+        Mat t = dev ? dev->rayleigh_matrix() : mul_tn(basis, products);
         size_t p = t.rows;
         Mat sym(p, p);
         for (size_t i = 0; i < p; i++)
@@ -700,7 +704,7 @@ void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<
         size_t taken = std::min(k, p);
         symmetric_eigen(sym, taken, d, chosen);
         for (size_t c = 0; c < taken; c++) eigenvalues.push_back(std::max(d[p - 1 - c], 0.0));
-        loadings = mul_nn(basis, chosen);
+        loadings = dev ? dev->times(chosen) : mul_nn(basis, chosen);
     }
     size_t missing = k - eigenvalues.size();
     if (missing > 0) {

@@ -27,11 +27,29 @@ struct Mat {
 Mat mul_tn(const Mat &a, const Mat &b);
 Mat mul_nn(const Mat &a, const Mat &b);
 
-// pca_dense.rs:105-129.
-Mat extend_basis(const Mat &basis, const Mat &block);
+// The basis B and its products C resident on the device (rows x cap each, row-major),
+// for the dense steps' large FP64 products (dense_dev.hip). Holds the same columns as
+// the host's basis and products.
+struct DeviceBasis {
+    Context *ctx;
+    size_t rows, cap, cols = 0;
+    DevBuf<double> basis, products, work_r, work_p, partial;
+    DeviceBasis(Context &c, size_t rows, size_t cap);
+    void append(const Mat &block, const Mat &product);
+    // CGS2 against the held columns: rest -= B (B^T rest), twice.
+    void project_out(Mat &rest);
+    // B^T C (cols x cols).
+    Mat rayleigh_matrix();
+    // B S (rows x S.cols).
+    Mat times(const Mat &small);
+};
 
-// pca_dense.rs:141-174.
+// pca_dense.rs:105-129. With `dev` (holding the same columns as `basis`), the
+// projections run on the device.
+Mat extend_basis(const Mat &basis, const Mat &block, DeviceBasis *dev = nullptr);
+
+// pca_dense.rs:141-174. With `dev`, B^T C and B U run on the device.
 void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<double> &eigenvalues,
-                   Mat &loadings);
+                   Mat &loadings, DeviceBasis *dev = nullptr);
 
 }  // namespace ky
