@@ -486,6 +486,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     // The first halves of K of every (g, b), then the second halves: each
                     // accumulator's two MFMAs are MB * NB apart.
                     f32x4 z[MB][NB];
+                    // Raised wave priority while this wave issues its MFMAs (and reads the
+                    // next fragments), back to normal for the hit test: the SIMD's MFMA
+                    // pipe is fed first (0.9 ms a launch, A/B on one box).
+                    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
                     for (int g = 0; g < MB; g++)
 #pragma unroll
@@ -504,6 +508,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                             else acc[b].scdef = r;
                         }
                     if (sb + 1 < TILE / SPB) fragments(sb + 1);
+                    __builtin_amdgcn_s_setprio(0);
                     // One test for the super-block: the smallest of each lane's 4 MB keys of
                     // a query (a tree of 3-input mins) against the query's threshold; the
                     // column blocks' ballots OR'ed in scalar registers, one scalar branch.

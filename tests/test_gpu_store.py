@@ -103,3 +103,26 @@ def test_plan_from_store_to_store(device, counts_store, tmp_path):
     assert np.array_equal(lix, ix) and np.array_equal(lv.view(np.uint32), want.view(np.uint32))
     with open(out / "cells.parquet", "rb") as f:
         assert f.read(4) == b"PAR1"
+
+
+def test_store_reader_as_a_block_reader(device, counts_store):
+    # The C ABI a Rust caller binds: ky_store_layer_read passed to ky_csr_stream as its
+    # ky_block_reader, the ky_layer as its user pointer.
+    import ctypes
+    from kyto_amd.device import CsrLayer
+    path, (ip, ix, v, g) = counts_store
+    lib = N.gpu_lib()
+    reader = ctypes.cast(lib.ky_store_layer_read, N.BlockReader)
+    want = device.upload_counts(ip, ix, v.astype(np.float32), g, target_sum=1e4, log1p=True)
+    with ks.StoreLayer(path, "counts") as layer:
+        h = ctypes.c_void_p()
+        info = N.StreamInfo()
+        N.check(lib.ky_csr_stream(device._h, layer.n_cells, layer.n_genes, 5000, reader, layer.handle, 0, 1e4, 1,
+                                  ctypes.byref(h), ctypes.byref(info)))
+    got = CsrLayer._adopt(device, h, len(ip) - 1, g, int(info.nnz))
+    try:
+        assert int(info.nnz) == int(ip[-1])
+        assert np.array_equal(got.values().view(np.uint32), want.values().view(np.uint32))
+    finally:
+        got.free()
+        want.free()
