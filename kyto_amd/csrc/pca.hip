@@ -160,7 +160,7 @@ __global__ void omega_rows(uint64_t n_rows, uint64_t seed, uint64_t cell_offset,
 // would. Every lane loads (a branch around the loads would wait for each one): a lane
 // past the last column reads into the next dense row, or into the DENSE_PAD floats
 // after the last one, and its sum is never stored.
-constexpr int UNROLL = 8;
+constexpr int UNROLL = 16;
 // Floats allocated after every dense operand of the products (>= 64 * max CPL): the
 // lanes past the last column of the last row read them.
 constexpr size_t DENSE_PAD = 256;
