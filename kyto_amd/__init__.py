@@ -136,6 +136,7 @@ class _Pca:
     n_components: int
     seed: int
     genes: Any = None
+    power_iterations: int = PCA_POWER_ITERATIONS
 
 
 @dataclass(frozen=True)
@@ -200,12 +201,18 @@ class Atlas:
         """atlas.rs:628-632: log(1 + x) of every value (kyto's log1p, on the device)."""
         return self._add_map("log1p", layer, into, "log1p")
 
-    def pca(self, *, layer=None, into=None, n_components=None, seed=None, genes=None) -> "Atlas":
+    def pca(self, *, layer=None, into=None, n_components=None, seed=None, genes=None,
+            power_iterations=None) -> "Atlas":
         """atlas.rs:822-856: n_components defaults to 50, seed is required; layer None
-        reads the layer the latest normalize_total / log1p wrote, else the default."""
+        reads the layer the latest normalize_total / log1p wrote, else the default.
+        power_iterations (not in kyto's signature, whose plan fixes it at 9,
+        physical_plan.rs:73) sets the block-Krylov passes for a workload that names
+        another count."""
         if seed is None:
             raise _invalid("pca", "seed", None, "a seed is required (R7)", "an int of at least 0")
         seed = _integer("pca", "seed", seed, 0)
+        q = PCA_POWER_ITERATIONS if power_iterations is None else _integer("pca", "power_iterations",
+                                                                          power_iterations, 1)
         n_components = 50 if n_components is None else _integer("pca", "n_components", n_components, 1)
         name = layer if layer is not None else self._latest_layer()
         if name not in self._layer_names():
@@ -217,7 +224,7 @@ class Atlas:
             if not genes.any():
                 raise _invalid("pca", "genes", "mask", "the predicate is true for none of the genes reaching pca",
                                f"a predicate true for at least {n_components} genes (n_components)")
-        return replace(self, steps=self.steps + (_Pca(name, into or "X_pca", n_components, seed, genes),))
+        return replace(self, steps=self.steps + (_Pca(name, into or "X_pca", n_components, seed, genes, q),))
 
     def neighbors(self, embedding=None, /, *, n_neighbors=None, n_components=None, metric="euclidean",
                   search="exact", seed=None, into=None, precision="bf16") -> "Atlas":
@@ -431,7 +438,7 @@ def _run_pca(dev: Device, layer, step: _Pca, maps=None, keep_layer=False):
     if step.genes is None:
         csr = _upload(dev, layer, maps)
         try:
-            res = dev.pca(csr, step.n_components, step.seed, PCA_POWER_ITERATIONS)
+            res = dev.pca(csr, step.n_components, step.seed, step.power_iterations)
             res.made_values = csr.values() if keep_layer and maps is not None else None
         finally:
             csr.free()
@@ -462,7 +469,7 @@ def _run_pca(dev: Device, layer, step: _Pca, maps=None, keep_layer=False):
                        f"an int from 1 to {min(layer.n_cells, n_set)}")
     csr = dev.upload_csr(indptr, pos[layer.indices[keep]].astype(np.uint32), layer.values[keep], n_set)
     try:
-        res = dev.pca(csr, step.n_components, step.seed, PCA_POWER_ITERATIONS)
+        res = dev.pca(csr, step.n_components, step.seed, step.power_iterations)
     finally:
         csr.free()
     full = np.zeros((layer.n_genes, step.n_components), np.float32)

@@ -203,12 +203,12 @@ __global__ void query_positions(const uint32_t *__restrict__ q_rows, uint64_t nq
     vals[i] = (uint32_t)i;
 }
 
-__global__ void bf16_delta(const double *__restrict__ nb, uint64_t n, double nb_max, double c,
+__global__ void bf16_delta(const double *__restrict__ nb, uint64_t n, double nb_max, double c, double absolute,
                            double *__restrict__ delta) {
     uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (r >= n) return;
     double t = nb[r] + nb_max;
-    delta[r] = c * t * t;
+    delta[r] = c * t * t + absolute;
 }
 
 // ------------------------------------------------------------------- search ----
@@ -883,9 +883,16 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
     double nb_max;
     std::memcpy(&nb_max, &hm[1], sizeof(double));
     // Accumulation bound of a K-term f32 dot product in any order, doubled:
-    // 2 (K + 8) u over (|b_q| + |b_p|)^2 (the terms' absolute sum).
+    // 2 (K + 8) u over (|b_q| + |b_p|)^2 (the terms' absolute sum), plus an absolute
+    // term for what the relative bound cannot see below f32's normal range: a product
+    // or partial sum under FLT_MIN flushed or rounded to the subnormal grid (less than
+    // FLT_MIN each, 2 K of them), the three bf16 parts of o_q and of o_p likewise, and
+    // a subnormal component flushed (at most FLT_MIN max|b|, below the relative term
+    // once max|b| > FLT_MIN / c, below FLT_MIN otherwise). A tiny embedding's keys are
+    // then all within the bound and its rows go to the exact search.
     const double cb = 2.0 * (KDIM + 8.0) / 16777216.0;
-    bf16_delta<<<div_up(n, 256), 256, 0, st>>>(nb.ptr, n, nb_max, cb, err.ptr + n);
+    const double absolute = 4.0 * (KDIM + 8.0) * (double)FLT_MIN;
+    bf16_delta<<<div_up(n, 256), 256, 0, st>>>(nb.ptr, n, nb_max, cb, absolute, err.ptr + n);
     KY_HIP(hipGetLastError());
     acc_rel = cb;
     // Queries in scan order: every row (positions 0..n, outputs at the original row),

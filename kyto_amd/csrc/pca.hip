@@ -818,10 +818,14 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
     std::vector<double> eigenvalues;
     Mat loadings;
     bool finished = false;
-    auto block_means_of = [&](const Mat &b) {
+    // mu^T M for the f32-rounded operand M the device multiplies by: each row's
+    // projection x M32 - mu^T M32 is then (x - mu) M32 to f64 accuracy whatever the
+    // means (with mu^T M in f64 the rounding of M would leave mu (M32 - M) in every
+    // row, as large as the data's spread at means 1e6 times it).
+    auto rounded_means_of = [&](const Mat &b) {
         std::vector<double> bm(b.cols, 0.0);
         for (size_t g = 0; g < genes; g++)
-            for (size_t j = 0; j < b.cols; j++) bm[j] += means[g] * b.at(g, j);
+            for (size_t j = 0; j < b.cols; j++) bm[j] += means[g] * (double)(float)b.at(g, j);
         return bm;
     };
     auto finish_basis = [&]() {
@@ -833,7 +837,7 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
     uint32_t passes = 0;
     for (uint32_t p = 1; p <= q && !finished; p++) {
         const size_t w = block.cols;
-        std::vector<double> bm = block_means_of(block);
+        std::vector<double> bm = rounded_means_of(block);
         Mat product(genes, w);
         std::vector<double> wsum(w);
         t0 = now_ms();
@@ -859,10 +863,8 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
         finish_basis();
         dense_ms += now_ms() - td;
     }
-    // mean_projection = mu^T V.
-    std::vector<double> mean_projection(k, 0.0);
-    for (size_t g = 0; g < genes; g++)
-        for (size_t j = 0; j < k; j++) mean_projection[j] += means[g] * loadings.at(g, j);
+    // mean_projection = mu^T V32 (V as the device rounds it, above).
+    const std::vector<double> mean_projection = rounded_means_of(loadings);
     std::vector<double> squares(genes);
     std::vector<uint64_t> stored(genes);
     t0 = now_ms();

@@ -61,6 +61,17 @@ def lib():
                                      _f64p, _f64p, _f64p]
         L.ko_pca_rows.argtypes = [_u64p, _u32p, _f32p, ctypes.c_size_t, _f64p, ctypes.c_size_t,
                                   _f64p, _f64p, _f64p, _f64p, _u64p]
+        L.ko_pca_sketch_tree.argtypes = [_u64p, _u32p, _f32p, ctypes.c_size_t, _u64p, ctypes.c_uint64,
+                                         ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _u64p, _f64p, _f64p,
+                                         _f64p]
+        L.ko_pca_product_tree.argtypes = [_u64p, _u32p, _f32p, ctypes.c_size_t, _f64p, ctypes.c_size_t, _f64p,
+                                          ctypes.c_size_t, ctypes.c_size_t, _f64p, _f64p]
+        L.ko_pca_rows_tree.argtypes = [_u64p, _u32p, _f32p, ctypes.c_size_t, _f64p, ctypes.c_size_t, _f64p, _f64p,
+                                       ctypes.c_size_t, ctypes.c_size_t, _f64p, _f64p, _u64p]
+        L.ko_gram.argtypes = [_u64p, _u32p, _f32p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _f64p, _f64p]
+        L.ko_knn_exact_list.restype = ctypes.c_int
+        L.ko_knn_exact_list.argtypes = [_f32p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t,
+                                        _u32p, ctypes.c_size_t, ctypes.c_size_t, _f64p, _u32p, _u64p]
         L.ko_squared_distance.restype = ctypes.c_double
         L.ko_squared_distance.argtypes = [_f32p, _f32p, ctypes.c_size_t]
         L.ko_dot.restype = ctypes.c_double
@@ -211,11 +222,18 @@ class PcaResult:
     passes: int                # product passes run
 
 
+# kyto's default store tiling: cell blocks of 4,096 cells (crates/kyto-core/src/schema.rs:16).
+CELL_BLOCK = 4096
+
+
 def pca(indptr, indices, values, n_genes: int, n_components: int = 50, seed: int = 0,
-        q: int = PCA_POWER_ITERATIONS, cells=None) -> PcaResult:
-    """kyto's streamed block-Krylov PCA on one leaf (pca.rs): pass 0 sketch
-    (:588-622), end_pass (:712-801), q product passes (:623-645), the Rayleigh-Ritz
-    step (finish_basis :848-862), the rows pass (:649-677) and finish (:804-845)."""
+        q: int = PCA_POWER_ITERATIONS, cells=None, leaf_rows: int | None = CELL_BLOCK) -> PcaResult:
+    """kyto's streamed block-Krylov PCA (pca.rs): pass 0 sketch (:588-622), end_pass
+    (:712-801), q product passes (:623-645), the Rayleigh-Ritz step (finish_basis
+    :848-862), the rows pass (:649-677) and finish (:804-845). Each pass folds leaves
+    of `leaf_rows` rows (kyto's store cell blocks) in parallel and combines them in
+    kyto's fixed tree (reduction.rs:30-78); leaf_rows=None folds every row into one
+    leaf, in cell order."""
     L = lib()
     indptr = np.ascontiguousarray(indptr, np.uint64)
     indices = np.ascontiguousarray(indices, np.uint32)
@@ -230,13 +248,18 @@ def pca(indptr, indices, values, n_genes: int, n_components: int = 50, seed: int
         raise ValueError("KY-E215: n_components above min(cells, genes)")
     width = min(n_components + PCA_OVERSAMPLES, n_genes)
     a = (_p(indptr, _u64p), _p(indices, _u32p), _p(values, _f32p), n)
+    tree = leaf_rows is not None and n > leaf_rows
 
     n_cells = np.zeros(1, np.uint64)
     sums = np.zeros(n_genes)
     omega_sums = np.zeros(width)
     sketch = np.zeros((n_genes, width))
-    L.ko_pca_sketch(*a, _p(cells, _u64p), seed, width, _p(n_cells, _u64p), _p(sums, _f64p),
-                    _p(omega_sums, _f64p), _p(sketch, _f64p))
+    if tree:
+        L.ko_pca_sketch_tree(*a, _p(cells, _u64p), seed, width, n_genes, leaf_rows, _p(n_cells, _u64p),
+                             _p(sums, _f64p), _p(omega_sums, _f64p), _p(sketch, _f64p))
+    else:
+        L.ko_pca_sketch(*a, _p(cells, _u64p), seed, width, _p(n_cells, _u64p), _p(sums, _f64p),
+                        _p(omega_sums, _f64p), _p(sketch, _f64p))
     means = sums / float(n)
     if not np.all(np.isfinite(means)):
         raise ValueError("KY-E215: a value that is not finite")
@@ -254,8 +277,12 @@ def pca(indptr, indices, values, n_genes: int, n_components: int = 50, seed: int
         blk = np.ascontiguousarray(block)
         prod = np.zeros((n_genes, w))
         wsum = np.zeros(w)
-        L.ko_pca_product(*a, _p(blk, _f64p), w, _p(block_means, _f64p), _p(prod, _f64p),
-                         _p(wsum, _f64p))
+        if tree:
+            L.ko_pca_product_tree(*a, _p(blk, _f64p), w, _p(block_means, _f64p), n_genes, leaf_rows,
+                                  _p(prod, _f64p), _p(wsum, _f64p))
+        else:
+            L.ko_pca_product(*a, _p(blk, _f64p), w, _p(block_means, _f64p), _p(prod, _f64p),
+                             _p(wsum, _f64p))
         passes += 1
         prod -= means[:, None] * wsum[None, :]
         basis = np.hstack([basis, block])
@@ -272,8 +299,12 @@ def pca(indptr, indices, values, n_genes: int, n_components: int = 50, seed: int
     squares = np.zeros(n_genes)
     stored = np.zeros(n_genes, np.uint64)
     lo = np.ascontiguousarray(loadings)
-    L.ko_pca_rows(*a, _p(lo, _f64p), k, _p(mean_projection, _f64p), _p(means, _f64p),
-                  _p(rows, _f64p), _p(squares, _f64p), _p(stored, _u64p))
+    if tree:
+        L.ko_pca_rows_tree(*a, _p(lo, _f64p), k, _p(mean_projection, _f64p), _p(means, _f64p), n_genes, leaf_rows,
+                           _p(rows, _f64p), _p(squares, _f64p), _p(stored, _u64p))
+    else:
+        L.ko_pca_rows(*a, _p(lo, _f64p), k, _p(mean_projection, _f64p), _p(means, _f64p),
+                      _p(rows, _f64p), _p(squares, _f64p), _p(stored, _u64p))
     total = float(np.sum(squares + (float(n) - stored.astype(np.float64)) * means * means))
     total_variance = total / (n - 1.0)
     denom = n - 1.0
@@ -302,6 +333,34 @@ def pca_truth(indptr, indices, values, n_genes: int, n_components: int):
     v = np.ascontiguousarray(vecs[:, order])
     flip_signs(v)
     return vals[order], v, vals[order] / total
+
+
+def pca_truth_sparse(indptr, indices, values, n_genes: int, n_components: int, parts: int = 16):
+    """The FP64 truth at scale: the centred covariance (X^T X - n mu mu^T) / (n - 1)
+    from the CSR's Gram matrix (ko_gram, no densified rows), its eigh. Returns
+    (variance, loadings genes x k with flip_signs' signs, ratio)."""
+    indptr = np.ascontiguousarray(indptr, np.uint64)
+    indices = np.ascontiguousarray(indices, np.uint32)
+    values = np.ascontiguousarray(values, np.float32)
+    n = len(indptr) - 1
+    gram = np.zeros((n_genes, n_genes))
+    sums = np.zeros(n_genes)
+    lib().ko_gram(_p(indptr, _u64p), _p(indices, _u32p), _p(values, _f32p), n, n_genes, parts, _p(gram, _f64p),
+                  _p(sums, _f64p))
+    mu = sums / n
+    cov = (gram - n * np.outer(mu, mu)) / (n - 1.0)
+    vals, vecs = np.linalg.eigh(cov)
+    order = np.argsort(vals)[::-1][:n_components]
+    v = np.ascontiguousarray(vecs[:, order])
+    flip_signs(v)
+    return vals[order], v, vals[order] / np.trace(cov)
+
+
+def subspace_angles(a, b):
+    """Principal angles (radians, ascending) between the column spans of a and b
+    (orthonormal columns)."""
+    s = np.linalg.svd(np.asarray(a, np.float64).T @ np.asarray(b, np.float64), compute_uv=False)
+    return np.sort(np.arccos(np.clip(s, -1.0, 1.0)))
 
 
 # ------------------------------------------------------------------- neighbours --
@@ -368,6 +427,23 @@ def knn_exact_range(points, k, q_first, q_count, threads=None):
     fb = np.zeros(1, np.uint64)
     L.ko_knn_exact_range(_p(pts, _f32p), n, w, 0, k, q_first, q_count, _p(keys, _f64p), _p(rows, _u32p),
                          _p(fb, _u64p))
+    return keys, rows, int(fb[0])
+
+
+def knn_exact_rows(points, k, queries, metric="euclidean", threads=None):
+    """The filtered exact search for the listed query rows only (sampled rows of a
+    large embedding). Returns (keys f64 len x k, rows u32 len x k, fallback count)."""
+    L = lib()
+    if threads is not None:
+        L.ko_set_threads(int(threads))
+    pts = np.ascontiguousarray(points, np.float32)
+    n, w = pts.shape
+    qs = np.ascontiguousarray(queries, np.uint32)
+    keys = np.zeros((len(qs), k))
+    rows = np.zeros((len(qs), k), np.uint32)
+    fb = np.zeros(1, np.uint64)
+    L.ko_knn_exact_list(_p(pts, _f32p), n, w, 1 if metric == "cosine" else 0, k, _p(qs, _u32p), 0, len(qs),
+                        _p(keys, _f64p), _p(rows, _u32p), _p(fb, _u64p))
     return keys, rows, int(fb[0])
 
 

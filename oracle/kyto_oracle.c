@@ -146,6 +146,137 @@ void ko_pca_rows(const uint64_t *indptr, const uint32_t *indices, const float *v
     }
 }
 
+/* ---------------------------------------------- leaves and the fixed tree ---- */
+/* kyto folds a pass in parallel: one leaf per store cell block (its rows folded in
+ * cell order from an empty leaf), the leaves combined in a tree whose shape depends
+ * only on their number (crates/kyto-exec/src/reduction.rs:1-20, `BlockOrderTree`
+ * :30-78: one leaf as it is, else the tree of the first p leaves plus the tree of the
+ * rest, p the largest power of two below the count), each combine an elementwise
+ * f64 add, left + right (`PcaPartial::combine`, pca.rs:222-292, `add` :295-299). */
+
+static size_t pow2_below(size_t n) {
+    size_t p = 1;
+    while (2 * p < n) p *= 2;
+    return p;
+}
+
+/* The tree of leaves [lo, hi) at element e (leaves `stride` doubles apart). */
+static double tree_sum(const double *leaves, size_t stride, size_t lo, size_t hi, size_t e) {
+    if (hi - lo == 1) return leaves[lo * stride + e];
+    size_t p = pow2_below(hi - lo);
+    double left = tree_sum(leaves, stride, lo, lo + p, e);
+    double right = tree_sum(leaves, stride, lo + p, hi, e);
+    return left + right;
+}
+
+static void tree_combine(const double *leaves, size_t n_leaves, size_t stride, double *out) {
+    #pragma omp parallel for schedule(static)
+    for (size_t e = 0; e < stride; e++) out[e] = tree_sum(leaves, stride, 0, n_leaves, e);
+}
+
+/* Pass 0 over leaves of `leaf_rows` rows folded in parallel (each as ko_pca_sketch
+ * folds one), combined in the tree. Outputs as ko_pca_sketch (overwritten). */
+void ko_pca_sketch_tree(const uint64_t *indptr, const uint32_t *indices, const float *values, size_t n_rows,
+                        const uint64_t *cells, uint64_t seed, size_t l, size_t n_genes, size_t leaf_rows,
+                        uint64_t *n_cells, double *sums, double *omega_sums, double *sketch) {
+    size_t nl = (n_rows + leaf_rows - 1) / leaf_rows;
+    size_t stride = n_genes + l + n_genes * l;
+    double *leaves = (double *)calloc(nl * stride, sizeof(double));
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (size_t b = 0; b < nl; b++) {
+        size_t r0 = b * leaf_rows, r1 = r0 + leaf_rows < n_rows ? r0 + leaf_rows : n_rows;
+        double *lf = leaves + b * stride;
+        uint64_t cnt = 0;
+        ko_pca_sketch(indptr + r0, indices, values, r1 - r0, cells + r0, seed, l, &cnt, lf, lf + n_genes,
+                      lf + n_genes + l);
+    }
+    double *out = (double *)malloc(sizeof(double) * stride);
+    tree_combine(leaves, nl, stride, out);
+    memcpy(sums, out, sizeof(double) * n_genes);
+    memcpy(omega_sums, out + n_genes, sizeof(double) * l);
+    memcpy(sketch, out + n_genes + l, sizeof(double) * n_genes * l);
+    *n_cells = n_rows;
+    free(out);
+    free(leaves);
+}
+
+/* Passes 1..q over leaves in parallel, combined in the tree (as ko_pca_product). */
+void ko_pca_product_tree(const uint64_t *indptr, const uint32_t *indices, const float *values, size_t n_rows,
+                         const double *block, size_t w, const double *block_means, size_t n_genes,
+                         size_t leaf_rows, double *product, double *sums) {
+    size_t nl = (n_rows + leaf_rows - 1) / leaf_rows;
+    size_t stride = n_genes * w + w;
+    double *leaves = (double *)calloc(nl * stride, sizeof(double));
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (size_t b = 0; b < nl; b++) {
+        size_t r0 = b * leaf_rows, r1 = r0 + leaf_rows < n_rows ? r0 + leaf_rows : n_rows;
+        double *lf = leaves + b * stride;
+        ko_pca_product(indptr + r0, indices, values, r1 - r0, block, w, block_means, lf, lf + n_genes * w);
+    }
+    double *out = (double *)malloc(sizeof(double) * stride);
+    tree_combine(leaves, nl, stride, out);
+    memcpy(product, out, sizeof(double) * n_genes * w);
+    memcpy(sums, out + n_genes * w, sizeof(double) * w);
+    free(out);
+    free(leaves);
+}
+
+/* The rows pass over leaves in parallel: the rows as ko_pca_rows gives them, the
+ * squares combined in the tree, the counts summed (integers: any order). */
+void ko_pca_rows_tree(const uint64_t *indptr, const uint32_t *indices, const float *values, size_t n_rows,
+                      const double *loadings, size_t k, const double *mean_projection, const double *means,
+                      size_t n_genes, size_t leaf_rows, double *rows, double *squares, uint64_t *stored) {
+    size_t nl = (n_rows + leaf_rows - 1) / leaf_rows;
+    double *leaves = (double *)calloc(nl * n_genes, sizeof(double));
+    uint64_t *counts = (uint64_t *)calloc(nl * n_genes, sizeof(uint64_t));
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (size_t b = 0; b < nl; b++) {
+        size_t r0 = b * leaf_rows, r1 = r0 + leaf_rows < n_rows ? r0 + leaf_rows : n_rows;
+        ko_pca_rows(indptr + r0, indices, values, r1 - r0, loadings, k, mean_projection, means, rows + r0 * k,
+                    leaves + b * n_genes, counts + b * n_genes);
+    }
+    tree_combine(leaves, nl, n_genes, squares);
+    for (size_t g = 0; g < n_genes; g++) {
+        uint64_t s = 0;
+        for (size_t b = 0; b < nl; b++) s += counts[b * n_genes + g];
+        stored[g] = s;
+    }
+    free(leaves);
+    free(counts);
+}
+
+/* The FP64 truth's input (no kyto counterpart; kyto's tests take the dense
+ * eigendecomposition as the exact PCA, crates/kyto-exec/tests/it/pca.rs:253-304):
+ * G = X^T X (genes x genes, full) and the gene sums, from the CSR, over `parts`
+ * contiguous row ranges in parallel, the parts combined in the same tree. */
+void ko_gram(const uint64_t *indptr, const uint32_t *indices, const float *values, size_t n_rows,
+             size_t n_genes, size_t parts, double *gram, double *sums) {
+    size_t stride = n_genes * n_genes + n_genes;
+    double *leaves = (double *)calloc(parts * stride, sizeof(double));
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (size_t b = 0; b < parts; b++) {
+        size_t r0 = n_rows * b / parts, r1 = n_rows * (b + 1) / parts;
+        double *g = leaves + b * stride, *s = g + n_genes * n_genes;
+        for (size_t r = r0; r < r1; r++) {
+            for (uint64_t i = indptr[r]; i < indptr[r + 1]; i++) {
+                size_t gi = indices[i];
+                double vi = (double)values[i];
+                s[gi] += vi;
+                double *row = g + gi * n_genes;
+                for (uint64_t j = i; j < indptr[r + 1]; j++) row[indices[j]] += vi * (double)values[j];
+            }
+        }
+    }
+    double *out = (double *)malloc(sizeof(double) * stride);
+    tree_combine(leaves, parts, stride, out);
+    for (size_t i = 0; i < n_genes; i++)
+        for (size_t j = 0; j < n_genes; j++)
+            gram[i * n_genes + j] = j >= i ? out[i * n_genes + j] : out[j * n_genes + i];
+    memcpy(sums, out + n_genes * n_genes, sizeof(double) * n_genes);
+    free(out);
+    free(leaves);
+}
+
 /* --------------------------------------------------------------- neighbours ---- */
 
 /* Exact keys: kernels.rs:422-448 (`pair_sum`): component j's difference (or
@@ -307,18 +438,33 @@ static void query_offer(query_t *s, float key, uint32_t row) {
     }
 }
 
-/* The f32 dot products of a block of rows with a tile of rows (kernels/products.rs:21;
- * the bits are not fixed there, and the slack proof makes the result independent
- * of them). Contraction is allowed in this one function. */
-__attribute__((optimize("fp-contract=fast")))
+/* The f32 dot products of a block of rows with a tile of rows (kernels/products.rs:21,
+ * a BLAS sgemm there; the bits are not fixed, and the slack proof makes the result
+ * independent of them). Blocked for the CPU baseline: the tile transposed once
+ * (`pt`, w x np), then each query against 32 points at a time, the 32 sums in vector
+ * registers over the components (each sum still in component order). Contraction
+ * and vectorisation are allowed in this one function. */
+#define PRODUCT_COLS 32
+__attribute__((optimize("O3", "fp-contract=fast")))
 static void f32_products(const float *qs, size_t nq, const float *ps, size_t np, size_t w,
-                         float *out) {
+                         float *pt, float *out) {
+    for (size_t j = 0; j < np; j++)
+        for (size_t c = 0; c < w; c++) pt[c * np + j] = ps[j * w + c];
     for (size_t i = 0; i < nq; i++) {
         const float *a = qs + i * w;
-        for (size_t j = 0; j < np; j++) {
-            const float *b = ps + j * w;
+        size_t j0 = 0;
+        for (; j0 + PRODUCT_COLS <= np; j0 += PRODUCT_COLS) {
+            float acc[PRODUCT_COLS] = {0};
+            for (size_t c = 0; c < w; c++) {
+                const float av = a[c];
+                const float *b = pt + c * np + j0;
+                for (size_t u = 0; u < PRODUCT_COLS; u++) acc[u] += av * b[u];
+            }
+            for (size_t u = 0; u < PRODUCT_COLS; u++) out[i * np + j0 + u] = acc[u];
+        }
+        for (size_t j = j0; j < np; j++) {
             float s = 0.0f;
-            for (size_t c = 0; c < w; c++) s += a[c] * b[c];
+            for (size_t c = 0; c < w; c++) s += a[c] * pt[c * np + j];
             out[i * np + j] = s;
         }
     }
@@ -328,8 +474,8 @@ static void f32_products(const float *qs, size_t nq, const float *ps, size_t np,
  * `Prepared::new` :92-180 (Euclidean and cosine), `search_block` :257-307, `scan`
  * :313-334, `rank` :338-355; the fallback is `search_row`. Returns 1 when the
  * filter ran, 0 when norms forced brute force (`Prepared::new` -> None). */
-int ko_knn_exact_range(const float *pts, size_t n, size_t w, int cosine, size_t k, size_t q_first,
-                       size_t q_count, double *keys_out, uint32_t *rows_out, uint64_t *fallback_rows) {
+int ko_knn_exact_list(const float *pts, size_t n, size_t w, int cosine, size_t k, const uint32_t *list,
+                      size_t q_first, size_t q_count, double *keys_out, uint32_t *rows_out, uint64_t *fallback_rows) {
     double factor = 2.0 * ((double)w + 8.0) * UNIT_ROUNDOFF;
     double underflow = 4.0 * ((double)w + 8.0) * (double)FLT_MIN;
     float *rows = (float *)malloc(sizeof(float) * n * w);
@@ -358,7 +504,7 @@ int ko_knn_exact_range(const float *pts, size_t n, size_t w, int cosine, size_t 
         if (!isfinite(largest) || largest > LARGEST_NORM) {
             free(squares); free(rows); free(slack);
             uint32_t *all = (uint32_t *)malloc(sizeof(uint32_t) * q_count);
-            for (size_t r = 0; r < q_count; r++) all[r] = (uint32_t)(q_first + r);
+            for (size_t r = 0; r < q_count; r++) all[r] = list ? list[q_first + r] : (uint32_t)(q_first + r);
             ko_search_rows(pts, n, w, NULL, k, all, q_count, keys_out, rows_out);
             free(all);
             if (fallback_rows) *fallback_rows = q_count;
@@ -386,23 +532,31 @@ int ko_knn_exact_range(const float *pts, size_t n, size_t w, int cosine, size_t 
     #pragma omp parallel reduction(+ : fallbacks)
     {
         float *products = (float *)malloc(sizeof(float) * QUERY_ROWS * POINT_ROWS);
+        float *ptile = (float *)malloc(sizeof(float) * POINT_ROWS * w);
+        float *qblock = (float *)malloc(sizeof(float) * QUERY_ROWS * w);
+        size_t qrows[QUERY_ROWS];
         query_t states[QUERY_ROWS];
         #pragma omp for schedule(dynamic, 1)
         for (size_t b = 0; b < n_blocks; b++) {
             size_t first = q_first + b * QUERY_ROWS, last = first + QUERY_ROWS < q_end ? first + QUERY_ROWS : q_end;
             size_t nq = last - first;
+            /* The block's query rows (listed rows gathered into one block). */
+            for (size_t i = 0; i < nq; i++) {
+                qrows[i] = list ? list[first + i] : first + i;
+                memcpy(qblock + i * w, rows + qrows[i] * w, sizeof(float) * w);
+            }
             for (size_t i = 0; i < nq; i++) {
                 memset(&states[i], 0, sizeof(query_t));
                 states[i].k = k;
-                states[i].slack = slack[first + i];
+                states[i].slack = slack[qrows[i]];
                 states[i].threshold = INFINITY;
                 states[i].compact_at = 64;
             }
             for (size_t start = 0; start < n; start += POINT_ROWS) {
                 size_t end = start + POINT_ROWS < n ? start + POINT_ROWS : n, np = end - start;
-                f32_products(rows + first * w, nq, rows + start * w, np, w, products);
+                f32_products(qblock, nq, rows + start * w, np, w, ptile, products);
                 for (size_t i = 0; i < nq; i++) {
-                    size_t query = first + i;
+                    size_t query = qrows[i];
                     float *kk = products + i * np;
                     if (!cosine) {
                         float o = offsets[query];
@@ -422,10 +576,10 @@ int ko_knn_exact_range(const float *pts, size_t n, size_t w, int cosine, size_t 
                 }
             }
             for (size_t i = 0; i < nq; i++) {
-                size_t query = first + i;
+                size_t query = qrows[i];
                 query_t *s = &states[i];
-                double *ok = keys_out + (query - q_first) * k;
-                uint32_t *orow = rows_out + (query - q_first) * k;
+                double *ok = keys_out + (first + i - q_first) * k;
+                uint32_t *orow = rows_out + (first + i - q_first) * k;
                 nearest_t nn = {k, 0, ok, orow};
                 int good = !s->overflowed;
                 for (size_t c = 0; good && c < s->ncand; c++) {
@@ -444,10 +598,18 @@ int ko_knn_exact_range(const float *pts, size_t n, size_t w, int cosine, size_t 
             }
         }
         free(products);
+        free(ptile);
+        free(qblock);
     }
     if (fallback_rows) *fallback_rows = fallbacks;
     free(rows); free(offsets); free(slack); free(norms);
     return 1;
+}
+
+/* The filtered exact search for the queries [q_first, q_first + q_count). */
+int ko_knn_exact_range(const float *pts, size_t n, size_t w, int cosine, size_t k, size_t q_first,
+                       size_t q_count, double *keys_out, uint32_t *rows_out, uint64_t *fallback_rows) {
+    return ko_knn_exact_list(pts, n, w, cosine, k, NULL, q_first, q_count, keys_out, rows_out, fallback_rows);
 }
 
 int ko_knn_exact(const float *pts, size_t n, size_t w, int cosine, size_t k, double *keys_out,

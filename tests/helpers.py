@@ -113,3 +113,43 @@ def dense_union(idx, w, n):
     rows = np.repeat(np.arange(n), idx.shape[1])
     W[rows, idx.ravel()] = w.ravel()
     return (W + W.T) - W * W.T
+
+
+class Lcg:
+    """The deterministic generator of kyto's PCA tests (crates/kyto-exec/tests/it/
+    pca.rs:32-45): uniform in [0, 1) from the top 53 bits of a 64-bit LCG."""
+
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.s = (self.s * 6364136223846793005 + 1442695040888963407) & 0xFFFFFFFFFFFFFFFF
+        return (self.s >> 11) / float(1 << 53)
+
+
+def lcg_rows(n_cells, n_genes, rank, noise, seed):
+    """kyto's `data` (pca.rs tests :46-86): scores in [-1, 1) (cells x rank), then
+    loadings (rank x genes) in [-1, 1) times 4 / (1 + component), then each value the
+    f64 sum of score x loading over the components plus noise x U[-1, 1), clamped
+    at 0 and rounded to f32. Draws in that order."""
+    r = Lcg(seed)
+    scores = [[r.next() * 2.0 - 1.0 for _ in range(rank)] for _ in range(n_cells)]
+    loadings = [[(r.next() * 2.0 - 1.0) * (4.0 / (1.0 + c)) for _ in range(n_genes)] for c in range(rank)]
+    out = np.empty((n_cells, n_genes), np.float32)
+    for i in range(n_cells):
+        for g in range(n_genes):
+            v = 0.0
+            for c in range(rank):
+                v += scores[i][c] * loadings[c][g]
+            v += noise * (r.next() * 2.0 - 1.0)
+            out[i, g] = np.float32(max(v, 0.0))
+    return out
+
+
+def dense_to_csr(x):
+    """A dense f32 matrix as CSR (u64 indptr, u32 indices, f32 values), zeros not stored."""
+    x = np.asarray(x, np.float32)
+    rows, cols = np.nonzero(x)
+    indptr = np.zeros(x.shape[0] + 1, np.uint64)
+    np.cumsum(np.bincount(rows, minlength=x.shape[0]), out=indptr[1:])
+    return indptr, cols.astype(np.uint32), x[rows, cols].astype(np.float32)

@@ -156,3 +156,15 @@ def test_scan_tail_split_over_point_segments(device):
     keys, rows, _ = oracle.knn_exact_range(pts, 14, first, count)
     o = np.argsort(rows, axis=1)
     assert np.array_equal(g16.rows()[first:first + count], np.take_along_axis(rows, o, 1))
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("scale", [1e-21, 1e-12])
+def test_tiny_embeddings_stay_exact(device, precision, scale):
+    # Products of components near 1e-21 fall below f32's normal range: the BF16 and
+    # FP32 key bounds carry an absolute term for it, so such rows go to the exact
+    # search rather than being certified on flushed keys. The graph is kyto's exact
+    # graph bit for bit.
+    pts = (clustered(5, 1200, 20, 6, 0.0).astype(np.float64) * scale).astype(np.float32)
+    g = _exact(device, pts, 15, precision)
+    _assert_oracle_graph(g, pts, 15)

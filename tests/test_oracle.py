@@ -300,3 +300,86 @@ def test_connectivities_union_equals_the_dense_formula(ties):
     free = (c.rho > 0) & (c.sigma.astype(np.float64) > floor * (1 + 1e-6))
     assert free.sum() > n // 2
     assert np.abs(s[free] - target).max() < 1e-4
+
+
+def _block_order_tree(leaves):
+    """reduction.rs:30-78 as a binary counter: push each leaf, combining two subtrees
+    of one level as soon as the right one is complete; finish from the right."""
+    stack = []
+    for leaf in leaves:
+        level, right = 0, leaf
+        while stack and stack[-1][0] == level:
+            _, left = stack.pop()
+            right = left + right
+            level += 1
+        stack.append((level, right))
+    _, right = stack.pop()
+    while stack:
+        _, left = stack.pop()
+        right = left + right
+    return right
+
+
+def test_parallel_fold_is_the_leaves_in_kytos_tree():
+    # Leaves of 500 rows (13 of them, a ragged last one), each folded in cell order by
+    # the pinned one-leaf fold, combined in the binary-counter tree: bit for bit.
+    from kyto_amd import synthetic as S
+    L = S.log1p_layer(6100, 80, 4, 0.1, seed=5)
+    n, g, w, leaf = L.n_cells, 80, 7, 500
+    rng = np.random.default_rng(2)
+    q = rng.standard_normal((g, w))
+    bm = rng.standard_normal(w)
+    lib, P = oracle.lib(), oracle._p
+    leaves = []
+    for r0 in range(0, n, leaf):
+        r1 = min(r0 + leaf, n)
+        prod, sums = np.zeros((g, w)), np.zeros(w)
+        ip = np.ascontiguousarray(L.indptr[r0:r1 + 1])
+        lib.ko_pca_product(P(ip, oracle._u64p), P(L.indices, oracle._u32p), P(L.values, oracle._f32p), r1 - r0,
+                           P(q, oracle._f64p), w, P(bm, oracle._f64p), P(prod, oracle._f64p), P(sums, oracle._f64p))
+        leaves.append(np.concatenate([prod.ravel(), sums]))
+    want = _block_order_tree(leaves)
+    prod, sums = np.zeros((g, w)), np.zeros(w)
+    lib.ko_pca_product_tree(P(L.indptr, oracle._u64p), P(L.indices, oracle._u32p), P(L.values, oracle._f32p), n,
+                            P(q, oracle._f64p), w, P(bm, oracle._f64p), g, leaf, P(prod, oracle._f64p),
+                            P(sums, oracle._f64p))
+    got = np.concatenate([prod.ravel(), sums])
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    # A left-to-right combine (a wrong tree) differs in the last bits somewhere.
+    seq = leaves[0].copy()
+    for lf in leaves[1:]:
+        seq = seq + lf
+    assert not np.array_equal(seq.view(np.uint64), want.view(np.uint64))
+    # The whole PCA through the tree stays within rounding of the one-leaf fold.
+    a = oracle.pca(L.indptr, L.indices, L.values, g, 10, seed=0, q=4, leaf_rows=leaf)
+    b = oracle.pca(L.indptr, L.indices, L.values, g, 10, seed=0, q=4, leaf_rows=None)
+    assert np.abs(a.variance - b.variance).max() / b.variance.max() < 1e-12
+
+
+def test_sparse_truth_is_the_dense_truth():
+    # ko_gram's X^T X from the CSR against the dense product; the covariance's
+    # eigenpairs against pca_truth's (which densifies the rows).
+    from kyto_amd import synthetic as S
+    L = S.log1p_layer(3000, 60, 4, 0.1, seed=8)
+    x = np.zeros((L.n_cells, 60))
+    x[np.repeat(np.arange(L.n_cells), np.diff(L.indptr.astype(np.int64))), L.indices.astype(np.int64)] = L.values
+    v1, l1, r1 = oracle.pca_truth(L.indptr, L.indices, L.values, 60, 12)
+    v2, l2, r2 = oracle.pca_truth_sparse(L.indptr, L.indices, L.values, 60, 12, parts=5)
+    assert np.abs(v1 - v2).max() / v1.max() < 1e-12
+    assert np.abs(r1 - r2).max() < 1e-12
+    assert oracle.subspace_angles(l1[:, :5], l2[:, :5]).max() < 1e-6  # arccos near 1: ~sqrt(eps)
+    gram = np.zeros((60, 60))
+    sums = np.zeros(60)
+    lib, P = oracle.lib(), oracle._p
+    lib.ko_gram(P(L.indptr, oracle._u64p), P(L.indices, oracle._u32p), P(L.values, oracle._f32p), L.n_cells, 60, 3,
+                P(gram, oracle._f64p), P(sums, oracle._f64p))
+    assert np.abs(gram - x.T @ x).max() / np.abs(x.T @ x).max() < 1e-13
+    assert np.allclose(sums, x.sum(axis=0), rtol=1e-13)
+
+
+def test_listed_queries_equal_the_full_search():
+    pts = clustered(31, 1500, 12, 9, 0.0)
+    keys, rows, _ = oracle.knn_exact(pts, 14)
+    qs = np.array([1499, 3, 700, 0, 42, 42, 1000], np.uint32)
+    k2, r2, _ = oracle.knn_exact_rows(pts, 14, qs)
+    assert np.array_equal(r2, rows[qs]) and np.array_equal(k2.view(np.uint64), keys[qs].view(np.uint64))
