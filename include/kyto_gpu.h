@@ -37,6 +37,7 @@ extern "C" {
 #define KY_E116 116 /* the path is not a kyto store */
 #define KY_E126 126 /* a stored embedding or graph breaks sections 6 and 7 */
 #define KY_E215 215 /* invalid argument or data (KytoError::InvalidArgument) */
+#define KY_E401 401 /* a plan needs more memory than its limit (KytoError::MemoryLimit) */
 #define KY_E501 501 /* no kernel for the placed step */
 #define KY_E503 503 /* cancelled */
 #define KY_E504 504 /* device not found */
@@ -259,7 +260,9 @@ typedef struct ky_knn_info {
     uint64_t certified;     /* rows whose candidates were proven to hold the k nearest */
     uint64_t fallback;      /* rows searched again exactly against every row */
     double search_ms, rerank_ms, fallback_ms, total_ms;
-    double scan_kernel_ms;  /* BF16: device time of the full candidate scan launch (HIP events) */
+    double scan_kernel_ms;  /* BF16: device time of the candidate scan launches (HIP events) */
+    uint64_t rounds;        /* rounds of queries the candidate search ran in */
+    uint64_t scratch_bytes; /* device scratch of the call (one region, every buffer of the search) */
 } ky_knn_info;
 
 /* Exact neighbours of every row of `points` (its first `n_components` columns), as
@@ -275,6 +278,25 @@ typedef struct ky_knn_info {
 int ky_neighbors(ky_ctx *ctx, const ky_dense *points, uint32_t n_components,
                  uint32_t n_neighbors, int metric, int precision, uint32_t *indices,
                  float *distances, ky_knn_info *info);
+
+/* ky_neighbors with the candidate search run in rounds of `round_rows` queries (0:
+ * every row in one round; rounded down to the kernel's 256 or 64 queries), so its
+ * device scratch (ky_neighbors_bytes) fits a memory limit. The same graph. */
+int ky_neighbors_rounds(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uint32_t n_neighbors,
+                        int metric, int precision, uint64_t round_rows, uint32_t *indices, float *distances,
+                        ky_knn_info *info);
+/* The device memory ky_neighbors_rounds takes beyond its input: its scratch region
+ * (every buffer of the search and the graph) for `rows` rows of an embedding of `cols`
+ * columns, `n_components` of them used (0: all). ctx may be NULL: the device's resident
+ * blocks are then taken as an MI355X's. */
+int ky_neighbors_bytes(ky_ctx *ctx, uint64_t rows, uint32_t cols, uint32_t n_components, uint32_t n_neighbors,
+                       int metric, int precision, uint64_t round_rows, uint64_t *bytes);
+/* The largest round whose bytes fit `limit` (memory_limit's search over block sizes,
+ * crates/kyto-plan/src/memory_limit.rs:119-175): *round_rows and its *bytes; KY_E401
+ * ("the plan needs at least ... bytes") when even the smallest round does not fit,
+ * with *bytes the smallest. */
+int ky_neighbors_plan(ky_ctx *ctx, uint64_t rows, uint32_t cols, uint32_t n_components, uint32_t n_neighbors,
+                      int metric, int precision, uint64_t limit, uint64_t *round_rows, uint64_t *bytes);
 
 /* The candidate search alone (DeviceKernels::nearest_keys, launch_matrix.rs:253-286):
  * for each of the rows [first, first + count) of `points`, its m nearest other rows

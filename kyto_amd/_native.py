@@ -47,7 +47,8 @@ class PcaTiming(ctypes.Structure):
 class KnnInfo(ctypes.Structure):
     _fields_ = [("candidates", c_uint64), ("certified", c_uint64), ("fallback", c_uint64),
                 ("search_ms", c_double), ("rerank_ms", c_double), ("fallback_ms", c_double),
-                ("total_ms", c_double), ("scan_kernel_ms", c_double)]
+                ("total_ms", c_double), ("scan_kernel_ms", c_double), ("rounds", c_uint64),
+                ("scratch_bytes", c_uint64)]
 
 
 class StreamInfo(ctypes.Structure):
@@ -104,6 +105,10 @@ SIGNATURES = {
     "ky_csr_stream": [c_void_p, c_uint64, c_uint32, c_uint64, BlockReader, c_void_p, c_uint64, c_double, c_int,
                       POINTER(c_void_p), POINTER(StreamInfo)],
     "ky_neighbors": [c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_int, u32p, f32p, POINTER(KnnInfo)],
+    "ky_neighbors_rounds": [c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_int, c_uint64, u32p, f32p,
+                            POINTER(KnnInfo)],
+    "ky_neighbors_bytes": [c_void_p, c_uint64, c_uint32, c_uint32, c_uint32, c_int, c_int, c_uint64, u64p],
+    "ky_neighbors_plan": [c_void_p, c_uint64, c_uint32, c_uint32, c_uint32, c_int, c_int, c_uint64, u64p, u64p],
     "ky_nearest_keys": [c_void_p, c_void_p, c_uint32, c_uint64, c_uint64, c_uint32, c_int, f32p, u32p],
     "ky_connectivities": [c_void_p, c_uint64, c_uint32, u32p, f32p, POINTER(c_void_p), f32p, f32p,
                           POINTER(ConnInfo)],

@@ -77,6 +77,46 @@ struct DevBuf {
     size_t bytes() const { return sizeof(T) * n; }
 };
 
+// A bump allocator over one device region (a call's scratch): pieces taken in
+// order, each rounded up to 256 bytes, released back to a mark. The same sequence of
+// takes on a SizeArena (no memory) gives the region's size before it is allocated.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0, high = 0;
+    Arena() = default;
+    Arena(void *b, size_t c) : base(static_cast<char *>(b)), cap(c) {}
+    static size_t piece(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+    template <typename T>
+    T *take(size_t count) {
+        const size_t bytes = piece(sizeof(T) * count);
+        if (used + bytes > cap)
+            throw Error(KY_E505, "device scratch: a call needed more than the " + std::to_string(cap) +
+                                     " bytes planned for it");
+        T *p = reinterpret_cast<T *>(base + used);
+        used += bytes;
+        if (used > high) high = used;
+        return p;
+    }
+    size_t mark() const { return used; }
+    void release(size_t m) { used = m; }
+};
+
+// The sizes of an Arena's takes, for planning (the pointers are null).
+struct SizeArena {
+    size_t used = 0, high = 0;
+    template <typename T>
+    T *take(size_t count) {
+        used += Arena::piece(sizeof(T) * count);
+        if (used > high) high = used;
+        return nullptr;
+    }
+    size_t mark() const { return used; }
+    void release(size_t m) { used = m; }
+};
+
+// Fill `count` 32-bit words with `value` (a kernel: usable where a memset is not).
+void fill_words(hipStream_t st, void *dst, size_t count, uint32_t value);
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -92,9 +132,13 @@ struct Context {
     // Pinned staging for downloads (memory.cpp) and the events of its two halves.
     void *staging = nullptr;
     hipEvent_t staged[2] = {nullptr, nullptr};
+    // Events for kernel timings, made when the context opens.
+    hipEvent_t timer[2] = {nullptr, nullptr};
     ~Context() {
         if (staged[0]) (void)hipEventDestroy(staged[0]);
         if (staged[1]) (void)hipEventDestroy(staged[1]);
+        if (timer[0]) (void)hipEventDestroy(timer[0]);
+        if (timer[1]) (void)hipEventDestroy(timer[1]);
         if (staging) (void)hipHostFree(staging);
     }
 };

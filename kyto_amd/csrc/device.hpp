@@ -70,6 +70,15 @@ __device__ __forceinline__ double exact_sq_distance(const float *a, const float 
     return (h0 + h2) + (h1 + h3);
 }
 
+// kyto's cosine distance from a dot product and the two row norms
+// (crates/kyto-exec/src/neighbors.rs:440-445): 1 when either norm is 0, else
+// 1 - dot / (|q| |p|) clipped to [0, 2], in f64.
+__device__ __forceinline__ double cosine_key(double dot, double left, double right) {
+    if (left == 0.0 || right == 0.0) return 1.0;
+    const double v = 1.0 - dot / (left * right);
+    return v < 0.0 ? 0.0 : (v > 2.0 ? 2.0 : v);
+}
+
 // Ascending bitonic sort of N (power of two, >= 2) (key, row) pairs in LDS by one
 // wavefront, ties broken by row.
 template <typename K>

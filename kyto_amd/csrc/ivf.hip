@@ -565,7 +565,7 @@ void ivf_search(Context &ctx, const ky_ivf &ivf, const Dense &points, uint32_t n
     timer.start();
     DevBuf<uint32_t> g_idx(n * k);
     DevBuf<float> g_dist(n * k);
-    knn_graph(ctx, keys.ptr, rows.ptr, n, k, g_idx.ptr, g_dist.ptr);
+    knn_graph(ctx, keys.ptr, rows.ptr, n, k, KY_METRIC_EUCLIDEAN, g_idx.ptr, g_dist.ptr);
     download_parts(ctx, {{h_idx, g_idx.ptr, sizeof(uint32_t) * n * k}, {h_dist, g_dist.ptr, sizeof(float) * n * k}});
     const double graph_ms = timer.stop();
     if (info) {

@@ -106,6 +106,63 @@ __global__ __launch_bounds__(256) void embed_centre(const float *__restrict__ x,
     }
 }
 
+// Cosine rows (neighbors_exact.rs:76-89, `Prepared` for cosine; neighbors.rs:336-350
+// for the norms): each row's norm |x| = sqrt of kyto's 8-lane f64 dot product (lane t
+// sums components t, t + 8, ... in order, then the fixed tree), its unit row
+// u = f32(x / |x|) (zero for a zero row), sq = |u|^2 in f64, offset f32(sq), the finite
+// flag and the largest sq, as embed_centre. One wavefront a row.
+__global__ __launch_bounds__(256) void embed_unit(const float *__restrict__ x, uint64_t n, uint32_t d, uint32_t dp,
+                                                  float *__restrict__ cent, float *__restrict__ offs,
+                                                  double *__restrict__ sqn, double *__restrict__ norms,
+                                                  unsigned long long *__restrict__ largest,
+                                                  unsigned *__restrict__ bad) {
+    __shared__ double wave_max[4];
+    __shared__ unsigned wave_bad[4];
+    const unsigned lane = lane_id(), wave = threadIdx.x / 64;
+    double mx = 0.0;
+    bool ok = true;
+    for (unsigned i = wave; i < EMBED_ROWS; i += 4) {
+        const uint64_t r = (uint64_t)blockIdx.x * EMBED_ROWS + i;
+        if (r >= n) break;
+        const float v = lane < d ? x[r * d + lane] : 0.0f;
+        ok = ok && isfinite(v);
+        const double prod = (double)v * (double)v;
+        // kyto's lane t (t < 8) on lane t: components t, t + 8, ... added in order.
+        double acc = 0.0;
+#pragma unroll
+        for (int j0 = 0; j0 < 64; j0 += 8) {
+            const double term = __shfl(prod, j0 + (int)(lane & 7));
+            if ((unsigned)j0 + (lane & 7) < d) acc += term;
+        }
+        const double h = acc + __shfl(acc, (int)(lane & 3) + 4);   // lanes 0..3: l_t + l_{t+4}
+        const double hq = h + __shfl(h, (int)(lane & 1) + 2);      // lanes 0, 1: h_t + h_{t+2}
+        const double total = __shfl(hq, 0) + __shfl(hq, 1);
+        const double norm = sqrt(total);
+        const float u = (lane < d && norm != 0.0) ? (float)((double)v / norm) : 0.0f;
+        if (lane < dp) cent[r * dp + lane] = u;
+        double sq = (double)u * (double)u;
+#pragma unroll
+        for (int o = 32; o >= 1; o /= 2) sq += __shfl_xor(sq, o);
+        if (lane == 0) {
+            offs[r] = (float)sq;
+            sqn[r] = sq;
+            norms[r] = norm;
+        }
+        mx = fmax(mx, sq);
+    }
+    const bool wave_ok = __ballot(!ok) == 0;
+    if (lane == 0) {
+        wave_max[wave] = mx;
+        wave_bad[wave] = wave_ok ? 0u : 1u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = fmax(fmax(wave_max[0], wave_max[1]), fmax(wave_max[2], wave_max[3]));
+        atomicMax(largest, (unsigned long long)__double_as_longlong(m));
+        if (wave_bad[0] | wave_bad[1] | wave_bad[2] | wave_bad[3]) atomicOr(bad, 1u);
+    }
+}
+
 // slack[r] = factor (sqrt(sq_r) + largest)^2 + underflow (neighbors_exact.rs:76-89).
 __global__ void embed_slack(const double *__restrict__ sqn, uint64_t n, double largest, double factor,
                             double underflow, double *__restrict__ slack) {
@@ -140,8 +197,9 @@ __global__ void pack_columns(const float *__restrict__ x, uint64_t n, uint32_t d
 // segment b % n_seg for the queries of group b / n_seg, and writes its `keep` pairs at
 // slot i * n_seg + segment (merge_segments then keeps each query's smallest), so a
 // short list of queries still fills the device.
-template <bool EXACT, int DP, int CAP>
+template <bool EXACT, bool COS, int DP, int CAP>
 __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, const float *__restrict__ offs,
+                                                   const double *__restrict__ norms,
                                                    uint64_t n, uint32_t d, const uint32_t *__restrict__ q_list,
                                                    uint64_t q_first, uint64_t q_count, uint32_t keep, uint32_t n_seg,
                                                    const double *__restrict__ seed_keys,
@@ -262,12 +320,17 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
                 }
 #pragma unroll
                 for (int c = 0; c < DP; c++) {
-                    // Past d both sides are 0, and adding t * t = +0.0 leaves the sum as it is.
-                    const double t = qv[c] - pd[c];
-                    l[c & 7] += t * t;
+                    // Past d both sides are 0, and adding +0.0 leaves the sum as it is.
+                    if constexpr (COS) {
+                        l[c & 7] += qv[c] * pd[c];
+                    } else {
+                        const double t = qv[c] - pd[c];
+                        l[c & 7] += t * t;
+                    }
                 }
                 double h0 = l[0] + l[4], h1 = l[1] + l[5], h2 = l[2] + l[6], h3 = l[3] + l[7];
                 double kd = (h0 + h2) + (h1 + h3);
+                if constexpr (COS) kd = cosine_key(kd, norms[qg], norms[valid ? p : p_begin]);
                 key = (K)__double_as_longlong(kd);
             }
             const K tk = tkey[ql];
@@ -358,8 +421,9 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
                                               uint64_t q_count, uint32_t m, uint32_t k,
                                               const double *__restrict__ slack, const double *__restrict__ bf_err,
                                               double bf_err_max, double bf_acc_rel, int floor_mode,
+                                              const double *__restrict__ norms,
                                               double *__restrict__ out_keys, uint32_t *__restrict__ out_rows,
-                                              uint8_t *__restrict__ certified) {
+                                              uint32_t *__restrict__ unc_list, uint32_t *__restrict__ unc_count) {
     const unsigned wave = threadIdx.x / 64, lane = lane_id();
     const uint64_t i = (uint64_t)blockIdx.x * 4 + wave;
     if (i >= q_count) return;
@@ -394,9 +458,15 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 // Past d the difference is 0 and adds +0.0 (no change to the sum).
-                const float p = j0 + t + 8 * u < d ? pv[r][u] : qv[u];
-                const double diff = (double)qv[u] - (double)p;
-                part[r] += diff * diff;
+                if (norms) {
+                    // Cosine: kyto's dot product lanes (past d the product is +0.0).
+                    const float p = j0 + t + 8 * u < d ? pv[r][u] : 0.0f;
+                    part[r] += (double)qv[u] * (double)p;
+                } else {
+                    const float p = j0 + t + 8 * u < d ? pv[r][u] : qv[u];
+                    const double diff = (double)qv[u] - (double)p;
+                    part[r] += diff * diff;
+                }
             }
     }
     // Lane 8g + t keeps the key of candidate 8t + g; every lane of a group ends the
@@ -409,7 +479,8 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
         // ((l0 + l4) + (l2 + l6)) + ((l1 + l5) + (l3 + l7)) over the group's lanes.
         const double h = part[r] + __shfl_xor(part[r], 4, 8);
         const double hq = h + __shfl_xor(h, 2, 8);
-        const double key = hq + __shfl_xor(hq, 1, 8);
+        double key = hq + __shfl_xor(hq, 1, 8);
+        if (norms) key = cosine_key(key, norms[q], norms[lrow[r]]);
         if (t == (unsigned)r) {
             mine = c < cnt ? (unsigned long long)__double_as_longlong(key) : ~0ull;
             mine_row = rowv[r];
@@ -440,6 +511,11 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
                 const double r = s > 0.0 ? sqrt(s) - bf_err[q] - bf_err_max : -1.0;
                 floor = r > 0.0 ? r * r * (1.0 - 1e-12) : -1.0;
             }
+            // Cosine: the keys are half the squared distance between unit rows (u =
+            // x / |x|, a zero row 0), so the cosine floor is half the Euclidean one, less
+            // the f64 rounding of kyto's 1 - dot / (|q| |p|); a zero row's key, 1, lies
+            // above half its distance to any row (0.5 or 0), so the floor still holds.
+            if (norms) floor = 0.5 * floor - 1e-15;
             cert = isfinite(far) && floor > kth;
         }
     }
@@ -447,7 +523,7 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
         out_keys[q * k + lane] = __longlong_as_double((long long)skey);
         out_rows[q * k + lane] = srow;
     }
-    if (lane == 0) certified[q] = cert ? 1 : 0;
+    if (lane == 0 && !cert) unc_list[atomicAdd(unc_count, 1u)] = (uint32_t)q;
 }
 
 // Exact search results for listed rows: copy the scan's output (per list position)
@@ -468,7 +544,7 @@ __global__ void scatter_exact(const uint32_t *__restrict__ list, uint64_t count,
 
 // Per row: its k entries sorted by row index; distance f32(sqrt(key)) (Euclidean).
 __global__ void graph_rows(const double *__restrict__ keys, const uint32_t *__restrict__ rows, uint64_t n, uint32_t k,
-                           uint32_t *__restrict__ g_idx, float *__restrict__ g_dist) {
+                           int cosine, uint32_t *__restrict__ g_idx, float *__restrict__ g_dist) {
     __shared__ uint32_t sr[64][65];
     __shared__ double sk[64][65];
     const unsigned t = threadIdx.x;
@@ -488,21 +564,21 @@ __global__ void graph_rows(const double *__restrict__ keys, const uint32_t *__re
     }
     for (uint32_t e = 0; e < k; e++) {
         g_idx[q * k + e] = sr[t][e];
-        g_dist[q * k + e] = (float)sqrt(sk[t][e]);
+        g_dist[q * k + e] = cosine ? (float)sk[t][e] : (float)sqrt(sk[t][e]);
     }
 }
 
 // ================================================================== host side ====
 
-template <bool EXACT, int CAP>
-static void launch_scan(int dp, hipStream_t st, unsigned blocks, const float *a, const float *offs, uint64_t n,
-                        uint32_t d, const uint32_t *list, uint64_t q_first, uint64_t q_count, uint32_t keep,
-                        uint32_t *out_rows, void *out_keys, uint32_t *out_count, uint32_t n_seg = 1,
-                        const double *seed_keys = nullptr, const uint32_t *seed_rows = nullptr) {
-#define KY_SCAN(DPV)                                                                                   \
-    scan_points<EXACT, DPV, CAP><<<blocks * n_seg, 256, 0, st>>>(a, offs, n, d, list, q_first, q_count, keep, \
-                                                                 n_seg, seed_keys, seed_rows, out_rows, out_keys, \
-                                                                 out_count)
+template <bool EXACT, bool COS, int CAP>
+static void launch_scan(int dp, hipStream_t st, unsigned blocks, const float *a, const float *offs,
+                        const double *norms, uint64_t n, uint32_t d, const uint32_t *list, uint64_t q_first,
+                        uint64_t q_count, uint32_t keep, uint32_t *out_rows, void *out_keys, uint32_t *out_count,
+                        uint32_t n_seg = 1, const double *seed_keys = nullptr, const uint32_t *seed_rows = nullptr) {
+#define KY_SCAN(DPV)                                                                                          \
+    scan_points<EXACT, COS, DPV, CAP><<<blocks * n_seg, 256, 0, st>>>(a, offs, norms, n, d, list, q_first, q_count, \
+                                                                      keep, n_seg, seed_keys, seed_rows, out_rows,  \
+                                                                      out_keys, out_count)
     switch (dp) {
         case 8: KY_SCAN(8); break;
         case 16: KY_SCAN(16); break;
@@ -514,7 +590,7 @@ static void launch_scan(int dp, hipStream_t st, unsigned blocks, const float *a,
     KY_HIP(hipGetLastError());
 }
 
-static int padded_width(uint32_t d) {
+int padded_width(uint32_t d) {
     if (d <= 8) return 8;
     if (d <= 16) return 16;
     if (d <= 32) return 32;
@@ -525,6 +601,8 @@ static int padded_width(uint32_t d) {
 // f32's unit roundoff and the Prepared slack's constants (neighbors_exact.rs:40-43).
 static const double UNIT_ROUNDOFF = 1.0 / 16777216.0;
 static const double LARGEST_NORM = 1e15;
+// Chunks of rows for the f64 column sums of the mean.
+static const uint32_t MEAN_CHUNK = 256;
 
 uint32_t candidates_for(uint32_t k, uint64_t n, int precision) {
     // neighbors_gpu.rs:42-44 (2k + 16, at most every other row); BF16 keeps at least 64.
@@ -534,102 +612,215 @@ uint32_t candidates_for(uint32_t k, uint64_t n, int precision) {
     return (uint32_t)std::min<uint64_t>(m, n - 1);
 }
 
-void knn_prepare(Context &ctx, const float *x, uint64_t n, uint32_t d, KnnPrep &prep) {
+unsigned bf16_resident(const Context &ctx);
+static const unsigned MI355X_BF16_RESIDENT = 3 * 256;  // 3 search blocks on each of 256 CUs
+
+KnnShape knn_shape(const Context *ctx, uint64_t n, uint32_t d, uint32_t k, int metric, int precision, bool list,
+                   uint64_t nq, uint64_t round) {
+    KnnShape s;
+    s.n = n;
+    s.d = d;
+    s.dp = (uint32_t)padded_width(d);
+    s.k = k;
+    s.metric = metric;
+    // The BF16 kernel's augmented rows hold d + 6 <= 64 components; wider embeddings
+    // take their candidates from the FP32 kernel (the graph is the same).
+    s.precision = precision == KY_PRECISION_BF16 && d > 58 ? KY_PRECISION_FP32 : precision;
+    s.m = candidates_for(k, n, s.precision);
+    s.nq = nq;
+    s.list = list;
+    const uint64_t granule = s.precision == KY_PRECISION_BF16 ? 256 : QB;
+    s.round = round == 0 || round >= nq ? nq : std::max<uint64_t>(granule, round / granule * granule);
+    s.resident = ctx ? bf16_resident(*ctx) : MI355X_BF16_RESIDENT;
+    return s;
+}
+
+template <class A>
+void knn_take_prep(A &a, const KnnShape &s, KnnPrep &p) {
+    p.mean = a.template take<double>(s.d);
+    p.cent = a.template take<float>(s.n * s.dp);
+    p.offs = a.template take<float>(s.n);
+    p.sqn = a.template take<double>(s.n);
+    p.slack = a.template take<double>(s.n);
+    p.norms = s.metric == KY_METRIC_COSINE ? a.template take<double>(s.n) : nullptr;
+    p.flags = a.template take<unsigned long long>(2);
+}
+
+static size_t prep_partial_count(const KnnShape &s) {
+    return (size_t)div_up(s.n, MEAN_CHUNK) * s.d * 2 + 2 * s.d;
+}
+
+void knn_prepare(Context &ctx, const KnnShape &s, const float *x, KnnPrep &prep, double *partial) {
     hipStream_t st = ctx.stream;
-    prep.n = n;
-    prep.d = d;
-    prep.dp = padded_width(d);
-    const uint32_t chunk = 256;
-    const uint32_t n_chunks = div_up(n, chunk);
-    DevBuf<double> partial((size_t)n_chunks * d * 2 + 2 * d);
-    prep.mean.alloc(d);
-    embed_col_sums<<<n_chunks, 64, 0, st>>>(x, n, d, d, chunk, partial.ptr);
-    double *total = partial.ptr + (size_t)n_chunks * d * 2 + d;
-    ordered_sum(st, partial.ptr, n_chunks, d, partial.ptr + (size_t)n_chunks * d, total);
-    embed_reduce<<<1, 64, 0, st>>>(total, d, 1, 1.0 / (double)n, prep.mean.ptr);
-    prep.cent.alloc(n * prep.dp);
-    prep.offs.alloc(n);
-    prep.sqn.alloc(n);
-    prep.slack.alloc(n);
-    DevBuf<unsigned long long> largest(1);
-    DevBuf<unsigned> bad(1);
-    KY_HIP(hipMemsetAsync(largest.ptr, 0, sizeof(unsigned long long), st));
-    KY_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned), st));
-    embed_centre<<<div_up(n, EMBED_ROWS), 256, 0, st>>>(x, n, d, d, prep.mean.ptr, prep.dp, prep.cent.ptr, prep.offs.ptr,
-                                                  prep.sqn.ptr, largest.ptr, bad.ptr);
+    const uint64_t n = s.n;
+    const uint32_t d = s.d;
+    fill_words(st, prep.flags, 4, 0u);
+    unsigned long long *largest = prep.flags;
+    unsigned *bad = reinterpret_cast<unsigned *>(prep.flags + 1);
+    if (s.metric == KY_METRIC_COSINE) {
+        fill_words(st, prep.mean, 2 * (size_t)d, 0u);
+        embed_unit<<<div_up(n, EMBED_ROWS), 256, 0, st>>>(x, n, d, s.dp, prep.cent, prep.offs, prep.sqn, prep.norms,
+                                                         largest, bad);
+    } else {
+        const uint32_t n_chunks = div_up(n, MEAN_CHUNK);
+        embed_col_sums<<<n_chunks, 64, 0, st>>>(x, n, d, d, MEAN_CHUNK, partial);
+        double *total = partial + (size_t)n_chunks * d * 2 + d;
+        ordered_sum(st, partial, n_chunks, d, partial + (size_t)n_chunks * d, total);
+        embed_reduce<<<1, 64, 0, st>>>(total, d, 1, 1.0 / (double)n, prep.mean);
+        embed_centre<<<div_up(n, EMBED_ROWS), 256, 0, st>>>(x, n, d, d, prep.mean, s.dp, prep.cent, prep.offs,
+                                                           prep.sqn, largest, bad);
+    }
     KY_HIP(hipGetLastError());
-    unsigned long long lb = 0;
-    unsigned hbad = 0;
-    KY_HIP(hipMemcpyAsync(&lb, largest.ptr, sizeof(lb), hipMemcpyDeviceToHost, st));
-    KY_HIP(hipMemcpyAsync(&hbad, bad.ptr, sizeof(hbad), hipMemcpyDeviceToHost, st));
+    unsigned long long flags[2] = {0, 0};
+    KY_HIP(hipMemcpyAsync(flags, prep.flags, sizeof(flags), hipMemcpyDeviceToHost, st));
     KY_HIP(hipStreamSynchronize(st));
-    prep.finite = hbad == 0;
+    prep.finite = (flags[1] & 0xFFFFFFFFull) == 0;
     double lsq;
-    std::memcpy(&lsq, &lb, sizeof(lsq));
+    std::memcpy(&lsq, &flags[0], sizeof(lsq));
     prep.largest = std::sqrt(lsq);
     prep.filter_ok = std::isfinite(prep.largest) && prep.largest <= LARGEST_NORM;
     const double factor = 2.0 * ((double)d + 8.0) * UNIT_ROUNDOFF;
     const double underflow = 4.0 * ((double)d + 8.0) * (double)FLT_MIN;
-    embed_slack<<<div_up(n, 256), 256, 0, st>>>(prep.sqn.ptr, n, prep.largest, factor, underflow, prep.slack.ptr);
+    embed_slack<<<div_up(n, 256), 256, 0, st>>>(prep.sqn, n, prep.largest, factor, underflow, prep.slack);
     KY_HIP(hipGetLastError());
+}
+
+// The exact search of `count` listed rows: point segments that fill the device (about
+// 2048 blocks), each at least 16 tiles, n_seg * k within the merge's 4096 entries.
+static uint32_t exact_segments(uint64_t n, uint32_t k, uint64_t count) {
+    const unsigned blocks = div_up(count, QB);
+    uint32_t n_seg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(2048, blocks), n / (16 * TP)));
+    return std::min<uint32_t>(n_seg, 4096 / k);
+}
+
+struct ExactBufs {
+    uint32_t *rows = nullptr, *cnt = nullptr, *seg_rows = nullptr, *seg_cnt = nullptr;
+    double *keys = nullptr, *seg_keys = nullptr;
+};
+
+template <class A>
+static void take_exact(A &a, const KnnShape &s, uint64_t count, uint32_t n_seg, ExactBufs &e) {
+    e.rows = a.template take<uint32_t>(count * s.k);
+    e.cnt = a.template take<uint32_t>(count);
+    e.keys = a.template take<double>(count * s.k);
+    if (n_seg > 1) {
+        e.seg_rows = a.template take<uint32_t>(count * n_seg * s.k);
+        e.seg_cnt = a.template take<uint32_t>(count * n_seg);
+        e.seg_keys = a.template take<double>(count * n_seg * s.k);
+    }
+}
+
+// The most bytes an exact-search chunk of at most `most` rows takes (its segments
+// change with the count at multiples of QB).
+static size_t exact_chunk_bytes(const KnnShape &s, uint64_t most) {
+    size_t worst = 0;
+    ExactBufs e;
+    auto at = [&](uint64_t count) {
+        SizeArena a;
+        take_exact(a, s, count, exact_segments(s.n, s.k, count), e);
+        worst = std::max(worst, a.high);
+    };
+    for (uint64_t c = QB; c < most; c += QB) at(c);
+    if (most) at(most);
+    return worst;
+}
+
+template <class A>
+static void take_round(A &a, const KnnShape &s, uint64_t count, KnnRound &r) {
+    r.rows = a.template take<uint32_t>(count * s.m);
+    r.keys = a.template take<float>(count * s.m);
+    r.count = a.template take<uint32_t>(count);
+    r.thr = a.template take<float>(count);
+    if (s.precision == KY_PRECISION_BF16) bf16_take_round(a, s, count, r);
+}
+
+size_t knn_scratch_bytes(const KnnShape &s, bool graph, bool packed) {
+    SizeArena a;
+    if (graph) {
+        if (packed) a.take<float>(s.n * s.d);
+        a.take<double>(s.n * s.k);
+        a.take<uint32_t>(s.n * s.k);
+    }
+    const size_t m0 = a.mark();
+    // exact_search's layout, phase by phase.
+    KnnPrep p;
+    knn_take_prep(a, s, p);
+    a.take<uint32_t>(s.nq);
+    a.take<uint32_t>(1);
+    size_t m1 = a.mark();
+    if (s.metric != KY_METRIC_COSINE) a.take<double>(prep_partial_count(s));
+    a.release(m1);
+    if (s.precision == KY_PRECISION_BF16) {
+        Bf16Points b;
+        bf16_take_points(a, s, b);
+        m1 = a.mark();
+        a.take<unsigned char>(bf16_prepare_bytes(s));
+        a.release(m1);
+    }
+    KnnRound r;
+    take_round(a, s, s.round, r);
+    a.release(m1);
+    if (s.nq % s.round) {
+        take_round(a, s, s.nq % s.round, r);
+        a.release(m1);
+    }
+    a.take<unsigned char>(exact_chunk_bytes(s, std::min(s.round, s.nq)));
+    a.release(m0);
+    if (graph) {
+        a.take<uint32_t>(s.n * s.k);
+        a.take<float>(s.n * s.k);
+    }
+    return a.high;
+}
+
+__global__ void iota_words(uint32_t *__restrict__ out, uint64_t count, uint32_t first) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < count) out[i] = first + (uint32_t)i;
 }
 
 // Exact search of the listed rows against every row; results into keys/rows (n x k).
-void knn_exact_rows(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32_t k, const uint32_t *d_list,
-                    uint64_t count, double *d_keys, uint32_t *d_rows, bool seeded) {
+static void knn_exact_rows(Context &ctx, Arena &a, const KnnShape &s, const float *pts, const double *norms,
+                           const uint32_t *d_list, uint64_t count, double *d_keys, uint32_t *d_rows, bool seeded) {
     if (count == 0) return;
     hipStream_t st = ctx.stream;
-    DevBuf<uint32_t> rows_tmp(count * k), cnt_tmp(count);
-    DevBuf<double> keys_tmp(count * k);
+    const uint64_t n = s.n;
+    const uint32_t d = s.d, k = s.k;
     const unsigned blocks = div_up(count, QB);
-    // Segments of the points: enough blocks to fill the device (about 2048), each
-    // segment at least 16 tiles, n_seg * k within the merge's 4096 entries.
-    uint32_t n_seg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(div_up(2048, blocks), n / (16 * TP)));
-    n_seg = std::min<uint32_t>(n_seg, 4096 / k);
+    const uint32_t n_seg = exact_segments(n, k, count);
+    ExactBufs e;
+    take_exact(a, s, count, n_seg, e);
     const double *sk = seeded ? d_keys : nullptr;
     const uint32_t *sr = seeded ? d_rows : nullptr;
-    if (n_seg <= 1) {
-        launch_scan<true, 128>(padded_width(d), st, blocks, pts, nullptr, n, d, d_list, 0, count, k, rows_tmp.ptr,
-                               keys_tmp.ptr, cnt_tmp.ptr, 1, sk, sr);
-    } else {
-        DevBuf<uint32_t> seg_rows(count * n_seg * k), seg_cnt(count * n_seg);
-        DevBuf<double> seg_keys(count * n_seg * k);
-        launch_scan<true, 128>(padded_width(d), st, blocks, pts, nullptr, n, d, d_list, 0, count, k, seg_rows.ptr,
-                               seg_keys.ptr, seg_cnt.ptr, n_seg, sk, sr);
-        if (n_seg * k <= 1024)
-            merge_segments<1024><<<(unsigned)count, 64, 0, st>>>(seg_rows.ptr, seg_keys.ptr, seg_cnt.ptr, count,
-                                                                  n_seg, k, rows_tmp.ptr, keys_tmp.ptr, cnt_tmp.ptr);
+    const bool cos = norms != nullptr;
+    auto scan = [&](uint32_t *rows, double *keys, uint32_t *cnt, uint32_t segs) {
+        if (cos)
+            launch_scan<true, true, 128>(padded_width(d), st, blocks, pts, nullptr, norms, n, d, d_list, 0, count, k,
+                                         rows, keys, cnt, segs, sk, sr);
         else
-            merge_segments<4096><<<(unsigned)count, 64, 0, st>>>(seg_rows.ptr, seg_keys.ptr, seg_cnt.ptr, count,
-                                                                  n_seg, k, rows_tmp.ptr, keys_tmp.ptr, cnt_tmp.ptr);
+            launch_scan<true, false, 128>(padded_width(d), st, blocks, pts, nullptr, nullptr, n, d, d_list, 0, count,
+                                          k, rows, keys, cnt, segs, sk, sr);
+    };
+    if (n_seg <= 1) {
+        scan(e.rows, e.keys, e.cnt, 1);
+    } else {
+        scan(e.seg_rows, e.seg_keys, e.seg_cnt, n_seg);
+        if (n_seg * k <= 1024)
+            merge_segments<1024><<<(unsigned)count, 64, 0, st>>>(e.seg_rows, e.seg_keys, e.seg_cnt, count, n_seg, k,
+                                                                  e.rows, e.keys, e.cnt);
+        else
+            merge_segments<4096><<<(unsigned)count, 64, 0, st>>>(e.seg_rows, e.seg_keys, e.seg_cnt, count, n_seg, k,
+                                                                  e.rows, e.keys, e.cnt);
         KY_HIP(hipGetLastError());
     }
-    scatter_exact<<<div_up(count * k, 256), 256, 0, st>>>(d_list, count, k, rows_tmp.ptr, keys_tmp.ptr, d_keys, d_rows);
+    scatter_exact<<<div_up(count * k, 256), 256, 0, st>>>(d_list, count, k, e.rows, e.keys, d_keys, d_rows);
     KY_HIP(hipGetLastError());
 }
 
-void knn_candidates(Context &ctx, const KnnPrep &prep, uint64_t q_first, uint64_t q_count, uint32_t m,
-                    uint32_t *d_rows, float *d_keys, uint32_t *d_count) {
-    const unsigned blocks = div_up(q_count, QB);
-    KY_CHECK(m <= 64, KY_E215, "neighbors: more than 64 candidates per row");
-    launch_scan<false, 128>(prep.dp, ctx.stream, blocks, prep.cent.ptr, prep.offs.ptr, prep.n, prep.d, nullptr,
-                            q_first, q_count, m, d_rows, d_keys, d_count);
-}
-
-void knn_rerank(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t q_first,
-                uint64_t q_count, uint32_t m, uint32_t k, const uint32_t *cand_rows, const float *cand_keys,
-                const uint32_t *cand_count, const float *cand_thr, int floor_mode, const double *bf_err,
-                double bf_err_max, double bf_acc_rel, double *d_keys, uint32_t *d_rows, uint8_t *d_cert) {
-    rerank<<<div_up(q_count, 4), 256, 0, ctx.stream>>>(pts, prep.n, prep.d, cand_rows, cand_keys, cand_count, cand_thr, q_rows,
-                                                       q_first, q_count, m, k, prep.slack.ptr, bf_err, bf_err_max,
-                                                       bf_acc_rel, floor_mode, d_keys, d_rows, d_cert);
-    KY_HIP(hipGetLastError());
-}
-
-void knn_graph(Context &ctx, const double *d_keys, const uint32_t *d_rows, uint64_t n, uint32_t k, uint32_t *g_idx,
-               float *g_dist) {
+void knn_graph(Context &ctx, const double *d_keys, const uint32_t *d_rows, uint64_t n, uint32_t k, int metric,
+               uint32_t *g_idx, float *g_dist) {
     KY_CHECK(k <= 64, KY_E215, "neighbors: n_neighbors above 65");
-    graph_rows<<<div_up(n, 64), 64, 0, ctx.stream>>>(d_keys, d_rows, n, k, g_idx, g_dist);
+    graph_rows<<<div_up(n, 64), 64, 0, ctx.stream>>>(d_keys, d_rows, n, k, metric == KY_METRIC_COSINE ? 1 : 0, g_idx,
+                                                     g_dist);
     KY_HIP(hipGetLastError());
 }
 
@@ -639,31 +830,6 @@ void knn_pack(Context &ctx, const float *x, uint64_t n, uint32_t d, uint32_t ld,
 }
 
 // ------------------------------------------------------------------ driver ----
-
-struct EventTimer {
-    hipStream_t st;
-    hipEvent_t a, b;
-    explicit EventTimer(hipStream_t s) : st(s) {
-        KY_HIP(hipEventCreate(&a));
-        KY_HIP(hipEventCreate(&b));
-    }
-    ~EventTimer() {
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-    }
-    void start() { KY_HIP(hipEventRecord(a, st)); }
-    double stop() {
-        KY_HIP(hipEventRecord(b, st));
-        KY_HIP(hipEventSynchronize(b));
-        float ms = 0.0f;
-        KY_HIP(hipEventElapsedTime(&ms, a, b));
-        return ms;
-    }
-};
-
-void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t nq,
-                     uint32_t m, uint32_t k, uint32_t *d_rows, float *d_keys, uint32_t *d_count, float *d_thr,
-                     DevBuf<double> &err, double &err_max, double &acc_rel, double &scan_ms);
 
 // The checks of neighbors::search (neighbors.rs:226-312) that precede the search.
 uint32_t check_search(const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric) {
@@ -678,109 +844,160 @@ uint32_t check_search(const Dense &points, uint32_t n_components, uint32_t n_nei
                  std::to_string(std::max<uint64_t>(points.rows, 2)) + ")");
     KY_CHECK(points.rows - 1 <= 0xFFFFFFFEull, KY_E215, "neighbors: graph indices are uint32 (at most 2^32 cells)");
     KY_CHECK(metric == KY_METRIC_EUCLIDEAN || metric == KY_METRIC_COSINE, KY_E215, "neighbors: unknown metric");
-    KY_CHECK(metric == KY_METRIC_EUCLIDEAN, KY_E501, "neighbors: no device kernel for metric='cosine'");
     KY_CHECK(used <= 64, KY_E501, "neighbors: no device kernel for more than 64 components");
     KY_CHECK(n_neighbors <= 65, KY_E501, "neighbors: no device kernel for n_neighbors above 65");
     return used;
 }
 
-// Exact neighbours (kyto's (key, row) order) of the rows q_rows[0..nq) (every row
-// when q_rows is null, nq = n) of the packed n x d embedding: candidates at
-// `precision`, exact re-rank and certificate, exact search again of the rows the
-// certificate does not cover. keys / rows are n x k, filled at the queried rows.
-void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32_t k, int precision,
-                  const uint32_t *q_rows, uint64_t nq, double *keys, uint32_t *rows, ky_knn_info *info) {
+void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t d, uint32_t k, int metric,
+                  int precision, const uint32_t *q_rows, uint64_t nq, uint64_t round, double *keys, uint32_t *rows,
+                  ky_knn_info *info, bool timed) {
     hipStream_t st = ctx.stream;
-    EventTimer timer(st);
+    const KnnShape s = knn_shape(&ctx, n, d, k, metric, precision, q_rows != nullptr, nq, round);
+    KY_CHECK(s.precision == KY_PRECISION_BF16 || q_rows == nullptr, KY_E501,
+             "neighbors: FP32 candidates for a row subset");
+    const size_t m0 = a.mark();
+    const double t_start = now_ms();
     KnnPrep prep;
-    timer.start();
-    knn_prepare(ctx, pts, n, d, prep);
+    knn_take_prep(a, s, prep);
+    uint32_t *unc = a.take<uint32_t>(nq);
+    uint32_t *unc_count = a.take<uint32_t>(1);
+    size_t m1 = a.mark();
+    knn_prepare(ctx, s, pts, prep, s.metric == KY_METRIC_COSINE ? nullptr : a.take<double>(prep_partial_count(s)));
+    a.release(m1);
     KY_CHECK(prep.finite, KY_E215, "neighbors: the embedding holds a value that is not finite");
-    DevBuf<uint8_t> cert(n);
-    uint32_t m = 0;
     double search_ms = 0, rerank_ms = 0, fallback_ms = 0, scan_ms = 0;
-    std::vector<uint32_t> list;
-    std::vector<uint32_t> hq;
-    if (q_rows) {
-        hq.resize(nq);
-        KY_HIP(hipMemcpyAsync(hq.data(), q_rows, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, st));
-    }
-    // The BF16 kernel's augmented rows hold d + 6 <= 64 components; wider embeddings
-    // take their candidates from the FP32 kernel (the graph is the same).
-    if (precision == KY_PRECISION_BF16 && d > 58) precision = KY_PRECISION_FP32;
+    uint64_t rounds = 0;
+    uint32_t total = 0;
+    fill_words(st, unc_count, 1, 0u);
     if (prep.filter_ok) {
-        m = candidates_for(k, n, precision);
-        DevBuf<uint32_t> c_rows(nq * m), c_count(nq);
-        DevBuf<float> c_keys(nq * m), c_thr;
-        DevBuf<double> err;
-        double err_max = 0.0, acc_rel = 0.0;
-        if (precision == KY_PRECISION_FP32) {
-            KY_CHECK(q_rows == nullptr, KY_E501, "neighbors: FP32 candidates for a row subset");
-            knn_candidates(ctx, prep, 0, n, m, c_rows.ptr, c_keys.ptr, c_count.ptr);
-        } else {
-            c_thr.alloc(nq);
-            bf16_candidates(ctx, pts, prep, q_rows, nq, m, k, c_rows.ptr, c_keys.ptr, c_count.ptr, c_thr.ptr, err,
-                            err_max, acc_rel, scan_ms);
+        Bf16Points b;
+        if (s.precision == KY_PRECISION_BF16) {
+            bf16_take_points(a, s, b);
+            m1 = a.mark();
+            bf16_prepare(ctx, s, pts, prep, q_rows, b, a.take<unsigned char>(bf16_prepare_bytes(s)), k);
+            a.release(m1);
         }
-        search_ms = timer.stop();
-        timer.start();
-        knn_rerank(ctx, pts, prep, q_rows, 0, nq, m, k, c_rows.ptr, c_keys.ptr, c_count.ptr, c_thr.ptr,
-                   precision == KY_PRECISION_FP32 ? 0 : 1, err.ptr, err_max, acc_rel, keys, rows, cert.ptr);
-        std::vector<uint8_t> hcert(n);
-        KY_HIP(hipMemcpyAsync(hcert.data(), cert.ptr, n, hipMemcpyDeviceToHost, st));
-        rerank_ms = timer.stop();
-        for (uint64_t i = 0; i < nq; i++) {
-            uint32_t q = q_rows ? hq[i] : (uint32_t)i;
-            if (!hcert[q]) list.push_back(q);
+        for (uint64_t lo = 0; lo < nq; lo += s.round) {
+            const uint64_t count = std::min<uint64_t>(s.round, nq - lo);
+            KnnRound r;
+            take_round(a, s, count, r);
+            double t0 = now_ms();
+            const uint32_t *round_rows = nullptr;
+            if (s.precision == KY_PRECISION_FP32) {
+                launch_scan<false, false, 128>(s.dp, st, div_up(count, QB), prep.cent, prep.offs, nullptr, n, d,
+                                               nullptr, lo, count, s.m, r.rows, r.keys, r.count);
+            } else {
+                bf16_round(ctx, s, b, lo, count, r, timed ? &scan_ms : nullptr);
+                round_rows = s.list ? b.qorig + lo : b.perm + lo;
+            }
+            if (timed) KY_HIP(hipStreamSynchronize(st));
+            const double t1 = now_ms();
+            search_ms += t1 - t0;
+            rerank<<<div_up(count, 4), 256, 0, st>>>(
+                pts, n, d, r.rows, r.keys, r.count, s.precision == KY_PRECISION_BF16 ? r.thr : nullptr, round_rows,
+                round_rows ? 0 : lo, count, s.m, k, prep.slack, b.err, b.err_max, b.acc_rel,
+                s.precision == KY_PRECISION_FP32 ? 0 : 1, prep.norms, keys, rows, unc, unc_count);
+            KY_HIP(hipGetLastError());
+            // The round's buffers are released for the next: wait for its work.
+            KY_HIP(hipStreamSynchronize(st));
+            rerank_ms += now_ms() - t1;
+            a.release(m1);
+            rounds++;
         }
-    } else {
-        search_ms = timer.stop();
-        for (uint64_t i = 0; i < nq; i++) list.push_back(q_rows ? hq[i] : (uint32_t)i);
-    }
-    timer.start();
-    if (!list.empty()) {
-        DevBuf<uint32_t> d_list(list.size());
-        KY_HIP(hipMemcpyAsync(d_list.ptr, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, st));
-        // Rows that went through the re-rank hold their candidates' k smallest exact
-        // pairs: an upper bound of their k-th pair that seeds the exact search.
-        knn_exact_rows(ctx, pts, n, d, k, d_list.ptr, list.size(), keys, rows, prep.filter_ok);
+        KY_HIP(hipMemcpyAsync(&total, unc_count, sizeof(total), hipMemcpyDeviceToHost, st));
         KY_HIP(hipStreamSynchronize(st));
+    } else {
+        // Norms too large for f32 keys: every query by the exact search.
+        if (q_rows)
+            KY_HIP(hipMemcpyAsync(unc, q_rows, sizeof(uint32_t) * nq, hipMemcpyDeviceToDevice, st));
+        else
+            iota_words<<<div_up(nq, 256), 256, 0, st>>>(unc, nq, 0u);
+        KY_HIP(hipGetLastError());
+        total = (uint32_t)nq;
+        search_ms = now_ms() - t_start;
     }
-    fallback_ms = timer.stop();
+    const double t2 = now_ms();
+    // Rows that went through the re-rank hold their candidates' k smallest exact pairs:
+    // an upper bound of their k-th pair that seeds the exact search.
+    const uint64_t chunk = std::min(s.round, s.nq);
+    for (uint64_t off = 0; off < total; off += chunk) {
+        const size_t mk = a.mark();
+        knn_exact_rows(ctx, a, s, pts, prep.norms, unc + off, std::min<uint64_t>(chunk, total - off), keys, rows,
+                       prep.filter_ok);
+        KY_HIP(hipStreamSynchronize(st));
+        a.release(mk);
+    }
+    fallback_ms = now_ms() - t2;
+    a.release(m0);
     if (info) {
-        info->candidates = m;
-        info->fallback = list.size();
-        info->certified = nq - list.size();
+        info->candidates = prep.filter_ok ? s.m : 0;
+        info->fallback = total;
+        info->certified = nq - total;
         info->search_ms = search_ms;
         info->rerank_ms = rerank_ms;
         info->fallback_ms = fallback_ms;
         info->scan_kernel_ms = scan_ms;
+        info->rounds = rounds;
     }
 }
 
+void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32_t k, int precision,
+                  const uint32_t *q_rows, uint64_t nq, double *keys, uint32_t *rows, ky_knn_info *info) {
+    const KnnShape s = knn_shape(&ctx, n, d, k, KY_METRIC_EUCLIDEAN, precision, q_rows != nullptr, nq, 0);
+    const size_t need = knn_scratch_bytes(s, false, false);
+    DevBuf<unsigned char> region(need);
+    Arena a(region.ptr, need);
+    exact_search(ctx, a, pts, n, d, k, KY_METRIC_EUCLIDEAN, precision, q_rows, nq, 0, keys, rows, info, true);
+}
+
 void neighbors_run(Context &ctx, const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric,
-                   int precision, uint32_t *h_idx, float *h_dist, ky_knn_info *info) {
+                   int precision, uint64_t round, uint32_t *h_idx, float *h_dist, ky_knn_info *info, void *scratch,
+                   size_t scratch_bytes) {
     const double t_start = now_ms();
     const uint32_t d = check_search(points, n_components, n_neighbors, metric);
     KY_CHECK(precision == KY_PRECISION_FP32 || precision == KY_PRECISION_BF16, KY_E215, "neighbors: unknown precision");
     hipStream_t st = ctx.stream;
     const uint64_t n = points.rows;
     const uint32_t k = n_neighbors - 1;
-    DevBuf<float> packed;
-    const float *pts = points.data.ptr;
-    if (d != points.cols) {
-        packed.alloc(n * d);
-        knn_pack(ctx, points.data.ptr, n, d, points.cols, packed.ptr);
-        pts = packed.ptr;
+    const bool packed = d != points.cols;
+    const KnnShape s = knn_shape(&ctx, n, d, k, metric, precision, false, n, round);
+    const size_t need = knn_scratch_bytes(s, true, packed);
+    DevBuf<unsigned char> own;
+    if (scratch) {
+        KY_CHECK(need <= scratch_bytes, KY_E215,
+                 "neighbors: " + std::to_string(n) + " rows of " + std::to_string(d) + " components need " +
+                     std::to_string(need) + " bytes of device scratch, above the " + std::to_string(scratch_bytes) +
+                     " this entry point holds");
+    } else {
+        own.alloc(need);
+        scratch = own.ptr;
     }
-    DevBuf<double> keys(n * k);
-    DevBuf<uint32_t> rows(n * k);
-    exact_search(ctx, pts, n, d, k, precision, nullptr, n, keys.ptr, rows.ptr, info);
-    DevBuf<uint32_t> g_idx(n * k);
-    DevBuf<float> g_dist(n * k);
-    knn_graph(ctx, keys.ptr, rows.ptr, n, k, g_idx.ptr, g_dist.ptr);
-    download_parts(ctx, {{h_idx, g_idx.ptr, sizeof(uint32_t) * n * k}, {h_dist, g_dist.ptr, sizeof(float) * n * k}});
-    if (info) info->total_ms = now_ms() - t_start;
+    Arena a(scratch, need);
+    const float *pts = points.data.ptr;
+    if (packed) {
+        float *p = a.take<float>(n * d);
+        knn_pack(ctx, points.data.ptr, n, d, points.cols, p);
+        pts = p;
+    }
+    double *keys = a.take<double>(n * k);
+    uint32_t *rows = a.take<uint32_t>(n * k);
+    exact_search(ctx, a, pts, n, d, k, metric, precision, nullptr, n, s.round, keys, rows, info, true);
+    uint32_t *g_idx = a.take<uint32_t>(n * k);
+    float *g_dist = a.take<float>(n * k);
+    knn_graph(ctx, keys, rows, n, k, metric, g_idx, g_dist);
+    if (own.ptr) {
+        download_parts(ctx, {{h_idx, g_idx, sizeof(uint32_t) * n * k}, {h_dist, g_dist, sizeof(float) * n * k}});
+    } else {
+        // A caller's scratch: nothing allocated, not even the pinned staging buffer.
+        KY_HIP(hipMemcpyAsync(h_idx, g_idx, sizeof(uint32_t) * n * k, hipMemcpyDeviceToHost, st));
+        KY_HIP(hipMemcpyAsync(h_dist, g_dist, sizeof(float) * n * k, hipMemcpyDeviceToHost, st));
+        KY_HIP(hipStreamSynchronize(st));
+    }
+    if (info) {
+        info->total_ms = now_ms() - t_start;
+        info->scratch_bytes = need;
+    }
 }
 
 void nearest_keys_run(Context &ctx, const Dense &points, uint32_t n_components, uint64_t first, uint64_t count,
@@ -791,22 +1008,35 @@ void nearest_keys_run(Context &ctx, const Dense &points, uint32_t n_components, 
     KY_CHECK(precision == KY_PRECISION_FP32, KY_E501, "nearest_keys: only FP32 keys are exposed");
     hipStream_t st = ctx.stream;
     const uint64_t n = points.rows;
-    DevBuf<float> packed;
+    KnnShape s = knn_shape(&ctx, n, d, 1, KY_METRIC_EUCLIDEAN, KY_PRECISION_FP32, false, count, 0);
+    SizeArena z;
+    KnnPrep zp;
+    if (d != points.cols) z.take<float>(n * d);
+    knn_take_prep(z, s, zp);
+    z.take<double>(prep_partial_count(s));
+    z.take<uint32_t>(count * m);
+    z.take<float>(count * m);
+    z.take<uint32_t>(count);
+    DevBuf<unsigned char> region(z.high);
+    Arena a(region.ptr, z.high);
     const float *pts = points.data.ptr;
     if (d != points.cols) {
-        packed.alloc(n * d);
-        knn_pack(ctx, points.data.ptr, n, d, points.cols, packed.ptr);
-        pts = packed.ptr;
+        float *p = a.take<float>(n * d);
+        knn_pack(ctx, points.data.ptr, n, d, points.cols, p);
+        pts = p;
     }
     KnnPrep prep;
-    knn_prepare(ctx, pts, n, d, prep);
+    knn_take_prep(a, s, prep);
+    knn_prepare(ctx, s, pts, prep, a.take<double>(prep_partial_count(s)));
     KY_CHECK(prep.finite, KY_E215, "nearest_keys: the embedding holds a value that is not finite");
     KY_CHECK(prep.filter_ok, KY_E501, "nearest_keys: row norms too large for f32 keys");
-    DevBuf<uint32_t> c_rows(count * m), c_count(count);
-    DevBuf<float> c_keys(count * m);
-    knn_candidates(ctx, prep, first, count, m, c_rows.ptr, c_keys.ptr, c_count.ptr);
-    KY_HIP(hipMemcpyAsync(h_keys, c_keys.ptr, sizeof(float) * count * m, hipMemcpyDeviceToHost, st));
-    KY_HIP(hipMemcpyAsync(h_rows, c_rows.ptr, sizeof(uint32_t) * count * m, hipMemcpyDeviceToHost, st));
+    uint32_t *c_rows = a.take<uint32_t>(count * m);
+    float *c_keys = a.take<float>(count * m);
+    uint32_t *c_count = a.take<uint32_t>(count);
+    launch_scan<false, false, 128>(s.dp, st, div_up(count, QB), prep.cent, prep.offs, nullptr, n, d, nullptr, first,
+                                   count, m, c_rows, c_keys, c_count);
+    KY_HIP(hipMemcpyAsync(h_keys, c_keys, sizeof(float) * count * m, hipMemcpyDeviceToHost, st));
+    KY_HIP(hipMemcpyAsync(h_rows, c_rows, sizeof(uint32_t) * count * m, hipMemcpyDeviceToHost, st));
     KY_HIP(hipStreamSynchronize(st));
 }
 

@@ -70,6 +70,7 @@ constexpr int QPB = 4 * QPW;    // queries per block
 constexpr unsigned BROWS = 64;
 
 __global__ __launch_bounds__(256) void bf16_rows(const float *__restrict__ x, const double *__restrict__ mean,
+                                                 const double *__restrict__ norms,
                                                  const float *__restrict__ cent, const uint32_t *__restrict__ perm,
                                                  uint64_t n, uint32_t d, uint32_t dp, __bf16 *__restrict__ P,
                                                  __bf16 *__restrict__ Q, double *__restrict__ err,
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(256) void bf16_rows(const float *__restrict__ x, co
                                                  unsigned long long *__restrict__ nb_max) {
     __shared__ double wmax[2][4];
     const unsigned lane = lane_id(), wave = threadIdx.x / 64;
-    const double mu = lane < d ? mean[lane] : 0.0;
+    const double mu = (!norms && lane < d) ? mean[lane] : 0.0;
     double emx = 0.0, nmx = 0.0;
     for (unsigned t = wave; t < BROWS; t += 4) {
         const uint64_t i = (uint64_t)blockIdx.x * BROWS + t;
@@ -88,7 +89,10 @@ __global__ __launch_bounds__(256) void bf16_rows(const float *__restrict__ x, co
         if (lane < d) {
             b = (__bf16)cent[r * dp + lane];
             const double bv = (double)(float)b;
-            const double exact = (double)x[r * d + lane] - mu;
+            // The row b stands for: x - mean (Euclidean), or x / |x| (cosine; 0 for a
+            // zero row), in f64.
+            const double exact = norms ? (norms[r] != 0.0 ? (double)x[r * d + lane] / norms[r] : 0.0)
+                                       : (double)x[r * d + lane] - mu;
             const double tt = bv - exact;
             e2 = tt * tt;
             o = bv * bv;
@@ -201,6 +205,12 @@ __global__ void query_positions(const uint32_t *__restrict__ q_rows, uint64_t nq
     if (i >= nq) return;
     keys[i] = inv[q_rows[i]];
     vals[i] = (uint32_t)i;
+}
+
+// qorig[i] = q_rows[qorig[i]] (list indices made rows).
+__global__ void list_rows(const uint32_t *__restrict__ q_rows, uint64_t nq, uint32_t *__restrict__ qorig) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < nq) qorig[i] = q_rows[qorig[i]];
 }
 
 __global__ void bf16_delta(const double *__restrict__ nb, uint64_t n, double nb_max, double c, double absolute,
@@ -322,11 +332,12 @@ constexpr unsigned MAX_TAIL_SEG = 16;                // point segments of a tail
 constexpr unsigned MERGE_CAP = MAX_TAIL_SEG * 64;    // a tail query's merged lists (m <= 64)
 
 // The candidate scan. Points: np rows, point p being row p * pstride of the scan
-// order (P'). Queries: i < nq, query i being row qpos[i] of the scan order (row i
-// when qpos is null; ascending in i), outputs at slot o = qout[i].
+// order (P'). Queries: i < nq, query i being row qpos[i] of the scan order (row
+// qoff + i when qpos is null; ascending in i), outputs at slot o = i (a round of
+// queries has slots of its own).
 //
 // Each query keeps in its buffer (BCAP entries in device memory) every point whose
-// key is at most its threshold: thr_init[o] at the start (+inf when null), then,
+// key is at most its threshold: thr_init[i] at the start (+inf when null), then,
 // after a compaction, the m-th smallest key so far. Outputs per slot: the m smallest
 // (key, row) sorted (rows as original rows perm[.], when out_rows is not null), their
 // count, and out_thr[o]: the m-th key when m were kept, else the threshold every
@@ -343,7 +354,7 @@ constexpr unsigned MERGE_CAP = MAX_TAIL_SEG * 64;    // a tail query's merged li
 // and read the same tiles through that XCD's L2 at about the same time.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void search_bf16(
     const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q, uint64_t np, uint32_t pstride,
-    const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq,
+    const uint32_t *__restrict__ qpos, uint64_t qoff, uint64_t nq,
     const uint32_t *__restrict__ perm, uint32_t m, const float *__restrict__ thr_init, Entry *__restrict__ buf,
     uint32_t *__restrict__ out_rows, float *__restrict__ out_keys, uint32_t *__restrict__ out_count,
     float *__restrict__ out_thr, uint32_t main_blocks, uint32_t n_seg, uint32_t seg_tiles,
@@ -383,13 +394,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     for (int b = 0; b < NB; b++) {
         const uint64_t i = qbase + 16 * b + (lane % 16);
         const uint64_t ic = min<uint64_t>(i, nq - 1);
-        const uint32_t r = qpos ? qpos[ic] : (uint32_t)ic;
+        const uint32_t r = qpos ? qpos[ic] : (uint32_t)(qoff + ic);
         if (lane < 16) qrow[16 * b + lane] = r;
         const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Q + (uint64_t)r * KDIM);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) bq[b][ks] = src[4 * ks + lane / 16];
         // Slots past the last query take no point.
-        thr[b] = i < nq ? (thr_init ? thr_init[qout[i]] : INFINITY) : -INFINITY;
+        thr[b] = i < nq ? (thr_init ? thr_init[i] : INFINITY) : -INFINITY;
         cb[b] = 0;
     }
     unsigned staged = 0;  // wave-uniform
@@ -417,7 +428,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     };
     const uint64_t n_tiles = (np + TILE - 1) / TILE;
     const uint64_t q0 = min<uint64_t>((uint64_t)lblk * QPB, nq - 1);
-    const uint64_t own = ((qpos ? (uint64_t)qpos[q0] : q0) / pstride) / TILE;
+    const uint64_t own = ((qpos ? (uint64_t)qpos[q0] : qoff + q0) / pstride) / TILE;
     const uint64_t lead = min<uint64_t>(LEAD_TILES, n_tiles - 1);
     // Without a starting threshold a block starts a few tiles before its own queries'
     // position (their near neighbours first tighten the thresholds early); with one,
@@ -627,7 +638,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 kept = compact_query(b, ql % 16, true, stage_key[wave], stage_row[wave]);
                 t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr[b]), ql % 16));
             }
-        const uint64_t o = tail ? (i - main_q) * n_seg + seg : qout[i];
+        const uint64_t o = tail ? (i - main_q) * n_seg + seg : i;
         uint32_t *const o_rows = tail ? seg_rows : out_rows;
         float *const o_keys = tail ? seg_keys : out_keys;
         if (o_rows) {
@@ -649,12 +660,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 // points at or below its threshold (its m-th key, or the starting one), so below the
 // smallest of them, T, the union is whole; its m smallest pairs at or below T are the
 // query's candidates, as a scan of every point would leave them (the threshold the
-// m-th key when m were kept, else T). Output at slot qout[j].
+// m-th key when m were kept, else T). Output at slot slot0 + j.
 __global__ __launch_bounds__(64) void merge_tail(const uint32_t *__restrict__ seg_rows,
                                                  const float *__restrict__ seg_keys,
                                                  const uint32_t *__restrict__ seg_count,
                                                  const float *__restrict__ seg_thr, uint64_t nq, uint32_t n_seg,
-                                                 uint32_t m, const uint32_t *__restrict__ qout,
+                                                 uint32_t m, uint64_t slot0,
                                                  uint32_t *__restrict__ out_rows, float *__restrict__ out_keys,
                                                  uint32_t *__restrict__ out_count, float *__restrict__ out_thr) {
     __shared__ uint32_t skey[MERGE_CAP];
@@ -690,7 +701,7 @@ __global__ __launch_bounds__(64) void merge_tail(const uint32_t *__restrict__ se
     wave_sync();
     wave_bitonic<uint32_t>(skey, srow, N);
     const unsigned kept = min(total, m);
-    const uint64_t o = qout[j];
+    const uint64_t o = slot0 + j;
     for (unsigned e = lane; e < m; e += 64) {
         out_rows[o * m + e] = e < kept ? srow[e] : 0xFFFFFFFFu;
         out_keys[o * m + e] = e < kept ? unorder_f32(skey[e]) : INFINITY;
@@ -720,7 +731,7 @@ constexpr int EST_SUBSETS = 4 * 4 * EST_MB;   // per query: 4 lanes x 4 EST_MB k
 // only steers how much the candidate scan keeps; the candidates do not depend on it.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void estimate_bf16(
     const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q, uint64_t np, uint32_t pstride,
-    const uint32_t *__restrict__ qpos, const uint32_t *__restrict__ qout, uint64_t nq, uint32_t pick,
+    const uint32_t *__restrict__ qpos, uint64_t qoff, uint64_t nq, uint32_t pick,
     float *__restrict__ out_thr) {
     __shared__ __attribute__((aligned(16))) unsigned char tile[3][TILE * ROW_BYTES];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
@@ -731,7 +742,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 #pragma unroll
     for (int b = 0; b < EST_NB; b++) {
         const uint64_t ic = min<uint64_t>(qbase + 16 * b + (lane % 16), nq - 1);
-        const uint32_t r = qpos ? qpos[ic] : (uint32_t)ic;
+        const uint32_t r = qpos ? qpos[ic] : (uint32_t)(qoff + ic);
         const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Q + (uint64_t)r * KDIM);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) bq[b][ks] = src[4 * ks + lane / 16];
@@ -743,7 +754,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         for (int e = 0; e < 4 * EST_MB; e++) rmin[b][e] = INFINITY;
     const uint64_t n_tiles = (np + TILE - 1) / TILE;
     const uint64_t q0 = min<uint64_t>((uint64_t)lblk * EST_QPB, nq - 1);
-    const uint64_t own = ((qpos ? (uint64_t)qpos[q0] : q0) / pstride) / TILE;
+    const uint64_t own = ((qpos ? (uint64_t)qpos[q0] : qoff + q0) / pstride) / TILE;
     // Tiles as in search_bf16: global_load_lds into three buffers, 16-byte chunks
     // XOR-swizzled by row; rows past the last point read P's padding rows.
     auto fetch = [&](uint64_t t0, unsigned char *__restrict__ dst) {
@@ -841,45 +852,131 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             if (s == at) v[s] = INFINITY;
     }
     const uint64_t i = qbase + lane;
-    if (i < nq) out_thr[qout[i]] = t;
+    if (i < nq) out_thr[i] = t;
 }
 
 // ================================================================== host side ====
 
-void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const uint32_t *q_rows, uint64_t nq,
-                     uint32_t m, uint32_t k, uint32_t *d_rows, float *d_keys, uint32_t *d_count, float *d_thr,
-                     DevBuf<double> &err, double &err_max, double &acc_rel, double &scan_ms) {
-    KY_CHECK(prep.d + 6 <= KDIM, KY_E501, "neighbors: no BF16 kernel for more than 58 components");
-    KY_CHECK(m >= 1 && m <= 64, KY_E215, "neighbors: candidates out of range");
+unsigned bf16_resident(const Context &ctx) {
+    int per_cu = 0;
+    KY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, search_bf16, 256, 0));
+    return (unsigned)std::max(per_cu, 1) * (unsigned)std::max(ctx.n_cu, 1);
+}
+
+// P' has PAD_ROWS padding rows: a scan reads whole tiles, up to row
+// (ceil(n / stride) + TILE) * stride.
+constexpr uint64_t PAD_ROWS = (uint64_t)(TILE + 1) * EST_STRIDE;
+
+// The BF16 buffers over the points, kept for the whole search (the list mode's sorted
+// query positions and rows among them).
+template <class A>
+void bf16_take_points(A &a, const KnnShape &s, Bf16Points &b) {
+    b.perm = a.template take<uint32_t>(s.n);
+    b.P = a.template take<__bf16>((s.n + PAD_ROWS) * KDIM);
+    b.Q = a.template take<__bf16>(s.n * KDIM);
+    b.err = a.template take<double>(2 * s.n);
+    b.thr_est = s.n >= EST_MIN_ROWS ? a.template take<float>(s.nq) : nullptr;
+    b.small = a.template take<unsigned long long>(4);
+    if (s.list) {
+        b.qpos = a.template take<uint32_t>(s.nq);
+        b.qorig = a.template take<uint32_t>(s.nq);
+    }
+}
+template void bf16_take_points<Arena>(Arena &, const KnnShape &, Bf16Points &);
+template void bf16_take_points<SizeArena>(SizeArena &, const KnnShape &, Bf16Points &);
+
+// The preparation's temporaries: Morton keys and values, the sort's scratch, |b| per
+// row; for a list, the inverse order and the queries' keys and values.
+template <class A>
+static void bf16_take_temp(A &a, const KnnShape &s, uint32_t *&keys, uint32_t *&vals, uint32_t *&skeys, void *&tmp,
+                           double *&nb, uint32_t *&inv, uint32_t *&k2, uint32_t *&v2) {
+    keys = a.template take<uint32_t>(s.n);
+    vals = a.template take<uint32_t>(s.n);
+    skeys = a.template take<uint32_t>(s.n);
+    tmp = a.template take<unsigned char>(sort_pairs_bytes(std::max(s.n, s.nq)));
+    nb = a.template take<double>(s.n);
+    inv = k2 = v2 = nullptr;
+    if (s.list) {
+        inv = a.template take<uint32_t>(s.n);
+        k2 = a.template take<uint32_t>(s.nq);
+        v2 = a.template take<uint32_t>(s.nq);
+    }
+}
+
+size_t bf16_prepare_bytes(const KnnShape &s) {
+    SizeArena a;
+    uint32_t *k, *v, *sk, *inv, *k2, *v2;
+    void *tmp;
+    double *nb;
+    bf16_take_temp(a, s, k, v, sk, tmp, nb, inv, k2, v2);
+    return a.high;
+}
+
+// The split of one launch's query blocks (see bf16_round): the blocks that scan every
+// point, and the tail's point segments.
+static void bf16_split(const KnnShape &s, uint64_t count, unsigned &main_blocks, unsigned &n_seg,
+                       unsigned &seg_tiles) {
+    const unsigned blocks = div_up(count, QPB);
+    const unsigned resident = std::max<unsigned>(s.resident, 1);
+    const uint64_t n_tiles = div_up(s.n, TILE);
+    main_blocks = blocks;
+    n_seg = 1;
+    seg_tiles = 0;
+    if (blocks > resident && blocks % resident != 0 && 2 * (blocks % resident) <= resident) {
+        const unsigned rem = blocks % resident;
+        main_blocks = blocks - rem;
+        n_seg = (unsigned)std::min<uint64_t>(std::min<unsigned>(MAX_TAIL_SEG, resident / rem), n_tiles);
+        seg_tiles = div_up(n_tiles, n_seg);
+        n_seg = div_up(n_tiles, seg_tiles);  // no empty segment
+    }
+}
+
+template <class A>
+void bf16_take_round(A &a, const KnnShape &s, uint64_t count, KnnRound &r) {
+    bf16_split(s, count, r.main_blocks, r.n_seg, r.seg_tiles);
+    const uint64_t main_q = std::min<uint64_t>(count, (uint64_t)r.main_blocks * QPB);
+    const uint64_t tail_q = count - main_q;
+    r.buf = a.template take<Entry>((main_q + tail_q * r.n_seg) * BCAP);
+    r.seg_rows = a.template take<uint32_t>(tail_q * r.n_seg * s.m);
+    r.seg_keys = a.template take<float>(tail_q * r.n_seg * s.m);
+    r.seg_count = a.template take<uint32_t>(tail_q * r.n_seg);
+    r.seg_thr = a.template take<float>(tail_q * r.n_seg);
+}
+template void bf16_take_round<Arena>(Arena &, const KnnShape &, uint64_t, KnnRound &);
+template void bf16_take_round<SizeArena>(SizeArena &, const KnnShape &, uint64_t, KnnRound &);
+
+// The scan order, the augmented rows, the certificate's per-row terms and the starting
+// thresholds of every query (temporaries in `temp`, bf16_prepare_bytes of them).
+void bf16_prepare(Context &ctx, const KnnShape &s, const float *pts, const KnnPrep &prep, const uint32_t *q_rows,
+                  Bf16Points &b, void *temp, uint32_t k) {
+    KY_CHECK(s.d + 6 <= KDIM, KY_E501, "neighbors: no BF16 kernel for more than 58 components");
+    KY_CHECK(s.m >= 1 && s.m <= 64, KY_E215, "neighbors: candidates out of range");
     hipStream_t st = ctx.stream;
-    const uint64_t n = prep.n;
+    const uint64_t n = s.n;
+    Arena a(temp, bf16_prepare_bytes(s));
+    uint32_t *keys, *vals, *skeys, *inv, *k2, *v2;
+    void *sort_tmp;
+    double *nb;
+    bf16_take_temp(a, s, keys, vals, skeys, sort_tmp, nb, inv, k2, v2);
+    // small: [0] err max, [1] |b| max (f64 bits), [2] the first three components'
+    // absolute maxima (f32 bits, two a word).
+    fill_words(st, b.small, 8, 0u);
+    unsigned *absmax = reinterpret_cast<unsigned *>(b.small + 2);
     // The scan order: rows sorted by the Morton code of their first three components.
-    DevBuf<unsigned> absmax(3);
-    KY_HIP(hipMemsetAsync(absmax.ptr, 0, 3 * sizeof(unsigned), st));
-    const uint32_t c = std::min<uint32_t>(3, prep.d);
-    col_absmax<<<std::min<unsigned>(1024, div_up(n, 256)), 256, 0, st>>>(prep.cent.ptr, n, prep.dp, c, absmax.ptr);
-    DevBuf<uint32_t> keys(n), rows(n), skeys(n), perm(n);
-    morton_keys<<<div_up(n, 256), 256, 0, st>>>(prep.cent.ptr, n, prep.dp, c, absmax.ptr, keys.ptr, rows.ptr);
+    const uint32_t c = std::min<uint32_t>(3, s.d);
+    col_absmax<<<std::min<unsigned>(1024, div_up(n, 256)), 256, 0, st>>>(prep.cent, n, s.dp, c, absmax);
+    morton_keys<<<div_up(n, 256), 256, 0, st>>>(prep.cent, n, s.dp, c, absmax, keys, vals);
     KY_HIP(hipGetLastError());
-    DevBuf<unsigned char> sort_tmp;
-    sort_pairs(st, keys.ptr, skeys.ptr, rows.ptr, perm.ptr, n, 30, sort_tmp);
-    // Rows in scan order.
-    // P' has PAD_ROWS padding rows: a scan reads whole tiles, up to row
-    // (ceil(n / stride) + TILE) * stride.
-    const uint64_t PAD_ROWS = (uint64_t)(TILE + 1) * EST_STRIDE;
-    DevBuf<__bf16> P((n + PAD_ROWS) * KDIM), Q(n * KDIM);
-    pad_rows<<<div_up(PAD_ROWS * KDIM, 256), 256, 0, st>>>(P.ptr, n, PAD_ROWS, prep.d);
-    err.alloc(2 * n);  // err, then delta (original rows)
-    DevBuf<double> nb(n);
-    DevBuf<unsigned long long> maxima(2);
-    KY_HIP(hipMemsetAsync(maxima.ptr, 0, 2 * sizeof(unsigned long long), st));
-    bf16_rows<<<div_up(n, BROWS), 256, 0, st>>>(pts, prep.mean.ptr, prep.cent.ptr, perm.ptr, n, prep.d, prep.dp, P.ptr,
-                                               Q.ptr, err.ptr, nb.ptr, maxima.ptr, maxima.ptr + 1);
+    sort_pairs(st, keys, skeys, vals, b.perm, n, 30, sort_tmp);
+    __bf16 *P = static_cast<__bf16 *>(b.P), *Q = static_cast<__bf16 *>(b.Q);
+    pad_rows<<<div_up(PAD_ROWS * KDIM, 256), 256, 0, st>>>(P, n, PAD_ROWS, s.d);
+    bf16_rows<<<div_up(n, BROWS), 256, 0, st>>>(pts, prep.mean, prep.norms, prep.cent, b.perm, n, s.d, s.dp, P, Q,
+                                               b.err, nb, b.small, b.small + 1);
     KY_HIP(hipGetLastError());
     unsigned long long hm[2];
-    KY_HIP(hipMemcpyAsync(hm, maxima.ptr, sizeof(hm), hipMemcpyDeviceToHost, st));
+    KY_HIP(hipMemcpyAsync(hm, b.small, sizeof(hm), hipMemcpyDeviceToHost, st));
     KY_HIP(hipStreamSynchronize(st));
-    std::memcpy(&err_max, &hm[0], sizeof(double));
+    std::memcpy(&b.err_max, &hm[0], sizeof(double));
     double nb_max;
     std::memcpy(&nb_max, &hm[1], sizeof(double));
     // Accumulation bound of a K-term f32 dot product in any order, doubled:
@@ -892,93 +989,75 @@ void bf16_candidates(Context &ctx, const float *pts, const KnnPrep &prep, const 
     // then all within the bound and its rows go to the exact search.
     const double cb = 2.0 * (KDIM + 8.0) / 16777216.0;
     const double absolute = 4.0 * (KDIM + 8.0) * (double)FLT_MIN;
-    bf16_delta<<<div_up(n, 256), 256, 0, st>>>(nb.ptr, n, nb_max, cb, absolute, err.ptr + n);
+    bf16_delta<<<div_up(n, 256), 256, 0, st>>>(nb, n, nb_max, cb, absolute, b.err + n);
     KY_HIP(hipGetLastError());
-    acc_rel = cb;
-    // Queries in scan order: every row (positions 0..n, outputs at the original row),
-    // or the listed rows sorted by their position (outputs at their list index).
-    DevBuf<uint32_t> qpos_buf, qout_buf;
-    const uint32_t *qpos, *qout;
-    if (!q_rows) {
-        KY_CHECK(nq == n, KY_E215, "neighbors: every row or a list");
-        qpos = nullptr;
-        qout = perm.ptr;
-    } else {
-        DevBuf<uint32_t> inv(n), k2(nq), v2(nq);
-        invert_perm<<<div_up(n, 256), 256, 0, st>>>(perm.ptr, n, inv.ptr);
-        query_positions<<<div_up(nq, 256), 256, 0, st>>>(q_rows, nq, inv.ptr, k2.ptr, v2.ptr);
+    b.acc_rel = cb;
+    // A list of queries: their scan positions, ascending, and their rows in that order.
+    if (s.list) {
+        invert_perm<<<div_up(n, 256), 256, 0, st>>>(b.perm, n, inv);
+        query_positions<<<div_up(s.nq, 256), 256, 0, st>>>(q_rows, s.nq, inv, k2, v2);
         KY_HIP(hipGetLastError());
-        qpos_buf.alloc(nq);
-        qout_buf.alloc(nq);
-        sort_pairs(st, k2.ptr, qpos_buf.ptr, v2.ptr, qout_buf.ptr, nq, 32, sort_tmp);
-        KY_HIP(hipStreamSynchronize(st));
-        qpos = qpos_buf.ptr;
-        qout = qout_buf.ptr;
+        sort_pairs(st, k2, b.qpos, v2, b.qorig, s.nq, 32, sort_tmp);
+        // qorig holds list indices: make them rows.
+        list_rows<<<div_up(s.nq, 256), 256, 0, st>>>(q_rows, s.nq, b.qorig);
+        KY_HIP(hipGetLastError());
     }
-    // Phase 1 (large inputs): each query's starting threshold from the subset minima
-    // over every EST_STRIDE-th row of the scan order (estimate_bf16). Phase 2 starts
-    // every query at that threshold, so its buffer takes only the rows at or below it
-    // (any starting threshold gives the same candidates; a low one only leaves fewer
-    // than m).
-    DevBuf<float> thr_est;
-    const unsigned blocks = div_up(nq, QPB);
-    // The last round of blocks: when the query blocks fill the device's resident
-    // blocks r whole times and a remainder of at most half of them, the remainder
-    // would run almost alone at the end. Its query blocks are split instead over n_seg point segments (blocks
-    // of their own in the same launch, filling the free slots), and merge_tail joins
-    // their lists.
-    int per_cu = 0;
-    KY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, search_bf16, 256, 0));
-    const unsigned resident = (unsigned)std::max(per_cu, 1) * (unsigned)std::max(ctx.n_cu, 1);
-    unsigned main_blocks = blocks, n_seg = 1, seg_tiles = 0;
-    const uint64_t n_tiles = div_up(n, TILE);
-    if (blocks > resident && blocks % resident != 0 && 2 * (blocks % resident) <= resident) {
-        const unsigned rem = blocks % resident;
-        main_blocks = blocks - rem;
-        n_seg = (unsigned)std::min<uint64_t>(std::min<unsigned>(MAX_TAIL_SEG, resident / rem), n_tiles);
-        seg_tiles = div_up(n_tiles, n_seg);
-        n_seg = div_up(n_tiles, seg_tiles);  // no empty segment
-    }
-    const uint64_t main_q = std::min<uint64_t>(nq, (uint64_t)main_blocks * QPB);
-    const uint64_t tail_q = nq - main_q;
-    DevBuf<Entry> buf((main_q + tail_q * n_seg) * BCAP);
-    DevBuf<uint32_t> seg_rows(tail_q * n_seg * m), seg_count(tail_q * n_seg);
-    DevBuf<float> seg_keys(tail_q * n_seg * m), seg_thr(tail_q * n_seg);
-    if (n >= EST_MIN_ROWS) {
-        thr_est.alloc(nq);
+    // Each query's starting threshold from the subset minima over every EST_STRIDE-th
+    // row of the scan order (estimate_bf16); the scan starts every query at it, so its
+    // buffer takes only the rows at or below it (any starting threshold gives the same
+    // candidates; a low one only leaves fewer than m).
+    if (b.thr_est) {
         const uint64_t np = div_up(n, EST_STRIDE);
         uint32_t pick = 1;
         while (pick < EST_SUBSETS - 1 &&
                -std::log1p(-(double)pick / EST_SUBSETS) * EST_SUBSETS * EST_STRIDE < 7.0 * (k + 1))
             pick++;
-        estimate_bf16<<<div_up(nq, EST_QPB), 256, 0, st>>>(P.ptr, Q.ptr, np, EST_STRIDE, qpos, qout, nq, pick,
-                                                            thr_est.ptr);
+        estimate_bf16<<<div_up(s.nq, EST_QPB), 256, 0, st>>>(P, Q, np, EST_STRIDE, b.qpos, 0, s.nq, pick, b.thr_est);
         KY_HIP(hipGetLastError());
     }
-    hipEvent_t ev[2];
-    KY_HIP(hipEventCreate(&ev[0]));
-    KY_HIP(hipEventCreate(&ev[1]));
-    KY_HIP(hipEventRecord(ev[0], st));
-    const unsigned grid = main_blocks + (blocks - main_blocks) * n_seg;
-    search_bf16<<<grid, 256, 0, st>>>(P.ptr, Q.ptr, n, 1, qpos, qout, nq, perm.ptr, m, thr_est.ptr, buf.ptr, d_rows,
-                                      d_keys, d_count, d_thr, main_blocks, n_seg, seg_tiles, seg_rows.ptr,
-                                      seg_keys.ptr, seg_count.ptr, seg_thr.ptr);
+    // The temporaries are released when this returns: finish with them first.
+    KY_HIP(hipStreamSynchronize(st));
+}
+
+// One round: the candidates of the queries at sorted positions [lo, lo + count) (rows
+// perm[lo + i], or the list's qorig[lo + i]) into the round's slots 0..count.
+//
+// The last round of blocks: when the query blocks fill the device's resident blocks r
+// whole times and a remainder of at most half of them, the remainder would run almost
+// alone at the end. Its query blocks are split instead over n_seg point segments
+// (blocks of their own in the same launch, filling the free slots), and merge_tail
+// joins their lists.
+void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t lo, uint64_t count,
+                const KnnRound &r, double *scan_ms) {
+    hipStream_t st = ctx.stream;
+    const unsigned blocks = div_up(count, QPB);
+    const uint64_t main_q = std::min<uint64_t>(count, (uint64_t)r.main_blocks * QPB);
+    const uint64_t tail_q = count - main_q;
+    const unsigned grid = r.main_blocks + (blocks - r.main_blocks) * r.n_seg;
+    const uint32_t *qpos = s.list ? b.qpos + lo : nullptr;
+    const float *thr = b.thr_est ? b.thr_est + lo : nullptr;
+    hipEvent_t ev[2] = {ctx.timer[0], ctx.timer[1]};
+    if (scan_ms) KY_HIP(hipEventRecord(ev[0], st));
+    search_bf16<<<grid, 256, 0, st>>>(static_cast<const __bf16 *>(b.P), static_cast<const __bf16 *>(b.Q), s.n, 1,
+                                      qpos, lo, count, b.perm, s.m, thr, static_cast<Entry *>(r.buf), r.rows,
+                                      r.keys, r.count, r.thr, r.main_blocks, r.n_seg, r.seg_tiles, r.seg_rows,
+                                      r.seg_keys, r.seg_count, r.seg_thr);
     KY_HIP(hipGetLastError());
-    KY_HIP(hipEventRecord(ev[1], st));
+    if (scan_ms) KY_HIP(hipEventRecord(ev[1], st));
     if (tail_q) {
-        merge_tail<<<(unsigned)tail_q, 64, 0, st>>>(seg_rows.ptr, seg_keys.ptr, seg_count.ptr, seg_thr.ptr, tail_q,
-                                                    n_seg, m, qout + main_q, d_rows, d_keys, d_count, d_thr);
+        merge_tail<<<(unsigned)tail_q, 64, 0, st>>>(r.seg_rows, r.seg_keys, r.seg_count, r.seg_thr, tail_q, r.n_seg,
+                                                    s.m, main_q, r.rows, r.keys, r.count, r.thr);
         KY_HIP(hipGetLastError());
     }
     // A query whose starting threshold left fewer than k rows at or below it ends with
     // fewer than k candidates: the re-rank leaves it uncertified and the exact search
     // takes it (knn.hip).
-    KY_HIP(hipStreamSynchronize(st));
-    float ms = 0.0f;
-    KY_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
-    scan_ms = ms;
+    if (scan_ms) {
+        KY_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0.0f;
+        KY_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        *scan_ms += ms;
+    }
 }
 
 }  // namespace ky

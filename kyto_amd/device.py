@@ -215,7 +215,9 @@ class Device:
     # -------------------------------------------------------------- neighbours --
 
     def neighbors(self, points: "DenseMatrix", n_neighbors: int = 15, n_components: int | None = None,
-                  metric: str = "euclidean", precision: str = "fp32") -> Graph:
+                  metric: str = "euclidean", precision: str = "fp32", round_rows: int = 0) -> Graph:
+        """Exact neighbours (ky_neighbors_rounds): the candidate search in rounds of
+        round_rows queries (0: one round)."""
         n = points.rows
         k = int(n_neighbors) - 1
         if metric not in N.METRIC:
@@ -225,11 +227,22 @@ class Device:
         idx = np.empty(max(n * k, 1), np.uint32)
         dist = np.empty(max(n * k, 1), np.float32)
         info = N.KnnInfo()
-        N.check(self._lib.ky_neighbors(self._h, points.handle, int(n_components or 0), int(n_neighbors),
-                                       N.METRIC[metric], N.PRECISION[precision], N.ptr(idx, N.u32p),
-                                       N.ptr(dist, N.f32p), byref(info)))
+        N.check(self._lib.ky_neighbors_rounds(self._h, points.handle, int(n_components or 0), int(n_neighbors),
+                                              N.METRIC[metric], N.PRECISION[precision], int(round_rows),
+                                              N.ptr(idx, N.u32p), N.ptr(dist, N.f32p), byref(info)))
         indptr = np.arange(0, n * k + 1, k, dtype=np.uint64)
         return Graph(int(n_neighbors), indptr, idx[: n * k], dist[: n * k], _info(info))
+
+    def neighbors_bytes(self, rows: int, cols: int, n_neighbors: int = 15, n_components: int | None = None,
+                        metric: str = "euclidean", precision: str = "bf16", round_rows: int = 0) -> int:
+        """Device scratch of a neighbours search (ky_neighbors_bytes)."""
+        return neighbors_bytes(rows, cols, n_neighbors, n_components, metric, precision, round_rows, self._h)
+
+    def neighbors_plan(self, rows: int, cols: int, limit: int, n_neighbors: int = 15, n_components: int | None = None,
+                       metric: str = "euclidean", precision: str = "bf16") -> tuple[int, int]:
+        """(round_rows, bytes) of the largest round that fits `limit` bytes; KytoError
+        KY-E401 when none does (ky_neighbors_plan)."""
+        return neighbors_plan(rows, cols, limit, n_neighbors, n_components, metric, precision, self._h)
 
     def connectivities(self, graph: Graph) -> ConnectivityGraph:
         """scanpy's connectivities of a neighbour graph (ky_connectivities)."""
@@ -274,6 +287,25 @@ class Device:
         info = N.IvfInfo()
         N.check(self._lib.ky_ivf_build(self._h, points.handle, byref(p), byref(h), byref(info)))
         return IvfIndex(self, h, {f: getattr(info, f) for f, _ in N.IvfInfo._fields_})
+
+
+def neighbors_bytes(rows, cols, n_neighbors=15, n_components=None, metric="euclidean", precision="bf16",
+                    round_rows=0, handle=None) -> int:
+    """Device scratch of a neighbours search (ky_neighbors_bytes; no device needed)."""
+    out = c_uint64(0)
+    N.check(N.gpu_lib().ky_neighbors_bytes(handle, int(rows), int(cols), int(n_components or 0), int(n_neighbors),
+                                           N.METRIC[metric], N.PRECISION[precision], int(round_rows), byref(out)))
+    return int(out.value)
+
+
+def neighbors_plan(rows, cols, limit, n_neighbors=15, n_components=None, metric="euclidean", precision="bf16",
+                   handle=None) -> tuple[int, int]:
+    """(round_rows, bytes) of the largest round of a neighbours search whose scratch fits
+    `limit` bytes; KytoError KY-E401 when none does (ky_neighbors_plan)."""
+    rr, b = c_uint64(0), c_uint64(0)
+    N.check(N.gpu_lib().ky_neighbors_plan(handle, int(rows), int(cols), int(n_components or 0), int(n_neighbors),
+                                          N.METRIC[metric], N.PRECISION[precision], int(limit), byref(rr), byref(b)))
+    return int(rr.value), int(b.value)
 
 
 def _info(info) -> dict:

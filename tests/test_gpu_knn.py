@@ -22,8 +22,8 @@ def _exact(device, pts, n_neighbors, precision, n_components=None):
         m.free()
 
 
-def _assert_oracle_graph(g, pts, n_neighbors):
-    _, idx, dist = oracle.neighbors(pts, n_neighbors)
+def _assert_oracle_graph(g, pts, n_neighbors, metric="euclidean"):
+    _, idx, dist = oracle.neighbors(pts, n_neighbors, metric=metric)
     got = g.rows()
     assert np.array_equal(got, idx)
     assert np.array_equal(g.distances.reshape(idx.shape).view(np.uint32), dist.view(np.uint32))
@@ -97,8 +97,8 @@ def test_refusals(device):
         device.neighbors(m, 4)
     with pytest.raises(N.KytoError, match="KY-E215"):
         device.neighbors(m, 2, n_components=3)
-    with pytest.raises(N.KytoError, match="KY-E501"):
-        device.neighbors(m, 2, metric="cosine")
+    with pytest.raises(N.KytoError, match="KY-E215"):
+        device.neighbors(m, 2, metric="manhattan")
     bad = device.upload_dense(np.array([[0.0, np.nan], [1.0, 2.0]], np.float32))
     with pytest.raises(N.KytoError, match="KY-E215"):
         device.neighbors(bad, 2)
@@ -168,3 +168,58 @@ def test_tiny_embeddings_stay_exact(device, precision, scale):
     pts = (clustered(5, 1200, 20, 6, 0.0).astype(np.float64) * scale).astype(np.float32)
     g = _exact(device, pts, 15, precision)
     _assert_oracle_graph(g, pts, 15)
+
+
+def _cosine_cases():
+    rng = np.random.default_rng(11)
+    pts = clustered(12, 2500, 30, 7, 0.0)
+    with_zeros = pts.copy()
+    with_zeros[::97] = 0.0                       # zero rows: key 1 to every row
+    with_zeros[5::131] = with_zeros[4::131][:len(with_zeros[5::131])]  # duplicated rows
+    return [
+        (load("pbmc3k_pca.npz")["embedding"][:, :50].copy(), 15),
+        (with_zeros, 15),
+        ((pts.astype(np.float64) * 1e-20).astype(np.float32), 10),  # cosine ignores the scale
+        (clustered(13, 1500, 8, 4, 50.0), 6),        # far from the origin: keys near 0
+        (np.floor(rng.random((3000, 3)) * 3).astype(np.float32), 12),  # ties, zero rows
+    ]
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_cosine_graph_equals_oracle(device, precision):
+    # metric='cosine' (neighbors.rs:440-445): kyto's f64 key 1 - dot / (|q| |p|) in its
+    # lane order, clipped to [0, 2], 1 for a zero row; distances f32(key).
+    for pts, nn in _cosine_cases():
+        m = device.upload_dense(pts)
+        try:
+            g = device.neighbors(m, nn, metric="cosine", precision=precision)
+        finally:
+            m.free()
+        _assert_oracle_graph(g, pts, nn, metric="cosine")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_rounds_give_the_same_graph(device, precision):
+    # The candidate search in rounds (a memory limit's smaller scratch): the same graph,
+    # bit for bit, and the scratch the call reports is the one planned for it.
+    pts = _large_clustered(140_000, 50, 12, seed=31)
+    m = device.upload_dense(pts)
+    try:
+        whole = device.neighbors(m, 15, precision=precision)
+        for round_rows in (50_000, 12_800):
+            g = device.neighbors(m, 15, precision=precision, round_rows=round_rows)
+            assert g.info["rounds"] == -(-140_000 // round_rows)
+            assert np.array_equal(g.rows(), whole.rows())
+            assert np.array_equal(g.distances.view(np.uint32), whole.distances.view(np.uint32))
+            assert g.info["scratch_bytes"] == device.neighbors_bytes(140_000, 50, 15, precision=precision,
+                                                                     round_rows=round_rows)
+        assert whole.info["scratch_bytes"] > g.info["scratch_bytes"]
+        cos_whole = device.neighbors(m, 15, metric="cosine", precision=precision)
+        cos_rounds = device.neighbors(m, 15, metric="cosine", precision=precision, round_rows=30_000)
+        assert np.array_equal(cos_rounds.rows(), cos_whole.rows())
+        rows = np.arange(0, 140_000, 997, dtype=np.uint32)
+        keys, want, _ = oracle.knn_exact_rows(pts, 14, rows, metric="cosine")
+        o = np.argsort(want, axis=1)
+        assert np.array_equal(cos_whole.rows()[rows], np.take_along_axis(want, o, 1))
+    finally:
+        m.free()
