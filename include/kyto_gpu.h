@@ -298,6 +298,22 @@ int ky_neighbors_bytes(ky_ctx *ctx, uint64_t rows, uint32_t cols, uint32_t n_com
 int ky_neighbors_plan(ky_ctx *ctx, uint64_t rows, uint32_t cols, uint32_t n_components, uint32_t n_neighbors,
                       int metric, int precision, uint64_t limit, uint64_t *round_rows, uint64_t *bytes);
 
+/* Bytes of the fixed device region ky_neighbors_static works in (one a device). */
+#define KY_STATIC_SCRATCH_BYTES (64u << 20)
+/* No counterpart in the reference: ky_neighbors for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Every buffer of the search and the graph lives in a fixed device
+ * array of KY_STATIC_SCRATCH_BYTES per device, set up by ky_open and held until the
+ * process ends; the call allocates nothing (no hipMalloc, no pinned staging: the graph
+ * is copied straight into the caller's host arrays) and runs its candidate search in
+ * the largest round of queries that fits the array. KY_E215, with the bytes needed,
+ * when even the smallest round does not fit (at 50 components and n_neighbors 15,
+ * embeddings up to about 60,000 rows fit). The same graph as ky_neighbors. Calls that
+ * overlap on one device (two threads, two contexts) share the array and race: one at
+ * a time a device. */
+int ky_neighbors_static(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uint32_t n_neighbors,
+                        int metric, int precision, uint32_t *indices, float *distances, ky_knn_info *info);
+
 /* The candidate search alone (DeviceKernels::nearest_keys, launch_matrix.rs:253-286):
  * for each of the rows [first, first + count) of `points`, its m nearest other rows
  * by the approximate key at `precision`, ascending (keys f32 count x m, rows u32

@@ -15,10 +15,13 @@ SYNTH_LIB = os.path.join(HERE, "libkyto_synth.so")
 
 KY_OK = 0
 KY_E215 = 215
+KY_E401 = 401
 KY_E501 = 501
 KY_E503 = 503
 KY_E504 = 504
 KY_E505 = 505
+# include/kyto_gpu.h: the fixed device region of ky_neighbors_static.
+KY_STATIC_SCRATCH_BYTES = 64 << 20
 
 METRIC = {"euclidean": 0, "cosine": 1}
 PRECISION = {"fp32": 0, "bf16": 1}
@@ -107,6 +110,7 @@ SIGNATURES = {
     "ky_neighbors": [c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_int, u32p, f32p, POINTER(KnnInfo)],
     "ky_neighbors_rounds": [c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_int, c_uint64, u32p, f32p,
                             POINTER(KnnInfo)],
+    "ky_neighbors_static": [c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_int, u32p, f32p, POINTER(KnnInfo)],
     "ky_neighbors_bytes": [c_void_p, c_uint64, c_uint32, c_uint32, c_uint32, c_int, c_int, c_uint64, u64p],
     "ky_neighbors_plan": [c_void_p, c_uint64, c_uint32, c_uint32, c_uint32, c_int, c_int, c_uint64, u64p, u64p],
     "ky_nearest_keys": [c_void_p, c_void_p, c_uint32, c_uint64, c_uint64, c_uint32, c_int, f32p, u32p],

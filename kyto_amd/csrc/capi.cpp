@@ -105,6 +105,8 @@ int ky_open(int device, ky_ctx **out) {
             KY_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
             KY_HIP(hipEventCreate(&ctx->timer[0]));
             KY_HIP(hipEventCreate(&ctx->timer[1]));
+            // The device address of ky_neighbors_static's fixed region (once a device).
+            static_scratch_init(device);
         } catch (...) {
             delete ctx;
             throw;
@@ -378,6 +380,18 @@ int ky_neighbors_rounds(ky_ctx *ctx, const ky_dense *points, uint32_t n_componen
     });
 }
 
+int ky_neighbors_static(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uint32_t n_neighbors, int metric,
+                        int precision, uint32_t *indices, float *distances, ky_knn_info *info) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        need(points, "points");
+        need(indices, "indices");
+        need(distances, "distances");
+        use_device(*ctx);
+        neighbors_static(*ctx, *points, n_components, n_neighbors, metric, precision, indices, distances, info);
+    });
+}
+
 // The shape of a search over rows x cols (the checks of check_search on the sizes).
 static KnnShape search_shape(ky_ctx *ctx, uint64_t rows, uint32_t cols, uint32_t n_components, uint32_t n_neighbors,
                              int metric, int precision, uint64_t round_rows, bool &packed) {
@@ -408,35 +422,11 @@ int ky_neighbors_plan(ky_ctx *ctx, uint64_t rows, uint32_t cols, uint32_t n_comp
         need(bytes, "bytes");
         bool packed = false;
         KnnShape s = search_shape(ctx, rows, cols, n_components, n_neighbors, metric, precision, 0, packed);
-        const uint64_t granule = s.precision == KY_PRECISION_BF16 ? 256 : 64;
-        auto bytes_at = [&](uint64_t round) {
-            KnnShape t = knn_shape(ctx, s.n, s.d, s.k, metric, precision, false, s.n, round);
-            return (uint64_t)knn_scratch_bytes(t, true, packed);
-        };
-        // Every row in one round, else the largest round that fits (bisection over
-        // multiples of the granule; the bytes grow with the round).
-        const uint64_t whole = bytes_at(0);
-        if (whole <= limit) {
-            *round_rows = s.n;
-            *bytes = whole;
-            return;
-        }
-        const uint64_t smallest = bytes_at(granule);
-        if (smallest > limit) {
-            *bytes = smallest;
-            *round_rows = 0;
-            throw Error(KY_E401, "the plan needs at least " + std::to_string(smallest) +
-                                     " bytes but memory_limit is " + std::to_string(limit) +
-                                     " bytes (neighbors of " + std::to_string(rows) + " rows)");
-        }
-        uint64_t lo = 1, hi = (s.n + granule - 1) / granule;  // lo fits, hi does not
-        while (hi - lo > 1) {
-            const uint64_t mid = lo + (hi - lo) / 2;
-            if (bytes_at(mid * granule) <= limit) lo = mid;
-            else hi = mid;
-        }
-        *round_rows = lo * granule;
-        *bytes = bytes_at(lo * granule);
+        *round_rows = knn_plan_round(ctx, s.n, s.d, s.k, metric, precision, packed, limit, bytes);
+        if (*round_rows == 0)
+            throw Error(KY_E401, "the plan needs at least " + std::to_string(*bytes) + " bytes but memory_limit is " +
+                                     std::to_string(limit) + " bytes (neighbors of " + std::to_string(rows) +
+                                     " rows)");
     });
 }
 

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -951,9 +952,74 @@ void exact_search(Context &ctx, const float *pts, uint64_t n, uint32_t d, uint32
     exact_search(ctx, a, pts, n, d, k, KY_METRIC_EUCLIDEAN, precision, q_rows, nq, 0, keys, rows, info, true);
 }
 
+uint64_t knn_plan_round(const Context *ctx, uint64_t n, uint32_t d, uint32_t k, int metric, int precision, bool packed,
+                        uint64_t limit, uint64_t *bytes) {
+    const KnnShape s = knn_shape(ctx, n, d, k, metric, precision, false, n, 0);
+    const uint64_t granule = s.precision == KY_PRECISION_BF16 ? 256 : QB;
+    auto bytes_at = [&](uint64_t round) {
+        return (uint64_t)knn_scratch_bytes(knn_shape(ctx, n, d, k, metric, precision, false, n, round), true, packed);
+    };
+    // Every row in one round, else the largest round that fits (bisection over
+    // multiples of the granule; every round it settles on fits).
+    *bytes = bytes_at(0);
+    if (*bytes <= limit) return n;
+    *bytes = bytes_at(granule);
+    if (*bytes > limit) return 0;
+    uint64_t lo = 1, hi = (n + granule - 1) / granule;  // lo fits, hi does not
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (bytes_at(mid * granule) <= limit) lo = mid;
+        else hi = mid;
+    }
+    *bytes = bytes_at(lo * granule);
+    return lo * granule;
+}
+
+// ------------------------------------------------- the fixed scratch region ----
+// The scratch of ky_neighbors_static: one array a device, held from the library's
+// first use on that device until the process ends. Host code never passes the array's
+// name to a kernel (on the host it stands for no device address): its device address
+// is asked for once per device, with that device current, and kept.
+static __device__ __align__(256) unsigned char scratch[KY_STATIC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static void *g_scratch_ptr[MAX_DEVICES] = {};
+
+void static_scratch_init(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    KY_CHECK(device >= 0 && device < MAX_DEVICES, KY_E504, "device " + std::to_string(device) + " out of range");
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    KY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    KY_CHECK(p != nullptr, KY_E505, "device scratch: no device address for the fixed array");
+    g_scratch_ptr[device] = p;
+}
+
+static void *static_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    KY_CHECK(device >= 0 && device < MAX_DEVICES && g_scratch_ptr[device], KY_E505,
+             "device scratch: the context's device has no fixed array (ky_open sets it up)");
+    return g_scratch_ptr[device];
+}
+
+void neighbors_static(Context &ctx, const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric,
+                      int precision, uint32_t *h_idx, float *h_dist, ky_knn_info *info) {
+    const uint32_t d = check_search(points, n_components, n_neighbors, metric);
+    KY_CHECK(precision == KY_PRECISION_FP32 || precision == KY_PRECISION_BF16, KY_E215, "neighbors: unknown precision");
+    uint64_t bytes = 0;
+    const uint64_t round = knn_plan_round(&ctx, points.rows, d, n_neighbors - 1, metric, precision,
+                                          d != points.cols, KY_STATIC_SCRATCH_BYTES, &bytes);
+    KY_CHECK(round > 0, KY_E215,
+             "neighbors: " + std::to_string(points.rows) + " rows of " + std::to_string(d) +
+                 " components need at least " + std::to_string(bytes) + " bytes of device scratch, above the " +
+                 std::to_string((uint64_t)KY_STATIC_SCRATCH_BYTES) + " of the fixed region (use ky_neighbors)");
+    neighbors_run(ctx, points, n_components, n_neighbors, metric, precision, round, h_idx, h_dist, info,
+                  static_scratch(ctx.device), KY_STATIC_SCRATCH_BYTES);
+}
+
 void neighbors_run(Context &ctx, const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric,
-                   int precision, uint64_t round, uint32_t *h_idx, float *h_dist, ky_knn_info *info, void *scratch,
-                   size_t scratch_bytes) {
+                   int precision, uint64_t round, uint32_t *h_idx, float *h_dist, ky_knn_info *info, void *region,
+                   size_t region_bytes) {
     const double t_start = now_ms();
     const uint32_t d = check_search(points, n_components, n_neighbors, metric);
     KY_CHECK(precision == KY_PRECISION_FP32 || precision == KY_PRECISION_BF16, KY_E215, "neighbors: unknown precision");
@@ -964,16 +1030,16 @@ void neighbors_run(Context &ctx, const Dense &points, uint32_t n_components, uin
     const KnnShape s = knn_shape(&ctx, n, d, k, metric, precision, false, n, round);
     const size_t need = knn_scratch_bytes(s, true, packed);
     DevBuf<unsigned char> own;
-    if (scratch) {
-        KY_CHECK(need <= scratch_bytes, KY_E215,
+    if (region) {
+        KY_CHECK(need <= region_bytes, KY_E215,
                  "neighbors: " + std::to_string(n) + " rows of " + std::to_string(d) + " components need " +
-                     std::to_string(need) + " bytes of device scratch, above the " + std::to_string(scratch_bytes) +
+                     std::to_string(need) + " bytes of device scratch, above the " + std::to_string(region_bytes) +
                      " this entry point holds");
     } else {
         own.alloc(need);
-        scratch = own.ptr;
+        region = own.ptr;
     }
-    Arena a(scratch, need);
+    Arena a(region, need);
     const float *pts = points.data.ptr;
     if (packed) {
         float *p = a.take<float>(n * d);

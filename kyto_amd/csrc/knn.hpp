@@ -104,12 +104,21 @@ void knn_graph(Context &ctx, const double *d_keys, const uint32_t *d_rows, uint6
                uint32_t *g_idx, float *g_dist);
 void knn_pack(Context &ctx, const float *x, uint64_t n, uint32_t d, uint32_t ld, float *out);
 
-// neighbors(): the graph on the host. scratch / scratch_bytes: a caller's device region
+// neighbors(): the graph on the host. region / region_bytes: a caller's device region
 // (nothing allocated, the graph downloaded straight into the host arrays), or null for
 // a region of the call's own.
 void neighbors_run(Context &ctx, const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric,
                    int precision, uint64_t round, uint32_t *h_idx, float *h_dist, ky_knn_info *info,
-                   void *scratch = nullptr, size_t scratch_bytes = 0);
+                   void *region = nullptr, size_t region_bytes = 0);
+// The largest round of queries whose scratch (with the graph) fits `limit` bytes (every
+// row: n), or 0 when none does; *bytes: that round's, or the smallest round's.
+uint64_t knn_plan_round(const Context *ctx, uint64_t n, uint32_t d, uint32_t k, int metric, int precision, bool packed,
+                        uint64_t limit, uint64_t *bytes);
+// The fixed scratch region of the current device (set up by ky_open).
+void static_scratch_init(int device);
+// neighbors_run in the fixed region, in the largest round that fits it.
+void neighbors_static(Context &ctx, const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric,
+                      int precision, uint32_t *h_idx, float *h_dist, ky_knn_info *info);
 void nearest_keys_run(Context &ctx, const Dense &points, uint32_t n_components, uint64_t first, uint64_t count,
                       uint32_t m, int precision, float *h_keys, uint32_t *h_rows);
 
