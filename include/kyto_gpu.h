@@ -70,6 +70,27 @@ int ky_close(ky_ctx *ctx);
 /* Device name and total memory, for explain()/provenance (device.rs:24-64). */
 int ky_device_info(ky_ctx *ctx, char *name, size_t name_len, uint64_t *total_bytes);
 
+/* -------------------------------------------------------------- run control ---- */
+/* kyto's RunControl (crates/kyto-exec/src/run_control.rs:1-80) on a context: a run's
+ * calls check a cancel flag before each pass, streamed block and round of queries and
+ * stop with KY_E503; they count the cell rows they computed (summed over the passes).
+ * ky_cancel and ky_progress may be called from any thread while another uses the
+ * context. */
+int ky_run_begin(ky_ctx *ctx, uint64_t total_rows); /* clears the flag, done = 0 */
+int ky_cancel(ky_ctx *ctx);
+int ky_progress(const ky_ctx *ctx, uint64_t *done, uint64_t *total);
+/* Host threads of the library's worker pool (the run's `threads`, pipeline.rs:478;
+ * 0: every hardware thread, at most 32). Results do not depend on it. */
+int ky_set_threads(uint32_t threads);
+/* Release the context's pass buffers and the calling thread's cached device blocks
+ * (between the stages of a run under a memory limit). */
+int ky_trim(ky_ctx *ctx);
+/* Device memory models for a memory limit (crates/kyto-plan/src/memory_limit.rs):
+ * a resident layer and one ky_pca over it; ky_connectivities of n_cells rows. */
+int ky_pca_bytes(uint64_t n_rows, uint32_t n_cols, uint64_t nnz, uint32_t n_components, uint32_t power_iterations,
+                 uint64_t *bytes);
+int ky_connectivities_bytes(uint64_t n_cells, uint32_t n_neighbors, uint64_t *bytes);
+
 /* ---------------------------------------------------------------- CSR layers ---- */
 
 /* Upload a CSR layer (cells x genes, indices strictly increasing within a row) and

@@ -27,7 +27,7 @@ from typing import Any
 
 import numpy as np
 
-from ._native import KytoError, KY_E215, KY_E501
+from ._native import KytoError, KY_E215, KY_E401, KY_E501
 from . import store as _store
 from .device import ConnectivityGraph, CsrLayer, DenseMatrix, Device, Graph, IvfIndex, PcaOutput, device_count
 
@@ -36,6 +36,11 @@ __all__ = ["scan", "Atlas", "Device", "KytoError", "Graph", "ConnectivityGraph",
 
 # crates/kyto-plan/src/physical_plan.rs:73
 PCA_POWER_ITERATIONS = 9
+
+
+def _piece(nbytes: int) -> int:
+    """A device buffer's bytes as the library rounds them (256-byte pieces)."""
+    return (int(nbytes) + 255) // 256 * 256
 
 
 def _invalid(function, parameter, value, problem, accepted) -> KytoError:
@@ -343,25 +348,128 @@ class Atlas:
                             copy_layers=list(self.layers) if from_store else None, overwrite=overwrite,
                             zstd_level=zstd_level)
 
+    def _memory_plan(self, limit) -> dict:
+        """kyto's within_memory_limit for this path (memory_limit.rs:119-175): each
+        stage's device memory (the resident embeddings of earlier pca steps, plus a
+        pca's layer and passes, or a neighbors step's search and connectivities) from
+        the library's models; a neighbors search takes the largest round of queries
+        that fits. KY-E401, before anything runs, with the smallest peak the plan can
+        run in, when a stage does not fit. Returns each neighbors step's round."""
+        from .device import connectivities_bytes, neighbors_bytes, neighbors_plan, pca_bytes
+
+        rounds: dict = {}
+        if limit is None:
+            return rounds
+        resident, required = 0, 0
+        shapes: dict = {}
+        for i, step in enumerate(self.steps):
+            if isinstance(step, _Pca):
+                source, _ = self._source(step.layer)
+                layer = self.layers[source]
+                n, g, nnz = layer.n_cells, layer.n_genes, layer.nnz
+                required = max(required, resident + pca_bytes(n, g, nnz, step.n_components, step.power_iterations))
+                resident += _piece(4 * n * step.n_components)
+                shapes[step.into] = (n, step.n_components)
+            elif isinstance(step, _Neighbors):
+                n, cols = shapes[step.embedding]
+                precision = step.precision
+                conn = connectivities_bytes(n, step.n_neighbors)
+                smallest = neighbors_bytes(n, cols, step.n_neighbors, step.n_components, step.metric, precision,
+                                           256 if precision == "bf16" else 64)
+                need = resident + max(smallest, conn)
+                required = max(required, need)
+                if need <= limit:
+                    rounds[i], _ = neighbors_plan(n, cols, limit - resident, step.n_neighbors, step.n_components,
+                                                  step.metric, precision)
+        if required > limit:
+            raise KytoError(KY_E401, f"the plan needs at least {required} bytes but memory_limit is {limit} bytes")
+        return rounds
+
+    def _total_rows(self) -> int:
+        """Cell rows the run computes, summed over its passes (kyto's Progress.total):
+        a pca reads its layer once and makes 2 + q passes, a neighbors step one."""
+        total, cells = 0, {}
+        for step in self.steps:
+            if isinstance(step, _Pca):
+                n = self.layers[self._source(step.layer)[0]].n_cells
+                total += n * (step.power_iterations + 3)
+                cells[step.into] = n
+            elif isinstance(step, _Neighbors):
+                total += cells.get(step.embedding, 0)
+        return total
+
     def _run(self, memory_limit, threads, progress, keep_layers) -> dict:
+        """Run the plan on the device: on a worker thread, while the calling thread
+        reports progress(done, total) as the run moves and turns Ctrl-C into a cancel
+        (the run stops at its next pass, block or round with KY-E503)."""
+        import threading
+
+        from .device import set_threads
+
         limit = parse_memory(memory_limit)
         if threads is not None:
-            _integer("collect", "threads", threads, 1)
+            threads = _integer("collect", "threads", threads, 1)
+        if progress is not None and not callable(progress):
+            raise _invalid("collect", "progress", type(progress).__name__, "not callable",
+                           "a callable taking (done, total)")
         if self.device == "cpu":
             raise KytoError(KY_E501, "this build runs pca and neighbors on the GPU only: on(device='gpu')")
+        rounds = self._memory_plan(limit)
         ordinal = 0 if self.device in ("auto", "gpu", "cuda") else int(self.device)
+        dev = Device(ordinal)
+        outcome: dict = {}
+
+        def work():
+            try:
+                outcome["out"] = self._steps(dev, rounds, limit is not None, keep_layers)
+            except BaseException as e:  # noqa: BLE001 - handed to the calling thread
+                outcome["error"] = e
+
+        try:
+            set_threads(threads or 0)
+            dev.run_begin(self._total_rows())
+            worker = threading.Thread(target=work, name="kyto_amd-run", daemon=True)
+            worker.start()
+            seen = None
+            try:
+                while worker.is_alive():
+                    worker.join(0.01)
+                    if progress is not None:
+                        now = dev.progress()
+                        if now != seen:
+                            seen = now
+                            progress(*now)
+            except KeyboardInterrupt as interrupt:
+                # Ctrl-C (or a progress callback raising it) cancels the run cleanly: it
+                # stops at its next pass, block or round, its buffers freed.
+                dev.cancel()
+                worker.join()
+                error = outcome.get("error")
+                if isinstance(error, KytoError) and error.status == 503:
+                    raise error from interrupt
+                raise
+            if "error" in outcome:
+                raise outcome["error"]
+            if progress is not None:
+                total = dev.progress()[1]
+                if seen != (total, total):
+                    progress(total, total)
+            return outcome["out"]
+        finally:
+            set_threads(0)
+            dev.close()
+
+    def _steps(self, dev: Device, rounds: dict, limited: bool, keep_layers: bool) -> dict:
         out: dict = {}
         made: dict = {}
-        dev = Device(ordinal)
         resident: dict[str, DenseMatrix] = {}
         try:
-            for step in self.steps:
+            for i, step in enumerate(self.steps):
                 if isinstance(step, _Map):
                     continue  # applied on the device by the step that reads the layer
                 if isinstance(step, _Pca):
                     source, maps = self._source(step.layer)
                     layer = self.layers[source]
-                    _fits(layer, step, limit)
                     res, emb = _run_pca(dev, layer, step, _device_maps(maps), keep_layer=keep_layers)
                     if keep_layers and res.made_values is not None:
                         made[step.layer] = (source, res.made_values)
@@ -371,21 +479,19 @@ class Atlas:
                     out[f"{step.into}_variance"] = res.variance
                     out[f"{step.into}_variance_ratio"] = res.variance_ratio
                     out[f"{step.into}_means"] = res.means
-                    if progress:
-                        progress(("pca", step.into))
                 else:
                     emb = resident[step.embedding]
-                    g = _run_neighbors(dev, emb, step)
+                    g = _run_neighbors(dev, emb, step, rounds.get(i, 0))
                     key, conn = _graph_names(step.into)
                     out[key] = g
                     # The connectivities graph the step declares (atlas.rs:857-862).
                     out[conn] = dev.connectivities(g)
-                    if progress:
-                        progress(("neighbors", key))
+                if limited:
+                    dev.trim()  # the stage's cached blocks back to the device before the next
         finally:
             for m in resident.values():
                 m.free()
-            dev.close()
+            dev.trim()
         if keep_layers:
             out["__made_layers__"] = made
         return out
@@ -396,17 +502,6 @@ def _graph_names(into):
     if into is None:
         return "distances", "connectivities"
     return f"{into}_distances", f"{into}_connectivities"
-
-
-def _fits(layer: _Layer, step: _Pca, limit):
-    if limit is None:
-        return
-    n, g = layer.n_cells, layer.n_genes
-    width = min(step.n_components + 10, g) + 1
-    need = layer.nnz * 14 + (n + 1) * 8 + n * width * 4 + n * step.n_components * 4
-    if need > limit:
-        raise KytoError(KY_E215, f"collect: the resident layer needs {need} bytes of device memory, above "
-                                 f"memory_limit={limit}; pass a larger memory_limit")
 
 
 def _device_maps(maps):
@@ -479,9 +574,9 @@ def _run_pca(dev: Device, layer, step: _Pca, maps=None, keep_layer=False):
     return res, res.device_embedding
 
 
-def _run_neighbors(dev: Device, emb: DenseMatrix, step: _Neighbors) -> Graph:
+def _run_neighbors(dev: Device, emb: DenseMatrix, step: _Neighbors, round_rows: int = 0) -> Graph:
     if step.search == "exact":
-        return dev.neighbors(emb, step.n_neighbors, step.n_components, step.metric, step.precision)
+        return dev.neighbors(emb, step.n_neighbors, step.n_components, step.metric, step.precision, round_rows)
     if step.n_components is not None and step.n_components != emb.cols:
         raise KytoError(KY_E501, "neighbors: the approximate search uses every component")
     n_lists = max(1, min(4096, emb.rows // 256))

@@ -91,6 +91,13 @@ SIGNATURES = {
     "ky_open": [c_int, POINTER(c_void_p)],
     "ky_close": [c_void_p],
     "ky_device_info": [c_void_p, c_char_p, c_size_t, u64p],
+    "ky_run_begin": [c_void_p, c_uint64],
+    "ky_cancel": [c_void_p],
+    "ky_progress": [c_void_p, u64p, u64p],
+    "ky_set_threads": [c_uint32],
+    "ky_trim": [c_void_p],
+    "ky_pca_bytes": [c_uint64, c_uint32, c_uint64, c_uint32, c_uint32, u64p],
+    "ky_connectivities_bytes": [c_uint64, c_uint32, u64p],
     "ky_csr_upload": [c_void_p, u64p, u32p, f32p, c_uint64, c_uint32, POINTER(c_void_p)],
     "ky_csr_free": [c_void_p],
     "ky_csr_shape": [c_void_p, u64p, u32p, u64p],

@@ -127,6 +127,60 @@ int ky_close(ky_ctx *ctx) {
     });
 }
 
+int ky_run_begin(ky_ctx *ctx, uint64_t total_rows) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        ctx->cancelled.store(false);
+        ctx->rows_done.store(0);
+        ctx->rows_total.store(total_rows);
+    });
+}
+
+int ky_cancel(ky_ctx *ctx) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        ctx->cancelled.store(true);
+    });
+}
+
+int ky_progress(const ky_ctx *ctx, uint64_t *done, uint64_t *total) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        if (done) *done = ctx->rows_done.load();
+        if (total) *total = ctx->rows_total.load();
+    });
+}
+
+int ky_set_threads(uint32_t threads) {
+    return guarded([&] { host_threads(threads); });
+}
+
+int ky_trim(ky_ctx *ctx) {
+    return guarded([&] {
+        need(ctx, "ctx");
+        KY_HIP(hipSetDevice(ctx->device));
+        KY_HIP(hipStreamSynchronize(ctx->stream));
+        pca_release(*ctx);
+        device_cache_trim();
+    });
+}
+
+int ky_pca_bytes(uint64_t n_rows, uint32_t n_cols, uint64_t nnz, uint32_t n_components, uint32_t power_iterations,
+                 uint64_t *bytes) {
+    return guarded([&] {
+        need(bytes, "bytes");
+        *bytes = pca_device_bytes(n_rows, n_cols, nnz, n_components, power_iterations);
+    });
+}
+
+int ky_connectivities_bytes(uint64_t n_cells, uint32_t n_neighbors, uint64_t *bytes) {
+    return guarded([&] {
+        need(bytes, "bytes");
+        KY_CHECK(n_neighbors >= 2, KY_E215, "connectivities: n_neighbors counts the cell itself (at least 2)");
+        *bytes = connectivities_device_bytes(n_cells, n_neighbors);
+    });
+}
+
 int ky_device_info(ky_ctx *ctx, char *name, size_t name_len, uint64_t *total_bytes) {
     return guarded([&] {
         need(ctx, "ctx");

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <mutex>
 #include <cstdint>
@@ -134,6 +135,17 @@ struct Context {
     hipEvent_t staged[2] = {nullptr, nullptr};
     // Events for kernel timings, made when the context opens.
     hipEvent_t timer[2] = {nullptr, nullptr};
+    // The run the context's calls belong to (kyto's RunControl, run_control.rs:1-32): a
+    // cancel flag any thread may set, checked between passes, blocks and rounds of
+    // queries, and the cell rows computed (summed over the passes).
+    std::atomic<bool> cancelled{false};
+    std::atomic<uint64_t> rows_done{0}, rows_total{0};
+    void check_cancel() const {
+        if (cancelled.load(std::memory_order_relaxed))
+            throw Error(KY_E503, "the run was cancelled after " + std::to_string(rows_done.load()) + " of " +
+                                     std::to_string(rows_total.load()) + " cell rows were streamed");
+    }
+    void advance(uint64_t rows) { rows_done.fetch_add(rows, std::memory_order_relaxed); }
     ~Context() {
         if (staged[0]) (void)hipEventDestroy(staged[0]);
         if (staged[1]) (void)hipEventDestroy(staged[1]);
@@ -160,6 +172,9 @@ void upload_parts(Context &ctx, const std::vector<CopyPart> &parts);
 
 // f(t) for t in [0, tasks) on the persistent host worker pool (dense.cpp).
 void host_parallel(size_t tasks, const std::function<void(size_t)> &f);
+// Threads the pool's runs use (0: every worker); results do not depend on it.
+void host_threads(unsigned threads);
+unsigned host_threads_active();
 
 struct Csr {
     Context *ctx = nullptr;

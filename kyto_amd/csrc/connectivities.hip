@@ -339,4 +339,14 @@ void connectivities_run(Context &ctx, uint64_t n, uint32_t n_neighbors, const ui
     }
 }
 
+size_t connectivities_device_bytes(uint64_t n, uint32_t n_neighbors) {
+    auto b = [](size_t bytes) { return Arena::piece(std::max<size_t>(bytes, 1)); };
+    const uint64_t ne = n * (uint64_t)(n_neighbors - 1);
+    // The graph (indices, distances), ranked distances and memberships, sigma and rho;
+    // the entries sorted by column (entries, keys, values, the sort's scratch); the
+    // transposed offsets and counts; the union (indptr, at most 2 ne indices and weights).
+    return 4 * b(4 * ne) + 2 * b(4 * n) + 3 * b(4 * ne) + b(sort_pairs_bytes(ne)) + b(8 * (n + 1)) +
+           b(4 * (n + 1)) + b(scan_counts_bytes(n)) + b(8 * (n + 1)) + 2 * b(4 * 2 * ne);
+}
+
 }  // namespace ky

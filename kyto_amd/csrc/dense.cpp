@@ -34,23 +34,28 @@ class Pool {
         return pool;
     }
     unsigned size() const { return workers_ + 1; }
+    // Threads a run uses (the caller and active - 1 workers; at most size()).
+    void set_active(unsigned threads) { active_.store(threads == 0 ? size() : std::min(threads, size())); }
+    unsigned active() const { return active_.load(); }
     template <typename F>
     void run(size_t tasks, F &&f) {
         if (tasks == 0) return;
-        if (workers_ == 0 || tasks == 1) {
+        const unsigned active = active_.load();
+        if (active <= 1 || tasks == 1) {
             for (size_t t = 0; t < tasks; t++) f(t);
             return;
         }
         std::lock_guard<std::mutex> serial(run_mutex_);
         job_ = [&f](size_t t) { f(t); };
         tasks_ = tasks;
+        stride_ = active;
         done_.store(0, std::memory_order_relaxed);
         {
             std::lock_guard<std::mutex> lock(sleep_mutex_);
             epoch_.fetch_add(1, std::memory_order_release);
         }
         sleep_cv_.notify_all();
-        for (size_t t = 0; t < tasks; t += size()) job_(t);
+        for (size_t t = 0; t < tasks; t += stride_) job_(t);
         while (done_.load(std::memory_order_acquire) != workers_) spin_pause();
     }
     ~Pool() {
@@ -67,6 +72,7 @@ class Pool {
     Pool() {
         unsigned n = std::thread::hardware_concurrency();
         workers_ = std::max(1u, std::min(n, 32u)) - 1;
+        active_ = workers_ + 1;
         for (unsigned w = 1; w <= workers_; w++) threads_.emplace_back([this, w] { loop(w); });
     }
     static void spin_pause() { __builtin_ia32_pause(); }
@@ -85,11 +91,13 @@ class Pool {
             }
             seen = e;
             if (stop_) return;
-            for (size_t t = w; t < tasks_; t += size()) job_(t);
+            if (w < stride_)
+                for (size_t t = w; t < tasks_; t += stride_) job_(t);
             done_.fetch_add(1, std::memory_order_acq_rel);
         }
     }
-    unsigned workers_ = 0;
+    unsigned workers_ = 0, stride_ = 1;
+    std::atomic<unsigned> active_{1};
     std::vector<std::thread> threads_;
     std::function<void(size_t)> job_;
     size_t tasks_ = 0;
@@ -101,6 +109,9 @@ class Pool {
 };
 
 void host_parallel(size_t tasks, const std::function<void(size_t)> &f) { Pool::get().run(tasks, f); }
+
+void host_threads(unsigned threads) { Pool::get().set_active(threads); }
+unsigned host_threads_active() { return Pool::get().active(); }
 
 // f(begin, end) over [0, n) in contiguous ranges, one a worker (serially below 64).
 template <typename F>

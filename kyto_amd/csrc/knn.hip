@@ -881,6 +881,7 @@ void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t
         }
         for (uint64_t lo = 0; lo < nq; lo += s.round) {
             const uint64_t count = std::min<uint64_t>(s.round, nq - lo);
+            ctx.check_cancel();
             KnnRound r;
             take_round(a, s, count, r);
             double t0 = now_ms();
@@ -905,6 +906,7 @@ void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t
             rerank_ms += now_ms() - t1;
             a.release(m1);
             rounds++;
+            ctx.advance(count);
         }
         KY_HIP(hipMemcpyAsync(&total, unc_count, sizeof(total), hipMemcpyDeviceToHost, st));
         KY_HIP(hipStreamSynchronize(st));

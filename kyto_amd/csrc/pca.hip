@@ -447,10 +447,12 @@ void csr_build(Csr &csr, const uint64_t *indptr, const uint32_t *indices, const 
     csr.indptr.alloc(n + 1);
     csr.indices.alloc(std::max<uint64_t>(csr.nnz, 1));
     csr.values.alloc(std::max<uint64_t>(csr.nnz, 1));
+    ctx.check_cancel();
     upload_parts(ctx, {{csr.indptr.ptr, indptr, sizeof(uint64_t) * (n + 1)},
                        {csr.indices.ptr, indices, sizeof(uint32_t) * csr.nnz},
                        {csr.values.ptr, values, sizeof(float) * csr.nnz}});
     csr_finish(csr, maps);
+    ctx.advance(n);
 }
 
 // A pinned host buffer (hipHostMalloc), grown on demand.
@@ -528,6 +530,12 @@ void csr_stream(Csr &csr, uint64_t block_rows, ky_block_reader reader, void *use
     for (uint64_t first = 0; first < n; first += block_rows, blocks++) {
         const int k = (int)(blocks % 2);
         const uint64_t cnt = std::min<uint64_t>(block_rows, n - first);
+        if (ctx.cancelled.load()) {
+            // Stop before the next block, with no copy from the pinned buffers in flight.
+            KY_HIP(hipStreamSynchronize(cp));
+            KY_HIP(hipStreamSynchronize(st));
+            ctx.check_cancel();
+        }
         if (used[k]) KY_HIP(hipEventSynchronize(guard.e[k]));  // the set's previous copy is done
         const double t0 = now_ms();
         int64_t got = reader(user, first, cnt, pin_ptr[k].ptr, pin_idx[k].ptr, pin_val[k].ptr, pin_idx[k].n);
@@ -575,6 +583,7 @@ void csr_stream(Csr &csr, uint64_t block_rows, ky_block_reader reader, void *use
         KY_HIP(hipStreamWaitEvent(st, guard.e[k], 0));
         normalize_log1p(st, csr.indptr.ptr + first, cnt, csr.values.ptr, maps.target_sum, maps.normalize, maps.log1p);
         total += nnz;
+        ctx.advance(cnt);
     }
     KY_HIP(hipStreamSynchronize(cp));
     KY_HIP(hipStreamSynchronize(st));
@@ -769,6 +778,35 @@ void pca_project(Csr &csr, const double *v, uint32_t k, const double *mean_proje
         delete static_cast<ky_dense *>(emb);
 }
 
+size_t pca_device_bytes(uint64_t n, uint32_t g, uint64_t nnz, uint32_t k, uint32_t q) {
+    auto b = [](size_t bytes) { return Arena::piece(std::max<size_t>(bytes, 1)); };
+    const uint64_t chunks = (n + CHUNK_ROWS - 1) / CHUNK_ROWS;
+    const size_t width = std::min<size_t>((size_t)k + PCA_OVERSAMPLES, g), w1 = width + 1;
+    const size_t ld = padded_row((uint32_t)w1);
+    const size_t cap = width * (q + 1);
+    size_t total = 0;
+    // The layer: rows (indptr, indices, values) and chunked columns.
+    total += b(8 * (n + 1)) + 2 * b(4 * nnz) + b(4 * chunks * (g + 1)) + 2 * b(4 * nnz);
+    // Pass buffers: W (n rows of ld floats), the chunks' f64 partials and their
+    // reduction, the column sums, the dense operand.
+    total += b(4 * (n * ld + DENSE_PAD)) + b(8 * g * w1 * chunks * 2) + b(8 * (g * w1 + w1)) + b(8 * 4096 * w1);
+    total += b(4 * (g * ld + DENSE_PAD)) + b(8 * ((size_t)k + g + w1));
+    // The basis and its products on the device, the dense steps' partials and work.
+    total += 2 * b(8 * g * cap) + b(8 * ((g + 63) / 64) * cap * cap) + b(8 * cap * cap) + b(8 * g * cap);
+    // The embedding.
+    total += b(4 * n * k);
+    return total;
+}
+
+void pca_release(Context &ctx) {
+    ctx.partials.release();
+    ctx.reduced.release();
+    ctx.dense_f32.release();
+    ctx.dense_f64.release();
+    ctx.wbuf.release();
+    ctx.sums.release();
+}
+
 // The whole pca(): PcaAccumulator's passes (pca.rs:302-862) with GPU folds.
 void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *host_embedding, Dense **embedding,
              float *loadings_out, double *variance, double *variance_ratio, double *means_out, uint32_t *passes_out,
@@ -799,7 +837,10 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
     double t0 = now_ms();
     Mat sketch(genes, width);
     std::vector<double> sums(genes), omega_sums(width);
+    Context &run = *csr.ctx;
+    run.check_cancel();
     pca_sketch(csr, seed, 0, (uint32_t)width, sketch.v.data(), sums.data(), omega_sums.data());
+    run.advance(n);
     pass_time(t0);
     double td = now_ms();
     std::vector<double> means(genes);
@@ -841,7 +882,9 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
         Mat product(genes, w);
         std::vector<double> wsum(w);
         t0 = now_ms();
+        run.check_cancel();
         pca_product(csr, block.v.data(), (uint32_t)w, bm.data(), product.v.data(), wsum.data());
+        run.advance(n);
         pass_time(t0);
         passes++;
         td = now_ms();
@@ -868,8 +911,10 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
     std::vector<double> squares(genes);
     std::vector<uint64_t> stored(genes);
     t0 = now_ms();
+    run.check_cancel();
     pca_project(csr, loadings.v.data(), (uint32_t)k, mean_projection.data(), means.data(), host_embedding, embedding,
                 squares.data(), stored.data());
+    run.advance(n);
     pass_time(t0);
     // Total variance (pca.rs:776-791) and finish (pca.rs:804-845).
     double total = 0.0;

@@ -28,6 +28,12 @@ void pca_sketch(Csr &csr, uint64_t seed, uint64_t cell_offset, uint32_t l, doubl
 void pca_product(Csr &csr, const double *q, uint32_t w, const double *block_means, double *product, double *sums);
 void pca_project(Csr &csr, const double *v, uint32_t k, const double *mean_projection, const double *means,
                  float *host_embedding, Dense **embedding, double *squares, uint64_t *stored);
+// The device memory of a resident layer and one pca() over it (a model of the
+// allocations: the rows and the chunked columns, the pass buffers, the basis and its
+// products, the embedding), for a memory limit.
+size_t pca_device_bytes(uint64_t n_rows, uint32_t n_cols, uint64_t nnz, uint32_t n_components, uint32_t q);
+// The context's pass buffers released (for a memory limit between stages).
+void pca_release(Context &ctx);
 void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *host_embedding, Dense **embedding,
              float *loadings_out, double *variance, double *variance_ratio, double *means_out, uint32_t *passes_out,
              ky_pca_timing *timing);

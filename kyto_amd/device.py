@@ -100,6 +100,26 @@ class Device:
         except Exception:
             pass
 
+    # ------------------------------------------------------------ run control --
+
+    def run_begin(self, total_rows: int) -> None:
+        """A new run on this context: the cancel flag cleared, no rows done."""
+        N.check(self._lib.ky_run_begin(self._h, int(total_rows)))
+
+    def cancel(self) -> None:
+        """Stop the run at its next pass, block or round (KY-E503); any thread."""
+        N.check(self._lib.ky_cancel(self._h))
+
+    def progress(self) -> tuple[int, int]:
+        """(cell rows computed, cell rows of the run); any thread."""
+        done, total = c_uint64(0), c_uint64(0)
+        N.check(self._lib.ky_progress(self._h, byref(done), byref(total)))
+        return int(done.value), int(total.value)
+
+    def trim(self) -> None:
+        """Release the context's pass buffers and this thread's cached device blocks."""
+        N.check(self._lib.ky_trim(self._h))
+
     def info(self) -> dict:
         name = ctypes.create_string_buffer(128)
         total = c_uint64(0)
@@ -287,6 +307,26 @@ class Device:
         info = N.IvfInfo()
         N.check(self._lib.ky_ivf_build(self._h, points.handle, byref(p), byref(h), byref(info)))
         return IvfIndex(self, h, {f: getattr(info, f) for f, _ in N.IvfInfo._fields_})
+
+
+def set_threads(threads: int) -> None:
+    """Host threads of the library's worker pool (0: every hardware thread, at most 32)."""
+    N.check(N.gpu_lib().ky_set_threads(int(threads)))
+
+
+def pca_bytes(n_rows, n_cols, nnz, n_components, power_iterations) -> int:
+    """Device memory of a resident layer and one pca() over it (ky_pca_bytes)."""
+    out = c_uint64(0)
+    N.check(N.gpu_lib().ky_pca_bytes(int(n_rows), int(n_cols), int(nnz), int(n_components), int(power_iterations),
+                                     byref(out)))
+    return int(out.value)
+
+
+def connectivities_bytes(n_cells, n_neighbors) -> int:
+    """Device memory of ky_connectivities (ky_connectivities_bytes)."""
+    out = c_uint64(0)
+    N.check(N.gpu_lib().ky_connectivities_bytes(int(n_cells), int(n_neighbors), byref(out)))
+    return int(out.value)
 
 
 def neighbors_bytes(rows, cols, n_neighbors=15, n_components=None, metric="euclidean", precision="bf16",

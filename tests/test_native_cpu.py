@@ -9,6 +9,7 @@ import pytest
 
 import oracle
 import kyto_amd
+from kyto_amd import KytoError
 from kyto_amd import _native as N
 from kyto_amd import device as D
 from kyto_amd import synthetic as S
@@ -169,3 +170,41 @@ def test_plan_layer_maps_are_checked():
     assert b.pca(seed=0, layer="X").steps[-1].layer == "X"
     with pytest.raises(kyto_amd.KytoError):
         b.pca(seed=0, layer="nope")
+
+
+def _store_plan(n, g, nnz, k=50, nn=15, q=4, precision="bf16"):
+    import kyto_amd
+    ref = kyto_amd._StoreRef("/nowhere", "X", n, g, nnz, "float32", 4096, 1024)
+    return kyto_amd.Atlas({"X": ref}, "X").pca(n_components=k, seed=0, power_iterations=q).neighbors(
+        n_neighbors=nn, precision=precision)
+
+
+def test_memory_limit_plans_rounds_and_refuses_with_e401():
+    # memory_limit.rs:119-175: the search shrinks its rounds to fit; a plan that cannot
+    # fit raises KY-E401 before anything runs (the store here is never opened).
+    plan = _store_plan(1_000_000, 2000, 160_000_000)
+    assert plan._memory_plan(None) == {}
+    assert plan._memory_plan(64 << 30) == {1: 1_000_000}
+    rounds = plan._memory_plan(4 << 30)
+    assert 0 < rounds[1] < 1_000_000 and rounds[1] % 256 == 0
+    with pytest.raises(KytoError) as e:
+        plan._memory_plan(2 << 30)
+    assert e.value.code == "KY-E401" and "the plan needs at least" in str(e.value)
+    with pytest.raises(KytoError) as e:
+        plan.collect(memory_limit="1GB")
+    assert e.value.code == "KY-E401"
+    # 4M x 3,000 (~840M values), k = 30: runs in 32 GB.
+    big = _store_plan(4_000_000, 3000, 840_000_000, nn=30)
+    assert big._memory_plan(32 * 10**9)[1] > 0
+    assert big._total_rows() == 4_000_000 * (4 + 3) + 4_000_000
+
+
+def test_collect_arguments_are_checked():
+    plan = _store_plan(1000, 50, 5000, k=10)
+    for bad in (0, True, 2.5):
+        with pytest.raises(KytoError, match="KY-E215"):
+            plan.collect(threads=bad)
+    with pytest.raises(KytoError, match="KY-E215"):
+        plan.collect(progress=5)
+    with pytest.raises(KytoError, match="KY-E215"):
+        plan.collect(memory_limit="lots")
