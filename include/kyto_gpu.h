@@ -335,12 +335,21 @@ int ky_neighbors_plan(ky_ctx *ctx, uint64_t rows, uint32_t cols, uint32_t n_comp
 int ky_neighbors_static(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uint32_t n_neighbors,
                         int metric, int precision, uint32_t *indices, float *distances, ky_knn_info *info);
 
-/* The candidate search alone (DeviceKernels::nearest_keys, launch_matrix.rs:253-286):
- * for each of the rows [first, first + count) of `points`, its m nearest other rows
- * by the approximate key at `precision`, ascending (keys f32 count x m, rows u32
- * count x m). */
-int ky_nearest_keys(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uint64_t first,
-                    uint64_t count, uint32_t m, int precision, float *keys, uint32_t *rows);
+/* Pairwise keys of ky_nearest_keys (launch_matrix.rs:150-161, PairwiseKey). */
+#define KY_KEY_SQUARED_EUCLIDEAN 0 /* sum of squared differences in component order, f32 */
+#define KY_KEY_COSINE 1            /* 1 - dot / (|q| |p|) in f32, clipped to [0, 2], 1 for a zero norm */
+
+/* The candidate search alone (DeviceKernels::nearest_keys, launch_matrix.rs:240-286, the
+ * keys of kernels_matrix.rs:140-257): for each of the n_queries rows of `queries` (host,
+ * row-major, points' width a row), its m smallest keys against every row of `points`,
+ * the point skip[i] left out (UINT32_MAX for none), ranked by (key, point), smallest
+ * first: keys f32 and rows u32, n_queries x m (host). The cosine key takes each query's
+ * and each point's f32 norm. KY_E215 for m outside 1..the points left after skip (as
+ * the reference's panics), a missing norm or an unknown key; KY_E501 past 64 keys a
+ * query or 64 components. */
+int ky_nearest_keys(ky_ctx *ctx, const float *queries, uint64_t n_queries, const ky_dense *points, int key,
+                    const float *query_norms, const float *point_norms, const uint32_t *skip, uint32_t m,
+                    float *keys, uint32_t *rows);
 
 /* --------------------------------------------------------- connectivities ---- */
 
@@ -397,6 +406,12 @@ typedef struct ky_ivf_info {
 int ky_ivf_build(ky_ctx *ctx, const ky_dense *points, const ky_ivf_params *params,
                  ky_ivf **out, ky_ivf_info *info);
 int ky_ivf_free(ky_ivf *ivf);
+/* The index's k-means layout (any output may be NULL): centroids f32 n_lists x d,
+ * each row's list (n), the lists' offsets (n_lists + 1) and rows (n, ascending within
+ * a list), and the router's alpha (its logits are alpha (2 x.c - |c|^2) plus the MLP's
+ * correction). */
+int ky_ivf_layout(const ky_ivf *ivf, float *centroids, uint32_t *assign, uint32_t *list_ptr, uint32_t *list_rows,
+                  float *alpha);
 
 typedef struct ky_ivf_search_info {
     uint64_t fallback;      /* queries below the confidence threshold searched exactly */

@@ -23,6 +23,7 @@ KY_E505 = 505
 # include/kyto_gpu.h: the fixed device region of ky_neighbors_static.
 KY_STATIC_SCRATCH_BYTES = 64 << 20
 
+KEY = {"squared_euclidean": 0, "cosine": 1}
 METRIC = {"euclidean": 0, "cosine": 1}
 PRECISION = {"fp32": 0, "bf16": 1}
 
@@ -120,7 +121,7 @@ SIGNATURES = {
     "ky_neighbors_static": [c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_int, u32p, f32p, POINTER(KnnInfo)],
     "ky_neighbors_bytes": [c_void_p, c_uint64, c_uint32, c_uint32, c_uint32, c_int, c_int, c_uint64, u64p],
     "ky_neighbors_plan": [c_void_p, c_uint64, c_uint32, c_uint32, c_uint32, c_int, c_int, c_uint64, u64p, u64p],
-    "ky_nearest_keys": [c_void_p, c_void_p, c_uint32, c_uint64, c_uint64, c_uint32, c_int, f32p, u32p],
+    "ky_nearest_keys": [c_void_p, f32p, c_uint64, c_void_p, c_int, f32p, f32p, u32p, c_uint32, f32p, u32p],
     "ky_connectivities": [c_void_p, c_uint64, c_uint32, u32p, f32p, POINTER(c_void_p), f32p, f32p,
                           POINTER(ConnInfo)],
     "ky_graph_shape": [c_void_p, u64p, u64p],
@@ -128,6 +129,7 @@ SIGNATURES = {
     "ky_graph_free": [c_void_p],
     "ky_ivf_build": [c_void_p, c_void_p, POINTER(IvfParams), POINTER(c_void_p), POINTER(IvfInfo)],
     "ky_ivf_free": [c_void_p],
+    "ky_ivf_layout": [c_void_p, f32p, u32p, u32p, u32p, POINTER(c_float)],
     "ky_ivf_search": [c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_int, c_float, u32p, f32p,
                       POINTER(IvfSearchInfo)],
     "ky_router_grad": [c_void_p, c_uint32, c_uint32, c_uint32, f32p, f32p, c_uint32, u32p, f32p, c_uint32, f32p,

@@ -527,15 +527,20 @@ int ky_graph_free(ky_graph *g) {
     });
 }
 
-int ky_nearest_keys(ky_ctx *ctx, const ky_dense *points, uint32_t n_components, uint64_t first, uint64_t count,
-                    uint32_t m, int precision, float *keys, uint32_t *rows) {
+int ky_nearest_keys(ky_ctx *ctx, const float *queries, uint64_t n_queries, const ky_dense *points, int key,
+                    const float *query_norms, const float *point_norms, const uint32_t *skip, uint32_t m, float *keys,
+                    uint32_t *rows) {
     return guarded([&] {
         need(ctx, "ctx");
         need(points, "points");
-        need(keys, "keys");
-        need(rows, "rows");
+        if (n_queries) {
+            need(queries, "queries");
+            need(skip, "skip");
+            need(keys, "keys");
+            need(rows, "rows");
+        }
         use_device(*ctx);
-        nearest_keys_run(*ctx, *points, n_components, first, count, m, precision, keys, rows);
+        nearest_keys_run(*ctx, queries, n_queries, *points, key, query_norms, point_norms, skip, m, keys, rows);
     });
 }
 
@@ -547,6 +552,14 @@ int ky_ivf_build(ky_ctx *ctx, const ky_dense *points, const ky_ivf_params *param
         need(out, "out");
         use_device(*ctx);
         *out = ivf_build(*ctx, *points, *params, info);
+    });
+}
+
+int ky_ivf_layout(const ky_ivf *ivf, float *centroids, uint32_t *assign, uint32_t *list_ptr, uint32_t *list_rows,
+                  float *alpha) {
+    return guarded([&] {
+        need(ivf, "ivf");
+        ivf_layout(*ivf, centroids, assign, list_ptr, list_rows, alpha);
     });
 }
 

@@ -9,9 +9,12 @@
 // - labels: the exact k nearest neighbours (the exact search of knn.hip, itself
 //   kyto's graph) of a seeded sample of queries; a query's target distribution puts
 //   weight 1/k on the list of each of its neighbours;
-// - the router: an MLP d -> hidden -> hidden -> n_lists on (x - mean) / rms, trained
+// - the router: list logits alpha (2 x.c - |c|^2) + MLP((x - mean) / rms), the MLP
+//   d -> hidden -> hidden -> n_lists a learned correction of the nearest-centroid
+//   score (its output layer starts at zero, so the router starts as centroid probing;
+//   alpha = exp(gamma) starts at the scale that best fits one batch's labels), trained
 //   with the softmax cross-entropy against those targets by its own forward,
-//   backward and Adam kernels (router.hip);
+//   backward and Adam kernels (router.hip), gamma with them;
 // - search: each row's top n_probe lists (router logits, or nearest centroids), the
 //   probed lists' points scored with kyto's exact f64 key (8-lane order) and ranked
 //   by (key, row); rows whose router confidence (the softmax mass of the probed lists)
@@ -46,6 +49,8 @@ struct ky_ivf {
     ky::DevBuf<float> mean;          // d (router input centring)
     float inv_scale = 1.0f;
     ky::Router router;
+    ky::DevBuf<float> gamma;         // [gamma, its gradient, Adam m, Adam v]: alpha = exp(gamma)
+    float alpha = 0.0f;              // the trained alpha (for info)
 };
 
 namespace ky {
@@ -156,6 +161,34 @@ __global__ void make_targets(const uint32_t *__restrict__ q_rows, uint64_t count
     uint32_t row = nbr[(uint64_t)q_rows[i] * k + e];
     t_idx[t] = assign[row];
     t_w[t] = 1.0f / (float)k;
+}
+
+// L += exp(gamma) S (the router's correction added to the scaled centroid scores).
+__global__ void add_centroid_scores(float *__restrict__ L, const float *__restrict__ S, uint64_t count,
+                                    const float *__restrict__ gamma) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < count) L[i] += expf(gamma[0]) * S[i];
+}
+
+// Per row (one wavefront): sum over the lists of dL S, lane j summing lists j, j + 64,
+// ... in order, then a fixed butterfly.
+__global__ __launch_bounds__(256) void row_dot(const float *__restrict__ dL, const float *__restrict__ S,
+                                               uint64_t rows, uint32_t cols, float *__restrict__ out) {
+    const uint64_t r = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    if (r >= rows) return;
+    const unsigned lane = lane_id();
+    float acc = 0.0f;
+    for (uint32_t j = lane; j < cols; j += 64) acc += dL[r * cols + j] * S[r * cols + j];
+    for (int o = 32; o >= 1; o /= 2) acc += __shfl_xor(acc, o);
+    if (lane == 0) out[r] = acc;
+}
+
+// gamma's gradient: alpha times the rows' sums added in order (one thread).
+__global__ void gamma_grad(const float *__restrict__ row_sums, uint64_t rows, float *__restrict__ gamma) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float s = 0.0f;
+    for (uint64_t r = 0; r < rows; r++) s += row_sums[r];
+    gamma[1] = expf(gamma[0]) * s;
 }
 
 // confidence = sum over the probed lists of exp(logit - lse).
@@ -434,13 +467,43 @@ ky_ivf *ivf_build(Context &ctx, const Dense &points, const ky_ivf_params &p, ky_
         KY_HIP(hipMemcpyAsync(ivf->mean.ptr, muf.data(), sizeof(float) * d, hipMemcpyHostToDevice, st));
     }
     Router &R = ivf->router;
-    R.init(ctx, d, p.hidden, p.n_lists, p.seed);
+    R.init(ctx, d, p.hidden, p.n_lists, p.seed, true);
     const uint32_t B = (uint32_t)std::min<uint64_t>(p.batch, T);
     DevBuf<float> xb((size_t)B * d), h1((size_t)B * p.hidden), h2((size_t)B * p.hidden), lg((size_t)B * p.n_lists),
-        dh1((size_t)B * p.hidden), dh2((size_t)B * p.hidden), loss(B);
+        dh1((size_t)B * p.hidden), dh2((size_t)B * p.hidden), loss(B), xr((size_t)B * d),
+        sc((size_t)B * p.n_lists), rsum(B);
     const uint64_t steps_per_epoch = std::max<uint64_t>(1, T / B);
     std::vector<float> hloss(B);
     double last_loss = 0.0;
+    // alpha's start: the scale of the centroid scores whose softmax best fits the first
+    // batch's labels (the MLP's output is zero at the start), over powers of two.
+    ivf->gamma.alloc(4);
+    {
+        gather_rows<<<div_up((uint64_t)B * d, 256), 256, 0, st>>>(x, d, order.ptr, B, xr.ptr);
+        centroid_scores(st, *ivf, xr.ptr, (int)B, sc.ptr);
+        const double rms2 = 1.0 / ((double)ivf->inv_scale * ivf->inv_scale) * d;  // mean squared centred norm
+        double best_loss = INFINITY, best_alpha = 1.0 / rms2;
+        for (int e = -4; e <= 12; e++) {
+            const double alpha = std::ldexp(1.0 / rms2, e);
+            const float hg[4] = {(float)std::log(alpha), 0.0f, 0.0f, 0.0f};
+            KY_HIP(hipMemcpyAsync(ivf->gamma.ptr, hg, sizeof(hg), hipMemcpyHostToDevice, st));
+            fill_words(st, lg.ptr, (size_t)B * p.n_lists, 0u);
+            add_centroid_scores<<<div_up((uint64_t)B * p.n_lists, 256), 256, 0, st>>>(lg.ptr, sc.ptr,
+                                                                                   (uint64_t)B * p.n_lists,
+                                                                                   ivf->gamma.ptr);
+            softmax_xent(st, lg.ptr, B, (int)p.n_lists, t_idx.ptr, t_w.ptr, (int)k, 1.0f / (float)B, loss.ptr);
+            KY_HIP(hipMemcpyAsync(hloss.data(), loss.ptr, sizeof(float) * B, hipMemcpyDeviceToHost, st));
+            KY_HIP(hipStreamSynchronize(st));
+            double l = 0.0;
+            for (float v : hloss) l += v;
+            if (l < best_loss) {
+                best_loss = l;
+                best_alpha = alpha;
+            }
+        }
+        const float hg[4] = {(float)std::log(best_alpha), 0.0f, 0.0f, 0.0f};
+        KY_HIP(hipMemcpyAsync(ivf->gamma.ptr, hg, sizeof(hg), hipMemcpyHostToDevice, st));
+    }
     for (uint32_t ep = 0; ep < p.epochs; ep++) {
         double ep_loss = 0.0;
         for (uint64_t sidx = 0; sidx < steps_per_epoch; sidx++) {
@@ -448,10 +511,20 @@ ky_ivf *ivf_build(Context &ctx, const Dense &points, const ky_ivf_params &p, ky_
             normalize_rows<<<div_up((uint64_t)B * d, 256), 256, 0, st>>>(x, d, order.ptr + r0, B, ivf->mean.ptr,
                                                                           ivf->inv_scale, xb.ptr);
             R.forward(st, xb.ptr, B, h1.ptr, h2.ptr, lg.ptr);
+            gather_rows<<<div_up((uint64_t)B * d, 256), 256, 0, st>>>(x, d, order.ptr + r0, B, xr.ptr);
+            centroid_scores(st, *ivf, xr.ptr, (int)B, sc.ptr);
+            add_centroid_scores<<<div_up((uint64_t)B * p.n_lists, 256), 256, 0, st>>>(lg.ptr, sc.ptr,
+                                                                                   (uint64_t)B * p.n_lists,
+                                                                                   ivf->gamma.ptr);
             softmax_xent(st, lg.ptr, B, (int)p.n_lists, t_idx.ptr + r0 * k, t_w.ptr + r0 * k, (int)k, 1.0f / (float)B,
                          loss.ptr);
             R.backward(st, xb.ptr, B, h1.ptr, h2.ptr, lg.ptr, dh2.ptr, dh1.ptr);
+            // gamma: dLoss/dgamma = alpha sum dL S.
+            row_dot<<<div_up(B, 4), 256, 0, st>>>(lg.ptr, sc.ptr, B, p.n_lists, rsum.ptr);
+            gamma_grad<<<1, 1, 0, st>>>(rsum.ptr, B, ivf->gamma.ptr);
+            KY_HIP(hipGetLastError());
             R.adam_step(st, p.lr);
+            adam(st, ivf->gamma.ptr, ivf->gamma.ptr + 1, ivf->gamma.ptr + 2, ivf->gamma.ptr + 3, 1, p.lr, R.step);
             if (ep + 1 == p.epochs) {
                 KY_HIP(hipMemcpyAsync(hloss.data(), loss.ptr, sizeof(float) * B, hipMemcpyDeviceToHost, st));
                 KY_HIP(hipStreamSynchronize(st));
@@ -466,6 +539,11 @@ ky_ivf *ivf_build(Context &ctx, const Dense &points, const ky_ivf_params &p, ky_
         normalize_rows<<<div_up((uint64_t)B * d, 256), 256, 0, st>>>(x, d, order.ptr, B, ivf->mean.ptr,
                                                                       ivf->inv_scale, xb.ptr);
         R.forward(st, xb.ptr, B, h1.ptr, h2.ptr, lg.ptr);
+        gather_rows<<<div_up((uint64_t)B * d, 256), 256, 0, st>>>(x, d, order.ptr, B, xr.ptr);
+        centroid_scores(st, *ivf, xr.ptr, (int)B, sc.ptr);
+        add_centroid_scores<<<div_up((uint64_t)B * p.n_lists, 256), 256, 0, st>>>(lg.ptr, sc.ptr,
+                                                                               (uint64_t)B * p.n_lists,
+                                                                               ivf->gamma.ptr);
         const uint32_t P = std::min<uint32_t>(8, p.n_lists);
         DevBuf<uint32_t> ti((size_t)B * 8);
         DevBuf<float> tv((size_t)B * 8), mass(B);
@@ -477,6 +555,10 @@ ky_ivf *ivf_build(Context &ctx, const Dense &points, const ky_ivf_params &p, ky_
         double s = 0.0;
         for (float v : hm) s += v;
         top_mass = (float)(s / B);
+        float hg = 0.0f;
+        KY_HIP(hipMemcpyAsync(&hg, ivf->gamma.ptr, sizeof(float), hipMemcpyDeviceToHost, st));
+        KY_HIP(hipStreamSynchronize(st));
+        ivf->alpha = std::exp(hg);
     }
     const double train_ms = timer.stop();
     if (info) {
@@ -487,6 +569,19 @@ ky_ivf *ivf_build(Context &ctx, const Dense &points, const ky_ivf_params &p, ky_
         info->label_recall_top = top_mass;
     }
     return ivf.release();
+}
+
+void ivf_layout(const ky_ivf &ivf, float *centroids, uint32_t *assign, uint32_t *list_ptr, uint32_t *list_rows,
+                float *alpha) {
+    Context &ctx = *ivf.ctx;
+    KY_HIP(hipSetDevice(ctx.device));
+    std::vector<CopyPart> parts;
+    if (centroids) parts.push_back({centroids, ivf.centroids.ptr, sizeof(float) * ivf.n_lists * ivf.d});
+    if (assign) parts.push_back({assign, ivf.assign.ptr, sizeof(uint32_t) * ivf.n});
+    if (list_ptr) parts.push_back({list_ptr, ivf.list_ptr.ptr, sizeof(uint32_t) * (ivf.n_lists + 1)});
+    if (list_rows) parts.push_back({list_rows, ivf.list_rows.ptr, sizeof(uint32_t) * ivf.n});
+    if (!parts.empty()) download_parts(ctx, parts);
+    if (alpha) *alpha = ivf.alpha;
 }
 
 void ivf_free(ky_ivf *ivf) {
@@ -516,17 +611,23 @@ void ivf_search(Context &ctx, const ky_ivf &ivf, const Dense &points, uint32_t n
     {
         const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(n, (256ull << 20) / ivf.n_lists));
         DevBuf<float> xb(chunk * d), S(chunk * ivf.n_lists), tv(chunk * P), lse(chunk);
-        DevBuf<float> h1, h2;
+        DevBuf<float> h1, h2, C;
         if (use_router) {
             h1.alloc(chunk * ivf.router.hidden);
             h2.alloc(chunk * ivf.router.hidden);
+            C.alloc(chunk * ivf.n_lists);
         }
         for (uint64_t r0 = 0; r0 < n; r0 += chunk) {
             const uint64_t rows = std::min<uint64_t>(chunk, n - r0);
             if (use_router) {
+                // alpha times the centroid scores plus the MLP's correction.
                 normalize_rows<<<div_up(rows * d, 256), 256, 0, st>>>(x + r0 * d, d, nullptr, rows, ivf.mean.ptr,
                                                                        ivf.inv_scale, xb.ptr);
                 const_cast<Router &>(ivf.router).forward(st, xb.ptr, rows, h1.ptr, h2.ptr, S.ptr);
+                centroid_scores(st, ivf, x + r0 * d, (int)rows, C.ptr);
+                add_centroid_scores<<<div_up(rows * ivf.n_lists, 256), 256, 0, st>>>(S.ptr, C.ptr,
+                                                                                     rows * ivf.n_lists,
+                                                                                     ivf.gamma.ptr);
             } else {
                 centroid_scores(st, ivf, x + r0 * d, (int)rows, S.ptr);
             }

@@ -198,9 +198,21 @@ __global__ void pack_columns(const float *__restrict__ x, uint64_t n, uint32_t d
 // segment b % n_seg for the queries of group b / n_seg, and writes its `keep` pairs at
 // slot i * n_seg + segment (merge_segments then keeps each query's smallest), so a
 // short list of queries still fills the device.
-template <bool EXACT, bool COS, int DP, int CAP>
+// The approximate keys of nearest_keys (crates/kyto-gpu/src/kernels_matrix.rs:140-257):
+// MODE 0 the expanded key fl(fl(o_q + o_p) - 2 q.p) of the search; MODE 1 the squared
+// differences summed in component order in f32; MODE 2 the cosine key
+// 1 - dot / (|q| |p|) (dot in component order, the caller's f32 norms) clipped to
+// [0, 2], 1 when a norm is 0. Queries are rows of `qa` (null: rows of the points); the
+// point skip[i] is left out for query i (null: the query's own row).
+struct Pairwise {
+    const float *qa = nullptr;
+    const uint32_t *skip = nullptr;
+    const float *qn = nullptr, *pn = nullptr;
+};
+
+template <bool EXACT, bool COS, int DP, int CAP, int MODE = 0>
 __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, const float *__restrict__ offs,
-                                                   const double *__restrict__ norms,
+                                                   const double *__restrict__ norms, const Pairwise pw,
                                                    uint64_t n, uint32_t d, const uint32_t *__restrict__ q_list,
                                                    uint64_t q_first, uint64_t q_count, uint32_t keep, uint32_t n_seg,
                                                    const double *__restrict__ seed_keys,
@@ -215,7 +227,8 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
     __shared__ float sqo[QB];
     __shared__ K bkey[QB * CAP];
     __shared__ uint32_t brow[QB * CAP];
-    __shared__ uint32_t bcnt[QB], trow[QB], qrow[QB];
+    __shared__ uint32_t bcnt[QB], trow[QB], qrow[QB], qskip[QB];
+    __shared__ float qnorm[QB];
     __shared__ K tkey[QB];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
     const uint64_t i0 = (uint64_t)(blockIdx.x / n_seg) * QB;
@@ -228,6 +241,8 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
         uint32_t qg = 0xFFFFFFFFu;
         if (i < q_count) qg = q_list ? q_list[i] : (uint32_t)(q_first + i);
         qrow[tid] = qg;
+        qskip[tid] = (pw.skip && i < q_count) ? pw.skip[i] : qg;
+        qnorm[tid] = (MODE == 2 && qg != 0xFFFFFFFFu) ? pw.qn[qg] : 0.0f;
         bcnt[tid] = 0;
         tkey[tid] = ~K(0);
         trow[tid] = ~0u;
@@ -243,14 +258,15 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
                 }
             }
         }
-        sqo[tid] = (!EXACT && qg != 0xFFFFFFFFu) ? offs[qg] : 0.0f;
+        sqo[tid] = (!EXACT && MODE == 0 && qg != 0xFFFFFFFFu) ? offs[qg] : 0.0f;
     }
     __syncthreads();
+    const float *qsrc = (MODE != 0 && pw.qa) ? pw.qa : a;
     for (unsigned i = tid; i < QB * DP; i += 256) {
         unsigned r = i / DP, c = i % DP;
         uint32_t qg = qrow[r];
         float v = 0.0f;
-        if (qg != 0xFFFFFFFFu && c < stride) v = a[(uint64_t)qg * stride + c];
+        if (qg != 0xFFFFFFFFu && c < stride) v = qsrc[(uint64_t)qg * stride + c];
         if constexpr (EXACT) sqd[r * DP + c] = (double)v;
         else sq[r * QS + c] = v;
     }
@@ -287,7 +303,8 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
 #pragma unroll
             for (int c = 0; c < DP; c++) pr[c] = 0.0f;
         }
-        const float po = (!EXACT && valid) ? offs[p] : 0.0f;
+        const float po = (!EXACT && MODE == 0 && valid) ? offs[p] : 0.0f;
+        const float pnv = (MODE == 2 && valid) ? pw.pn[p] : 0.0f;
         for (unsigned qi = 0; qi < nq; qi++) {
             const unsigned ql = wave * 16 + qi;
             const uint32_t qg = qrow[ql];
@@ -295,7 +312,7 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
             uint32_t *br = brow + ql * CAP;
             if (bcnt[ql] > CAP - TP) compact<K, CAP>(bk, br, &bcnt[ql], &tkey[ql], &trow[ql], keep);
             K key;
-            if constexpr (!EXACT) {
+            if constexpr (!EXACT && MODE == 0) {
                 float dot = 0.0f;
                 const float4 *qv = reinterpret_cast<const float4 *>(sq + ql * QS);
 #pragma unroll
@@ -308,6 +325,30 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
                 }
                 float kf = (sqo[ql] + po) - 2.0f * dot;
                 key = order_f32(kf);
+            } else if constexpr (!EXACT) {
+                // Component order in f32, each product rounded before its add (this
+                // file is compiled without contraction); past the width both rows are 0.
+                const float *qv = sq + ql * QS;
+                float acc = 0.0f;
+#pragma unroll
+                for (int c = 0; c < DP; c++) {
+                    if constexpr (MODE == 1) {
+                        const float t = qv[c] - pr[c];
+                        acc += t * t;
+                    } else {
+                        acc += qv[c] * pr[c];
+                    }
+                }
+                if constexpr (MODE == 2) {
+                    const float qn = qnorm[ql];
+                    float kc = 1.0f;
+                    if (qn != 0.0f && pnv != 0.0f) {
+                        kc = 1.0f - acc / (qn * pnv);
+                        kc = kc < 0.0f ? 0.0f : (kc > 2.0f ? 2.0f : kc);
+                    }
+                    acc = kc;
+                }
+                key = order_f32(acc);
             } else {
                 double l[8] = {0, 0, 0, 0, 0, 0, 0, 0};
                 // The query's row: every LDS read first, then the arithmetic.
@@ -336,7 +377,7 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
             }
             const K tk = tkey[ql];
             const uint32_t tr = trow[ql];
-            const bool hit = valid && p != qg && (key < tk || (key == tk && (uint32_t)p < tr));
+            const bool hit = valid && p != qskip[ql] && (key < tk || (key == tk && (uint32_t)p < tr));
             const uint64_t mask = ballot(hit);
             if (mask) {
                 const unsigned base = bcnt[ql];
@@ -571,15 +612,16 @@ __global__ void graph_rows(const double *__restrict__ keys, const uint32_t *__re
 
 // ================================================================== host side ====
 
-template <bool EXACT, bool COS, int CAP>
+template <bool EXACT, bool COS, int CAP, int MODE = 0>
 static void launch_scan(int dp, hipStream_t st, unsigned blocks, const float *a, const float *offs,
                         const double *norms, uint64_t n, uint32_t d, const uint32_t *list, uint64_t q_first,
                         uint64_t q_count, uint32_t keep, uint32_t *out_rows, void *out_keys, uint32_t *out_count,
-                        uint32_t n_seg = 1, const double *seed_keys = nullptr, const uint32_t *seed_rows = nullptr) {
-#define KY_SCAN(DPV)                                                                                          \
-    scan_points<EXACT, COS, DPV, CAP><<<blocks * n_seg, 256, 0, st>>>(a, offs, norms, n, d, list, q_first, q_count, \
-                                                                      keep, n_seg, seed_keys, seed_rows, out_rows,  \
-                                                                      out_keys, out_count)
+                        uint32_t n_seg = 1, const double *seed_keys = nullptr, const uint32_t *seed_rows = nullptr,
+                        const Pairwise &pw = Pairwise()) {
+#define KY_SCAN(DPV)                                                                                         \
+    scan_points<EXACT, COS, DPV, CAP, MODE><<<blocks * n_seg, 256, 0, st>>>(a, offs, norms, pw, n, d, list, q_first, \
+                                                                            q_count, keep, n_seg, seed_keys,       \
+                                                                            seed_rows, out_rows, out_keys, out_count)
     switch (dp) {
         case 8: KY_SCAN(8); break;
         case 16: KY_SCAN(16); break;
@@ -1068,44 +1110,76 @@ void neighbors_run(Context &ctx, const Dense &points, uint32_t n_components, uin
     }
 }
 
-void nearest_keys_run(Context &ctx, const Dense &points, uint32_t n_components, uint64_t first, uint64_t count,
-                      uint32_t m, int precision, float *h_keys, uint32_t *h_rows) {
-    const uint32_t d = check_search(points, n_components, 2, KY_METRIC_EUCLIDEAN);
-    KY_CHECK(first + count <= points.rows, KY_E215, "nearest_keys: rows out of range");
-    KY_CHECK(m >= 1 && m <= 64 && (uint64_t)m <= points.rows - 1, KY_E215, "nearest_keys: m out of range");
-    KY_CHECK(precision == KY_PRECISION_FP32, KY_E501, "nearest_keys: only FP32 keys are exposed");
-    hipStream_t st = ctx.stream;
+// Pad rows of w floats (stride ld) to dp floats, zeros past w.
+__global__ void pad_rows_f32(const float *__restrict__ x, uint64_t n, uint32_t w, uint32_t ld, uint32_t dp,
+                             float *__restrict__ out) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n * dp) return;
+    const uint64_t r = i / dp;
+    const uint32_t c = (uint32_t)(i % dp);
+    out[i] = c < w ? x[r * ld + c] : 0.0f;
+}
+
+void nearest_keys_run(Context &ctx, const float *h_queries, uint64_t nq, const Dense &points, int key,
+                      const float *h_qn, const float *h_pn, const uint32_t *h_skip, uint32_t m, float *h_keys,
+                      uint32_t *h_rows) {
     const uint64_t n = points.rows;
-    KnnShape s = knn_shape(&ctx, n, d, 1, KY_METRIC_EUCLIDEAN, KY_PRECISION_FP32, false, count, 0);
+    const uint32_t w = points.cols;
+    KY_CHECK(w >= 1 && n >= 1, KY_E215, "nearest_keys: points without rows or components");
+    KY_CHECK(n - 1 <= 0xFFFFFFFEull, KY_E505, "nearest_keys: 2^32 points or more");
+    KY_CHECK(key == KY_KEY_SQUARED_EUCLIDEAN || key == KY_KEY_COSINE, KY_E215, "nearest_keys: unknown key");
+    KY_CHECK(key != KY_KEY_COSINE || (h_qn && h_pn), KY_E215, "nearest_keys: the cosine key needs both norms");
+    if (nq == 0) return;
+    bool skips_none = true;
+    for (uint64_t i = 0; i < nq && skips_none; i++) skips_none = h_skip[i] == 0xFFFFFFFFu;
+    KY_CHECK(m >= 1 && ((uint64_t)m < n || ((uint64_t)m == n && skips_none)), KY_E215,
+             "nearest_keys: 1 <= m <= the points left after skip");
+    KY_CHECK(m <= 64, KY_E501, "nearest_keys: no device kernel for more than 64 keys a query");
+    KY_CHECK(w <= 64, KY_E501, "nearest_keys: no device kernel for more than 64 components");
+    hipStream_t st = ctx.stream;
+    const uint32_t dp = (uint32_t)padded_width(w);
     SizeArena z;
-    KnnPrep zp;
-    if (d != points.cols) z.take<float>(n * d);
-    knn_take_prep(z, s, zp);
-    z.take<double>(prep_partial_count(s));
-    z.take<uint32_t>(count * m);
-    z.take<float>(count * m);
-    z.take<uint32_t>(count);
+    auto layout = [&](auto &a, float *&pts, float *&qs, float *&raw, uint32_t *&skip, float *&qn, float *&pn,
+                      uint32_t *&rows, float *&keys, uint32_t *&count) {
+        pts = a.template take<float>(n * dp);
+        qs = a.template take<float>(nq * dp);
+        raw = a.template take<float>(nq * w);
+        skip = a.template take<uint32_t>(nq);
+        qn = a.template take<float>(key == KY_KEY_COSINE ? nq : 0);
+        pn = a.template take<float>(key == KY_KEY_COSINE ? n : 0);
+        rows = a.template take<uint32_t>(nq * m);
+        keys = a.template take<float>(nq * m);
+        count = a.template take<uint32_t>(nq);
+    };
+    float *pts, *qs, *raw, *qn, *pn, *keys;
+    uint32_t *skip, *rows, *count;
+    layout(z, pts, qs, raw, skip, qn, pn, rows, keys, count);
     DevBuf<unsigned char> region(z.high);
     Arena a(region.ptr, z.high);
-    const float *pts = points.data.ptr;
-    if (d != points.cols) {
-        float *p = a.take<float>(n * d);
-        knn_pack(ctx, points.data.ptr, n, d, points.cols, p);
-        pts = p;
+    layout(a, pts, qs, raw, skip, qn, pn, rows, keys, count);
+    pad_rows_f32<<<div_up(n * dp, 256), 256, 0, st>>>(points.data.ptr, n, w, w, dp, pts);
+    KY_HIP(hipGetLastError());
+    std::vector<CopyPart> up = {{raw, h_queries, sizeof(float) * nq * w}, {skip, h_skip, sizeof(uint32_t) * nq}};
+    if (key == KY_KEY_COSINE) {
+        up.push_back({qn, h_qn, sizeof(float) * nq});
+        up.push_back({pn, h_pn, sizeof(float) * n});
     }
-    KnnPrep prep;
-    knn_take_prep(a, s, prep);
-    knn_prepare(ctx, s, pts, prep, a.take<double>(prep_partial_count(s)));
-    KY_CHECK(prep.finite, KY_E215, "nearest_keys: the embedding holds a value that is not finite");
-    KY_CHECK(prep.filter_ok, KY_E501, "nearest_keys: row norms too large for f32 keys");
-    uint32_t *c_rows = a.take<uint32_t>(count * m);
-    float *c_keys = a.take<float>(count * m);
-    uint32_t *c_count = a.take<uint32_t>(count);
-    launch_scan<false, false, 128>(s.dp, st, div_up(count, QB), prep.cent, prep.offs, nullptr, n, d, nullptr, first,
-                                   count, m, c_rows, c_keys, c_count);
-    KY_HIP(hipMemcpyAsync(h_keys, c_keys, sizeof(float) * count * m, hipMemcpyDeviceToHost, st));
-    KY_HIP(hipMemcpyAsync(h_rows, c_rows, sizeof(uint32_t) * count * m, hipMemcpyDeviceToHost, st));
-    KY_HIP(hipStreamSynchronize(st));
+    upload_parts(ctx, up);
+    pad_rows_f32<<<div_up(nq * dp, 256), 256, 0, st>>>(raw, nq, w, w, dp, qs);
+    KY_HIP(hipGetLastError());
+    Pairwise pw;
+    pw.qa = qs;
+    pw.skip = skip;
+    pw.qn = qn;
+    pw.pn = pn;
+    const unsigned blocks = div_up(nq, QB);
+    if (key == KY_KEY_SQUARED_EUCLIDEAN)
+        launch_scan<false, false, 128, 1>(dp, st, blocks, pts, nullptr, nullptr, n, w, nullptr, 0, nq, m, rows, keys,
+                                          count, 1, nullptr, nullptr, pw);
+    else
+        launch_scan<false, false, 128, 2>(dp, st, blocks, pts, nullptr, nullptr, n, w, nullptr, 0, nq, m, rows, keys,
+                                          count, 1, nullptr, nullptr, pw);
+    download_parts(ctx, {{h_keys, keys, sizeof(float) * nq * m}, {h_rows, rows, sizeof(uint32_t) * nq * m}});
 }
 
 }  // namespace ky

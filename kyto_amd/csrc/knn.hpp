@@ -119,7 +119,10 @@ void static_scratch_init(int device);
 // neighbors_run in the fixed region, in the largest round that fits it.
 void neighbors_static(Context &ctx, const Dense &points, uint32_t n_components, uint32_t n_neighbors, int metric,
                       int precision, uint32_t *h_idx, float *h_dist, ky_knn_info *info);
-void nearest_keys_run(Context &ctx, const Dense &points, uint32_t n_components, uint64_t first, uint64_t count,
-                      uint32_t m, int precision, float *h_keys, uint32_t *h_rows);
+// DeviceKernels::nearest_keys (launch_matrix.rs:253-286): each query's m smallest f32
+// keys against every point, the point skip[i] left out, ranked by (key, point).
+void nearest_keys_run(Context &ctx, const float *h_queries, uint64_t nq, const Dense &points, int key,
+                      const float *h_qn, const float *h_pn, const uint32_t *h_skip, uint32_t m, float *h_keys,
+                      uint32_t *h_rows);
 
 }  // namespace ky

@@ -404,7 +404,7 @@ void adam(hipStream_t st, float *w, const float *g, float *m, float *v, int64_t 
 
 // ------------------------------------------------------------------ the MLP ----
 
-void Router::init(Context &ctx, uint32_t in, uint32_t hid, uint32_t out, uint64_t seed) {
+void Router::init(Context &ctx, uint32_t in, uint32_t hid, uint32_t out, uint64_t seed, bool zero_last) {
     d_in = in;
     hidden = hid;
     n_out = out;
@@ -422,7 +422,7 @@ void Router::init(Context &ctx, uint32_t in, uint32_t hid, uint32_t out, uint64_
     const uint32_t fan_in[3] = {in, hid, hid};
     for (int layer = 0; layer < 3; layer++) {
         const size_t cnt = sizes[2 * layer];
-        const float bound = std::sqrt(6.0f / (float)fan_in[layer]);
+        const float bound = zero_last && layer == 2 ? 0.0f : std::sqrt(6.0f / (float)fan_in[layer]);
         for (size_t i = 0; i < cnt; i++) {
             uint64_t z = seed * 0x9E3779B97F4A7C15ULL + (uint64_t)layer * 0xD1B54A32D192ED03ULL + i;
             z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;

@@ -26,7 +26,9 @@ struct Router {
     DevBuf<float> params, grads, m1, m2;
     DevBuf<float> colsum_scratch;  // the bias gradients' chunk sums
     DevBuf<float> split_scratch;   // split-K parts of the weight gradients
-    void init(Context &ctx, uint32_t in, uint32_t hid, uint32_t out, uint64_t seed);
+    // He-uniform weights from the seed, zero biases; zero_last: the output layer's
+    // weights zero too (the MLP starts as a zero correction of another score).
+    void init(Context &ctx, uint32_t in, uint32_t hid, uint32_t out, uint64_t seed, bool zero_last = false);
     float *W(int layer, DevBuf<float> &buf) const;
     float *b(int layer, DevBuf<float> &buf) const;
     void forward(hipStream_t st, const float *X, int64_t rows, float *H1, float *H2, float *L);

@@ -290,13 +290,23 @@ class Device:
             N.check(self._lib.ky_graph_free(h))
         return ConnectivityGraph(indptr, cols[:edges], w[:edges], sigma[:n], rho[:n], _info(info))
 
-    def nearest_keys(self, points: "DenseMatrix", first: int, count: int, m: int,
-                     n_components: int | None = None):
-        keys = np.empty((count, m), np.float32)
-        rows = np.empty((count, m), np.uint32)
-        N.check(self._lib.ky_nearest_keys(self._h, points.handle, int(n_components or 0), int(first), int(count),
-                                          int(m), N.PRECISION["fp32"], N.ptr(keys, N.f32p), N.ptr(rows, N.u32p)))
-        return keys, rows
+    def nearest_keys(self, queries, points: "DenseMatrix", m: int, key: str = "squared_euclidean", skip=None,
+                     query_norms=None, point_norms=None):
+        """DeviceKernels::nearest_keys (ky_nearest_keys): each query row's m smallest f32
+        keys against every point, the point skip[i] left out (UINT32_MAX: none), ranked
+        by (key, point). Returns (keys f32, rows u32), n_queries x m."""
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        sk = np.full(nq, 0xFFFFFFFF, np.uint32) if skip is None else np.ascontiguousarray(skip, np.uint32)
+        keys = np.empty((max(nq, 1), m), np.float32)
+        rows = np.empty((max(nq, 1), m), np.uint32)
+        qn = None if query_norms is None else np.ascontiguousarray(query_norms, np.float32)
+        pn = None if point_norms is None else np.ascontiguousarray(point_norms, np.float32)
+        N.check(self._lib.ky_nearest_keys(self._h, N.ptr(q, N.f32p), nq, points.handle, N.KEY[key],
+                                          None if qn is None else N.ptr(qn, N.f32p),
+                                          None if pn is None else N.ptr(pn, N.f32p), N.ptr(sk, N.u32p), int(m),
+                                          N.ptr(keys, N.f32p), N.ptr(rows, N.u32p)))
+        return keys[:nq], rows[:nq]
 
     # ---------------------------------------------------------------------- IVF --
 
@@ -466,6 +476,17 @@ class IvfIndex:
                                             N.ptr(dist, N.f32p), byref(info)))
         indptr = np.arange(0, n * k + 1, k, dtype=np.uint64)
         return Graph(int(n_neighbors), indptr, idx, dist, _info(info))
+
+    def layout(self, n_rows: int, d: int, n_lists: int) -> dict:
+        """The k-means layout and the router's alpha (ky_ivf_layout)."""
+        c = np.empty((n_lists, d), np.float32)
+        assign = np.empty(n_rows, np.uint32)
+        ptr = np.empty(n_lists + 1, np.uint32)
+        rows = np.empty(n_rows, np.uint32)
+        alpha = ctypes.c_float(0)
+        N.check(self.dev._lib.ky_ivf_layout(self._h, N.ptr(c, N.f32p), N.ptr(assign, N.u32p), N.ptr(ptr, N.u32p),
+                                            N.ptr(rows, N.u32p), byref(alpha)))
+        return {"centroids": c, "assign": assign, "list_ptr": ptr, "list_rows": rows, "alpha": alpha.value}
 
     def free(self):
         if self._h:

@@ -447,6 +447,41 @@ def knn_exact_rows(points, k, queries, metric="euclidean", threads=None):
     return keys, rows, int(fb[0])
 
 
+def nearest_keys(queries, points, m, key="squared_euclidean", skip=None, query_norms=None, point_norms=None):
+    """DeviceKernels::nearest_keys (crates/kyto-gpu/src/launch_matrix.rs:240-286) with
+    its kernels (kernels_matrix.rs:140-257): squared_distances sums (q - p)^2 in
+    component order in f32, dot_products sums q p likewise and cosine_keys makes
+    1 - dot / (|q| |p|) in f32, clipped to [0, 2], 1 for a zero norm; `smallest` keeps
+    each query's m smallest by (key, point), the point skip[q] left out. Each f32
+    operation rounded on its own (no fused multiply-add). Returns (keys, points)."""
+    q = np.ascontiguousarray(queries, np.float32)
+    p = np.ascontiguousarray(points, np.float32)
+    nq, n = q.shape[0], p.shape[0]
+    keys = np.empty((nq, m), np.float32)
+    rows = np.empty((nq, m), np.uint32)
+    for i in range(nq):
+        acc = np.zeros(n, np.float32)
+        for c in range(q.shape[1]):
+            if key == "squared_euclidean":
+                t = np.float32(q[i, c]) - p[:, c]
+                acc = acc + t * t
+            else:
+                acc = acc + np.float32(q[i, c]) * p[:, c]
+        if key == "cosine":
+            qn = np.float32(query_norms[i])
+            pn = np.asarray(point_norms, np.float32)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                k = np.float32(1.0) - acc / (qn * pn)
+            k = np.clip(k, np.float32(0.0), np.float32(2.0))
+            acc = np.where((qn == 0) | (pn == 0), np.float32(1.0), k).astype(np.float32)
+        order = np.lexsort((np.arange(n), acc))
+        if skip is not None and skip[i] != 0xFFFFFFFF:
+            order = order[order != skip[i]]
+        keys[i] = acc[order[:m]]
+        rows[i] = order[:m]
+    return keys, rows
+
+
 def graph(keys, rows, metric="euclidean"):
     """`Space::graph` (neighbors.rs:412-435): per row, entries sorted by index,
     distance = f32(sqrt(key)) (Euclidean) or f32(key) (cosine)."""

@@ -104,17 +104,38 @@ def test_refusals(device):
         device.neighbors(bad, 2)
 
 
-def test_nearest_keys_are_the_smallest_approximate_keys(device):
+@pytest.mark.parametrize("key", ["squared_euclidean", "cosine"])
+def test_nearest_keys_equal_the_reference_kernels(device, key):
+    # DeviceKernels::nearest_keys: a query matrix of its own, a skip per query, the
+    # f32 keys of kernels_matrix.rs in component order: the oracle's bits, ranked by
+    # (key, point).
     pts = clustered(3, 2100, 16, 9, 0.0)
+    pts[7] = 0.0  # a zero row: cosine key 1
+    rng = np.random.default_rng(4)
+    queries = np.concatenate([pts[100:150], rng.standard_normal((20, 16)).astype(np.float32), pts[7:8]])
+    skip = np.full(len(queries), 0xFFFFFFFF, np.uint32)
+    skip[:50] = np.arange(100, 150)  # the queries that are rows of the points skip themselves
+    qn = pn = None
+    if key == "cosine":
+        qn = np.sqrt((queries.astype(np.float64) ** 2).sum(axis=1)).astype(np.float32)
+        pn = np.sqrt((pts.astype(np.float64) ** 2).sum(axis=1)).astype(np.float32)
     m = device.upload_dense(pts)
-    keys, rows = device.nearest_keys(m, 100, 50, 20)
-    assert np.all(np.diff(keys, axis=1) >= 0)
-    for i in range(50):
-        q = 100 + i
-        d = ((pts.astype(np.float64) - pts[q].astype(np.float64)) ** 2).sum(axis=1)
-        d[q] = np.inf
-        true = np.sort(d)[:20]
-        assert np.abs(keys[i].astype(np.float64) - true).max() < 1e-3 * (1 + true.max())
+    try:
+        keys, rows = device.nearest_keys(queries, m, 20, key=key, skip=skip, query_norms=qn, point_norms=pn)
+        with pytest.raises(N.KytoError, match="KY-E215"):
+            device.nearest_keys(queries[:2], m, 2100, skip=skip[:2])  # m above the points left after skip
+        with pytest.raises(N.KytoError, match="KY-E215"):
+            device.nearest_keys(queries[:2], m, 5, key="cosine")  # no norms
+    finally:
+        m.free()
+    want_keys, want_rows = oracle.nearest_keys(queries, pts, 20, key=key, skip=skip, query_norms=qn,
+                                               point_norms=pn)
+    assert np.array_equal(rows, want_rows)
+    assert np.array_equal(keys.view(np.uint32), want_keys.view(np.uint32))
+    # The Euclidean keys are within f32 rounding of the exact distances.
+    if key == "squared_euclidean":
+        exact = ((queries[:, None, :].astype(np.float64) - pts[rows].astype(np.float64)) ** 2).sum(axis=2)
+        assert np.abs(keys - exact).max() <= 1e-5 * (1 + exact.max())
 
 
 def _large_clustered(n, d, clusters, seed):

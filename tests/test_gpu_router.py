@@ -135,3 +135,70 @@ def test_exact_fallback_of_a_row_list_above_a_device_round(device):
             index.free()
     finally:
         m.free()
+
+
+def test_kmeans_layout_is_nearest_centroids_and_their_means(device):
+    # 2,048 rows (every row in the k-means sample: at most 64 a list) around 40 far
+    # apart centres: at the fixed point Lloyd's iterations reach, every row's list is its
+    # nearest centroid (f64 distances, numpy brute force) up to the error of the BF16
+    # scores 2 x.c - |c|^2 that choose it (bf16 operands: 2^-8 of 2 |x| |c|, doubled;
+    # where a cluster holds two centroids that error decides between them), and every
+    # non-empty list's centroid is the f64 mean of its rows, rounded to f32; the lists
+    # hold exactly the rows assigned to them, ascending.
+    rng = np.random.default_rng(17)
+    centres = rng.standard_normal((40, 16)) * 50.0
+    pts = (centres[rng.integers(0, 40, 2048)] + rng.standard_normal((2048, 16))).astype(np.float32)
+    m = device.upload_dense(pts)
+    try:
+        index = device.ivf_build(m, n_lists=40, kmeans_iters=10, seed=0, hidden=32, train_queries=1024,
+                                 batch=256, epochs=1, lr=1e-3, label_k=5)
+        try:
+            lay = index.layout(2048, 16, 40)
+        finally:
+            index.free()
+    finally:
+        m.free()
+    c, assign = lay["centroids"].astype(np.float64), lay["assign"]
+    d2 = ((pts.astype(np.float64)[:, None, :] - c[None, :, :]) ** 2).sum(axis=2)
+    chosen = d2[np.arange(2048), assign]
+    bound = 2.0 * 2.0 ** -8 * 2.0 * np.linalg.norm(pts, axis=1) * np.linalg.norm(c, axis=1).max() + 1e-3
+    assert (chosen - d2.min(axis=1) <= 2.0 * bound).all()
+    # A planted mistake (each row in its second nearest list) breaks the bound.
+    second = np.argsort(d2, axis=1)[:, 1]
+    assert not (d2[np.arange(2048), second] - d2.min(axis=1) <= 2.0 * bound).all()
+    for l in range(40):
+        members = np.flatnonzero(assign == l)
+        if len(members):
+            want = pts[members].astype(np.float64).mean(axis=0).astype(np.float32)
+            assert np.abs(lay["centroids"][l] - want).max() <= 1e-6 * max(1.0, np.abs(want).max())
+        rows = lay["list_rows"][lay["list_ptr"][l]:lay["list_ptr"][l + 1]]
+        assert np.array_equal(rows, members)
+    assert lay["list_ptr"][-1] == 2048 and lay["alpha"] > 0
+
+
+def test_router_gains_over_centroid_probing(device):
+    # 250,000 rows of a 60-component Gaussian mixture, 1,024 lists: the router's logits
+    # start as the scaled nearest-centroid scores (the MLP's output layer at zero) and
+    # training on the exact graph's list labels moves them; probing its top 8 lists
+    # finds more of the exact neighbours than probing the 8 nearest centroids.
+    rng = np.random.default_rng(29)
+    centres = rng.standard_normal((60, 50)).astype(np.float32) * 2.5
+    scales = rng.uniform(0.5, 2.0, 60).astype(np.float32)
+    lab = rng.integers(0, 60, 250_000)
+    pts = (centres[lab] + rng.standard_normal((250_000, 50)).astype(np.float32) * scales[lab, None]).astype(np.float32)
+    m = device.upload_dense(pts)
+    try:
+        exact = device.neighbors(m, 15, precision="bf16").rows()
+        index = device.ivf_build(m, n_lists=1024, kmeans_iters=10, seed=0, hidden=256, train_queries=200_000,
+                                 batch=4096, epochs=4, lr=1e-3, label_k=14)
+        try:
+            r_router = _recall(index.search(m, 15, n_probe=8, use_router=True, confidence=0.0).rows(), exact)
+            r_centroid = _recall(index.search(m, 15, n_probe=8, use_router=False).rows(), exact)
+            alpha = index.layout(250_000, 50, 1024)["alpha"]
+        finally:
+            index.free()
+    finally:
+        m.free()
+    print(f"recall@15 router {r_router:.4f} centroid probing {r_centroid:.4f} alpha {alpha:.4g} "
+          f"label mass in top 8 {index.info['label_recall_top']:.4f}")
+    assert r_router > r_centroid

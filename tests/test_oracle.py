@@ -383,3 +383,17 @@ def test_listed_queries_equal_the_full_search():
     qs = np.array([1499, 3, 700, 0, 42, 42, 1000], np.uint32)
     k2, r2, _ = oracle.knn_exact_rows(pts, 14, qs)
     assert np.array_equal(r2, rows[qs]) and np.array_equal(k2.view(np.uint64), keys[qs].view(np.uint64))
+
+
+def test_nearest_keys_known_answers():
+    # kernels_matrix.rs:140-257 on points whose keys are exact: a line (squared
+    # distances, ties to the earlier point, the skipped point left out) and cosine
+    # keys of axis vectors (0 same direction, 1 orthogonal or a zero row, 2 opposite).
+    pts = np.array([[0.0], [1.0], [3.0], [7.0], [1.0]], np.float32)
+    keys, rows = oracle.nearest_keys(np.array([[1.0], [5.0]], np.float32), pts, 3, skip=[1, 0xFFFFFFFF])
+    assert rows.tolist() == [[4, 0, 2], [2, 3, 1]]
+    assert keys.tolist() == [[0.0, 1.0, 4.0], [4.0, 4.0, 16.0]]
+    axes = np.array([[1, 0], [0, 1], [-1, 0], [0, 0], [2, 0]], np.float32)
+    norms = np.sqrt((axes ** 2).sum(axis=1)).astype(np.float32)
+    keys, rows = oracle.nearest_keys(axes[:1], axes, 5, key="cosine", query_norms=norms[:1], point_norms=norms)
+    assert rows.tolist() == [[0, 4, 1, 3, 2]] and keys.tolist() == [[0.0, 0.0, 1.0, 1.0, 2.0]]
