@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -281,6 +282,365 @@ __global__ __launch_bounds__(256) void column_product(const uint32_t *__restrict
             if (j < w) out[j] = acc[c];
         }
     }
+}
+
+// ------------------------------------------- the products with an LDS-staged slice ----
+//
+// The dense operand sliced by columns: a block holds one slice of SW (8 or 16) columns
+// in LDS (Q's slice for every gene, or W's slice for one chunk's rows), so an entry
+// reads its SW dense values from LDS instead of gathering a 256-byte row from L2.
+// Each run (a row's entries, or a gene's in a chunk) belongs to LPR = SW / 4 lanes of
+// one quad, each adding the exact products of 4 columns in f64 in the run's order, so
+// the sums are the bits of the kernels above. A wavefront holds 64 / LPR runs and walks
+// them 16 entries a step: a run's lanes load its next step's entries (index, value)
+// while the current step is added, and each entry's LDS offset and value reach the
+// run's other lanes by a DPP broadcast inside the quad. A lane group whose run ends
+// takes the block's next run at once (its bounds are loaded a run ahead). The n_slices
+// blocks of a row block (or chunk) follow one another on one XCD, so the entries they
+// all stream come from HBM about once and from that XCD's L2 after.
+constexpr unsigned LDS_FLOATS = 32768;  // 128 KB a block
+
+template <int SW>
+struct SliceShape {
+    static constexpr int LPR = SW / 4;      // lanes a run
+    static constexpr int RPW = 64 / LPR;    // runs a wavefront holds
+    static constexpr int STEP = 16;         // entries of a run a step
+    static constexpr int EPL = STEP / LPR;  // entries a lane loads a step
+};
+
+// x of lane SRC of this lane's run (the run's LPR lanes of its quad). Every source lane
+// exists, so bound_ctrl (which lets the broadcast fold into the add using it) changes
+// nothing.
+template <int LPR, int SRC>
+__device__ __forceinline__ int run_bcast(int x) {
+    if constexpr (LPR == 4)
+        return __builtin_amdgcn_mov_dpp(x, SRC * 0x55, 0xf, 0xf, true);
+    else
+        return __builtin_amdgcn_mov_dpp(x, SRC | SRC << 2 | (2 + SRC) << 4 | (2 + SRC) << 6, 0xf, 0xf, true);
+}
+
+// A lane's EPL entries of a step as loaded: the step's entry LPR * k + (the lane's
+// place in the run) is the lane's k-th.
+template <int SW>
+struct SliceStep {
+    uint32_t g[SliceShape<SW>::EPL];  // dense row in LDS
+    float v[SliceShape<SW>::EPL];     // value
+};
+
+// The step at entries p, p + 1, ... of a run, read through buffer resources over the
+// block's entries (an entry past them reads index 0 and value 0; one past the run's
+// end but inside them belongs to a run of this block, so its index is a dense row in
+// LDS).
+template <int SW>
+__device__ __forceinline__ void slice_step_load(__amdgpu_buffer_rsrc_t ri, __amdgpu_buffer_rsrc_t rv, uint32_t p,
+                                                unsigned place, SliceStep<SW> &st) {
+    using S = SliceShape<SW>;
+#pragma unroll
+    for (int k = 0; k < S::EPL; k++) {
+        const int at = (int)(4u * (p + S::LPR * k + place));
+        st.g[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(ri, at, 0, 0);
+        st.v[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rv, at, 0, 0));
+    }
+}
+
+// Adds the step's entries below e, 8 LDS reads in flight at a time.
+template <int SW, int H, int... J>
+__device__ __forceinline__ void slice_half(const char *qs, uint32_t quad_bytes, const uint32_t (&a)[SliceShape<SW>::EPL],
+                                           const float (&v)[SliceShape<SW>::EPL], double (&acc)[4],
+                                           std::integer_sequence<int, J...>) {
+    constexpr int LPR = SliceShape<SW>::LPR;
+    const float4 q[] = {*reinterpret_cast<const float4 *>(
+        qs + (uint32_t)run_bcast<LPR, (H + J) % LPR>((int)a[(H + J) / LPR]) + quad_bytes)...};
+    const float x[] = {__builtin_bit_cast(float, run_bcast<LPR, (H + J) % LPR>(__builtin_bit_cast(int, v[(H + J) / LPR])))...};
+#pragma unroll
+    for (int u = 0; u < (int)sizeof...(J); u++) {
+        const double dv = (double)x[u];
+        acc[0] = fma(dv, (double)q[u].x, acc[0]);
+        acc[1] = fma(dv, (double)q[u].y, acc[1]);
+        acc[2] = fma(dv, (double)q[u].z, acc[2]);
+        acc[3] = fma(dv, (double)q[u].w, acc[3]);
+    }
+}
+
+template <int SW>
+__device__ __forceinline__ void slice_add(const char *qs, uint32_t quad_bytes, const SliceStep<SW> &st, uint32_t p,
+                                          uint32_t e, unsigned place, double (&acc)[4]) {
+    using S = SliceShape<SW>;
+    uint32_t a[S::EPL];
+    float v[S::EPL];
+#pragma unroll
+    for (int k = 0; k < S::EPL; k++) {
+        a[k] = st.g[k] * (uint32_t)(SW * 4);
+        v[k] = p + S::LPR * k + place < e ? st.v[k] : 0.0f;
+    }
+    slice_half<SW, 0>(qs, quad_bytes, a, v, acc, std::make_integer_sequence<int, S::STEP / 2>{});
+    slice_half<SW, S::STEP / 2>(qs, quad_bytes, a, v, acc, std::make_integer_sequence<int, S::STEP / 2>{});
+}
+
+// The lane group's next run: the group's first lane takes it from the block's counter
+// in LDS and the group's other lanes receive it by the broadcast.
+template <int LPR>
+__device__ __forceinline__ uint32_t grab_run(uint32_t *counter, unsigned place) {
+    uint32_t t = 0;
+    if (place == 0) t = atomicAdd(counter, 1u);
+    return (uint32_t)run_bcast<LPR, 0>((int)t);
+}
+
+// The block's runs 0 .. n_runs-1 over its lane groups: group `grp` (of `groups`) starts
+// with run grp and then takes runs from `counter` (set to `groups` before) until none is
+// left, so the groups end together whatever the runs' lengths. bounds(run, s, e) gives a
+// run's entries [s, e) relative to the resources and whether this walk sums it (a run it
+// leaves to another engine has e = s); done(run, acc) takes its 4 sums (started from
+// init). Each step issues its loads first: the bounds of the run after the next (when
+// this step ends a run), then the next step's entries, so the step's own work waits only
+// for loads issued a step or more before.
+template <int SW, typename Bounds, typename Done>
+__device__ __forceinline__ void slice_walk(const float *qs, __amdgpu_buffer_rsrc_t ri, __amdgpu_buffer_rsrc_t rv,
+                                           uint32_t n_runs, uint32_t *counter, uint32_t grp, unsigned place,
+                                           const double (&init)[4], Bounds bounds, Done done) {
+    using S = SliceShape<SW>;
+    const char *qb = reinterpret_cast<const char *>(qs);
+    const uint32_t quad_bytes = 16u * place;
+    uint32_t run = grp, p = 0, e = 0, ns = 0, ne = 0;
+    bool own = false, nown = false;
+    if (run < n_runs) own = bounds(run, p, e);
+    uint32_t nrun = grab_run<S::LPR>(counter, place);
+    if (nrun < n_runs) nown = bounds(nrun, ns, ne);
+    double acc[4] = {init[0], init[1], init[2], init[3]};
+    // One step: returns false once no group of the wavefront has a run. Two step
+    // buffers in turn, so no loaded value is copied (a copy would wait for its load).
+    const auto step = [&](SliceStep<SW> &cur, SliceStep<SW> &next) {
+        if (!__builtin_amdgcn_ballot_w64(run < n_runs)) return false;
+        const bool ends = p + S::STEP >= e;
+        uint32_t np = p + S::STEP, nend = e, after = nrun;
+        bool aown = nown;
+        if (ends) {
+            np = ns;
+            nend = ne;
+            after = grab_run<S::LPR>(counter, place);
+            aown = false;
+            ns = ne = 0;
+            if (after < n_runs) aown = bounds(after, ns, ne);
+        }
+        slice_step_load<SW>(ri, rv, np, place, next);
+        slice_add<SW>(qb, quad_bytes, cur, p, e, place, acc);
+        if (ends) {
+            if (run < n_runs && own) done(run, acc);
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[c] = init[c];
+            run = nrun;
+            own = nown;
+            nrun = after;
+            nown = aown;
+        }
+        p = np;
+        e = nend;
+        return true;
+    };
+    SliceStep<SW> a, b;
+    slice_step_load<SW>(ri, rv, p, place, a);
+    while (step(a, b) && step(b, a)) {
+    }
+}
+
+// One long run [s, e) by a whole wavefront: lane (group eg, place) adds entries s + eg,
+// s + eg + EG, ... (EG = 64 / LPR groups, U of them a step, the next step's loaded while
+// this one is added; only the last step is masked), then the groups' sums are added in
+// a fixed butterfly, so every lane ends with its place's 4 sums.
+template <int SW>
+__device__ __forceinline__ void wave_run(const char *qb, __amdgpu_buffer_rsrc_t ri, __amdgpu_buffer_rsrc_t rv,
+                                         uint32_t s, uint32_t e, unsigned eg, unsigned place, double (&acc)[4]) {
+    constexpr int LPR = SW / 4, EG = 64 / LPR, U = 4, STEP = EG * U;
+    const uint32_t quad_bytes = 16u * place;
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[c] = 0.0;
+    uint32_t ag[U], bg[U];
+    float av[U], bv[U];
+    const auto load = [&](uint32_t p, uint32_t (&g)[U], float (&v)[U]) {
+        const int at = (int)(4u * (p + eg));
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            g[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(ri, at + 4 * EG * u, 0, 0);
+            v[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rv, at + 4 * EG * u, 0, 0));
+        }
+    };
+    const auto add = [&](uint32_t p, const uint32_t (&g)[U], float (&v)[U]) {
+        if (p + STEP > e) {
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (p + eg + EG * u >= e) v[u] = 0.0f;
+        }
+        float4 q[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            q[u] = *reinterpret_cast<const float4 *>(qb + g[u] * (uint32_t)(SW * 4) + quad_bytes);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const double dv = (double)v[u];
+            acc[0] = fma(dv, (double)q[u].x, acc[0]);
+            acc[1] = fma(dv, (double)q[u].y, acc[1]);
+            acc[2] = fma(dv, (double)q[u].z, acc[2]);
+            acc[3] = fma(dv, (double)q[u].w, acc[3]);
+        }
+    };
+    // Two step buffers in turn: a step's loads are issued before the step before it is
+    // added, and no loaded value is copied (a copy would wait for its load).
+    load(s, ag, av);
+    for (uint32_t p = s;;) {
+        load(p + STEP, bg, bv);
+        add(p, ag, av);
+        p += STEP;
+        if (p >= e) break;
+        load(p + STEP, ag, av);
+        add(p, bg, bv);
+        p += STEP;
+        if (p >= e) break;
+    }
+#pragma unroll
+    for (int o = LPR; o < 64; o <<= 1)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[c] += __shfl_xor(acc[c], o);
+}
+
+// Block (row block or chunk `unit`, slice): the n_slices blocks of a unit are
+// consecutive on XCD blockIdx % 8.
+__device__ __forceinline__ void slice_block(unsigned n_slices, unsigned &unit, unsigned &slice) {
+    const unsigned xcd = blockIdx.x % 8, local = blockIdx.x / 8;
+    slice = local % n_slices;
+    unit = (local / n_slices) * 8 + xcd;
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t entry_rsrc(const void *p, uint32_t count) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)(4u * count), 0x00020000);
+}
+
+// row_project with M's column slice in LDS (m_rows * SW <= LDS_FLOATS, w <= ldm, M's
+// columns up to n_slices * SW readable, rows 16-byte aligned).
+template <int SW>
+__global__ __launch_bounds__(1024) void row_project_lds(const uint64_t *__restrict__ indptr,
+                                                        const uint32_t *__restrict__ indices,
+                                                        const float *__restrict__ values, uint64_t n_rows,
+                                                        const float *__restrict__ M, uint32_t m_rows, uint32_t w,
+                                                        uint32_t ldm, const double *__restrict__ offsets,
+                                                        float *__restrict__ out, uint32_t ldo, uint32_t n_slices,
+                                                        uint32_t rows_per_block) {
+    using S = SliceShape<SW>;
+    __shared__ __attribute__((aligned(16))) float qs[LDS_FLOATS];
+    __shared__ uint32_t counter;
+    unsigned rb, slice;
+    slice_block(n_slices, rb, slice);
+    const uint64_t r0 = (uint64_t)rb * rows_per_block;
+    if (r0 >= n_rows) return;
+    const uint32_t groups = (blockDim.x / 64) * S::RPW;
+    if (threadIdx.x == 0) counter = groups;
+    const uint32_t nr = (uint32_t)min<uint64_t>(rows_per_block, n_rows - r0);
+    const unsigned c0 = slice * SW;
+    for (unsigned i = threadIdx.x; i < m_rows * (SW / 4); i += blockDim.x)
+        reinterpret_cast<float4 *>(qs)[i] =
+            *reinterpret_cast<const float4 *>(M + (size_t)(i / (SW / 4)) * ldm + c0 + 4 * (i % (SW / 4)));
+    __syncthreads();
+    const uint64_t base = indptr[r0];
+    const uint32_t count = (uint32_t)(indptr[r0 + nr] - base);
+    const __amdgpu_buffer_rsrc_t ri = entry_rsrc(indices + base, count), rv = entry_rsrc(values + base, count);
+    const unsigned lane = lane_id(), place = lane % S::LPR;
+    const unsigned col = c0 + 4 * place;
+    double init[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) init[c] = col + c < w ? -offsets[col + c] : 0.0;
+    slice_walk<SW>(
+        qs, ri, rv, nr, &counter, wave_in_block() * S::RPW + lane / S::LPR, place, init,
+        [&](uint32_t run, uint32_t &s, uint32_t &e) {
+            s = (uint32_t)(indptr[r0 + run] - base);
+            e = (uint32_t)(indptr[r0 + run + 1] - base);
+            return true;
+        },
+        [&](uint32_t run, const double (&acc)[4]) {
+            float *o = out + (r0 + run) * ldo + col;
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (col + c < w) o[c] = (float)acc[c];
+        });
+}
+
+// A gene's run in a chunk with at least this many entries is summed by a whole
+// wavefront (wave_run), a shorter one by a lane group (slice_walk).
+constexpr uint32_t LONG_RUN = 256;
+
+// column_product with W's column slice of the chunk's rows in LDS (chunk_rows * SW <=
+// LDS_FLOATS, W's columns up to n_slices * SW readable, rows 16-byte aligned). The
+// wavefronts first take the chunk's genes 64 at a time and sum each long run among them
+// whole, then sum the short runs as lane groups. partial as column_product's.
+template <int SW>
+__global__ __launch_bounds__(1024) void column_product_lds(const uint32_t *__restrict__ col_ptr,
+                                                           const uint32_t *__restrict__ col_row,
+                                                           const float *__restrict__ col_val, uint32_t n_cols,
+                                                           uint32_t n_chunks, uint32_t chunk_rows, uint64_t n_rows,
+                                                           const float *__restrict__ W, uint32_t w, uint32_t ldw,
+                                                           uint32_t n_slices, double *__restrict__ partial) {
+    using S = SliceShape<SW>;
+    __shared__ __attribute__((aligned(16))) float ws[LDS_FLOATS];
+    __shared__ uint32_t counter[2];  // next gene batch of the long runs, next short run
+    unsigned chunk, slice;
+    slice_block(n_slices, chunk, slice);
+    if (chunk >= n_chunks) return;
+    const uint32_t groups = (blockDim.x / 64) * S::RPW;
+    if (threadIdx.x == 0) {
+        counter[0] = 0;
+        counter[1] = groups;
+    }
+    const uint64_t r0 = (uint64_t)chunk * chunk_rows;
+    const uint32_t rows = (uint32_t)min<uint64_t>(chunk_rows, n_rows - r0);
+    const unsigned c0 = slice * SW;
+    for (unsigned i = threadIdx.x; i < rows * (SW / 4); i += blockDim.x)
+        reinterpret_cast<float4 *>(ws)[i] =
+            *reinterpret_cast<const float4 *>(W + (r0 + i / (SW / 4)) * ldw + c0 + 4 * (i % (SW / 4)));
+    __syncthreads();
+    const uint32_t *cp = col_ptr + (size_t)chunk * (n_cols + 1);
+    const uint32_t base = cp[0], count = cp[n_cols] - base;
+    const __amdgpu_buffer_rsrc_t ri = entry_rsrc(col_row + base, count), rv = entry_rsrc(col_val + base, count);
+    const unsigned lane = lane_id(), place = lane % S::LPR;
+    const unsigned col = c0 + 4 * place;
+    const auto store = [&](uint32_t g, const double (&acc)[4]) {
+        double *o = partial + ((size_t)chunk * n_cols + g) * w;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (col + c < w) o[col + c] = acc[c];
+    };
+    // The long runs.
+    for (;;) {
+        uint32_t b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&counter[0], 64u);
+        b0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)b0);
+        if (b0 >= n_cols) break;
+        const uint32_t g = b0 + lane;
+        uint32_t s = 0, e = 0;
+        if (g < n_cols) {
+            s = cp[g] - base;
+            e = cp[g + 1] - base;
+        }
+        uint64_t longs = __builtin_amdgcn_ballot_w64(e - s >= LONG_RUN);
+        while (longs) {
+            const int i = __builtin_ctzll(longs);
+            longs &= longs - 1;
+            const uint32_t rs = (uint32_t)__builtin_amdgcn_readlane((int)s, i);
+            const uint32_t re = (uint32_t)__builtin_amdgcn_readlane((int)e, i);
+            double acc[4];
+            wave_run<SW>(reinterpret_cast<const char *>(ws), ri, rv, rs, re, lane / S::LPR, place, acc);
+            if (lane < S::LPR) store(b0 + i, acc);
+        }
+    }
+    // The short runs.
+    const double init[4] = {0.0, 0.0, 0.0, 0.0};
+    slice_walk<SW>(
+        ws, ri, rv, n_cols, &counter[1], wave_in_block() * S::RPW + lane / S::LPR, place, init,
+        [&](uint32_t g, uint32_t &s, uint32_t &e) {
+            s = cp[g] - base;
+            e = cp[g + 1] - base;
+            if (e - s < LONG_RUN) return true;
+            e = s;
+            return false;
+        },
+        store);
 }
 
 // out[i] = sum over chunks c (in order) of partial[c * stride + i].
@@ -623,8 +983,8 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
     const size_t stride = (size_t)csr.n_cols * w;
     ctx.partials.reserve(stride * csr.n_chunks * 2);
     ctx.reduced.reserve(stride + w);
-    // 8 genes a block: the blocks resident on an XCD work on one or two chunks at a
-    // time, whose W rows (1 MB a chunk) stay in its L2.
+    // Without the LDS slice (more than 64 columns): 8 genes a block, and the blocks
+    // resident on an XCD work on one or two chunks at a time, whose W rows stay in its L2.
     const uint32_t gpb = 8;
     const unsigned groups = div_up(csr.n_cols, gpb);
     const unsigned slots = div_up(csr.n_chunks, 8);
@@ -633,7 +993,19 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
         t_clock->begin(1, st,
                        8.0 * csr.nnz + 4.0 * csr.n_chunks * (csr.n_cols + 1.0) + 4.0 * csr.n_rows * w +
                            8.0 * csr.n_chunks * csr.n_cols * w);
-    if (w <= 64)
+    if (w <= 64 && ldw >= 64 && ldw % 4 == 0 && (size_t)csr.chunk_rows * 16 <= LDS_FLOATS) {
+        // W's chunk rows in LDS, 16 columns a slice.
+        const uint32_t n_slices = div_up(w, 16);
+        column_product_lds<16><<<8 * div_up(csr.n_chunks, 8) * n_slices, 1024, 0, st>>>(
+            csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols, csr.n_chunks, csr.chunk_rows, csr.n_rows,
+            ctx.wbuf.ptr, w, ldw, n_slices, ctx.partials.ptr);
+    } else if (w <= 64 && ldw >= 64 && ldw % 4 == 0 && (size_t)csr.chunk_rows * 8 <= LDS_FLOATS) {
+        // W's chunk rows in LDS, 8 columns a slice.
+        const uint32_t n_slices = div_up(w, 8);
+        column_product_lds<8><<<8 * div_up(csr.n_chunks, 8) * n_slices, 1024, 0, st>>>(
+            csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols, csr.n_chunks, csr.chunk_rows, csr.n_rows,
+            ctx.wbuf.ptr, w, ldw, n_slices, ctx.partials.ptr);
+    } else if (w <= 64)
         column_product<1><<<blocks, 256, 0, st>>>(csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols,
                                                   csr.n_chunks, csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, gpb,
                                                   ctx.partials.ptr);
@@ -679,7 +1051,20 @@ static void launch_row_project(Csr &csr, const float *M, uint32_t w, uint32_t ld
     if (t_clock)
         t_clock->begin(0, ctx.stream,
                        8.0 * csr.nnz + 8.0 * (csr.n_rows + 1.0) + 4.0 * csr.n_cols * w + 4.0 * csr.n_rows * w);
-    if (w <= 64)
+    // M's column slice in LDS: 16 columns a slice up to 2,048 genes, 8 up to 4,096.
+    const unsigned lds_rows_per_block = 8192;
+    const unsigned lds_blocks = 8 * div_up(div_up(csr.n_rows, lds_rows_per_block), 8);
+    if (w <= 64 && ldm >= 64 && ldm % 4 == 0 && (size_t)csr.n_cols * 16 <= LDS_FLOATS) {
+        const uint32_t n_slices = div_up(w, 16);
+        row_project_lds<16><<<lds_blocks * n_slices, 1024, 0, ctx.stream>>>(
+            csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows, M, csr.n_cols, w, ldm, offsets, out, ldo,
+            n_slices, lds_rows_per_block);
+    } else if (w <= 64 && ldm >= 64 && ldm % 4 == 0 && (size_t)csr.n_cols * 8 <= LDS_FLOATS) {
+        const uint32_t n_slices = div_up(w, 8);
+        row_project_lds<8><<<lds_blocks * n_slices, 1024, 0, ctx.stream>>>(
+            csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows, M, csr.n_cols, w, ldm, offsets, out, ldo,
+            n_slices, lds_rows_per_block);
+    } else if (w <= 64)
         row_project<1><<<blocks, 256, 0, ctx.stream>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows,
                                                        M, csr.n_cols, w, ldm, offsets, out, ldo);
     else if (w <= 128)

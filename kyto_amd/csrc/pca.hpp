@@ -6,7 +6,7 @@
 namespace ky {
 
 // Rows of one chunk of the column-major copy (the transposed product's unit).
-constexpr uint32_t CHUNK_ROWS = 4096;
+constexpr uint32_t CHUNK_ROWS = 2048;
 // estimate.rs:153: the sketch's extra columns.
 constexpr uint32_t PCA_OVERSAMPLES = 10;
 
