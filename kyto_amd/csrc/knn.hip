@@ -454,6 +454,7 @@ __global__ __launch_bounds__(64) void merge_segments(const uint32_t *__restrict_
 // group reads the candidate's row in contiguous pieces. floor_mode 0: f32 keys of
 // the centred expanded form, floor = key - slack[q] (neighbors_exact.rs:76-89);
 // 1: BF16 keys (knn_bf16.hip). m <= 64.
+template <bool COS>
 __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uint64_t n, uint32_t d,
                                               const uint32_t *__restrict__ cand_rows,
                                               const float *__restrict__ cand_keys,
@@ -500,7 +501,7 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 // Past d the difference is 0 and adds +0.0 (no change to the sum).
-                if (norms) {
+                if constexpr (COS) {
                     // Cosine: kyto's dot product lanes (past d the product is +0.0).
                     const float p = j0 + t + 8 * u < d ? pv[r][u] : 0.0f;
                     part[r] += (double)qv[u] * (double)p;
@@ -522,7 +523,7 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
         const double h = part[r] + __shfl_xor(part[r], 4, 8);
         const double hq = h + __shfl_xor(h, 2, 8);
         double key = hq + __shfl_xor(hq, 1, 8);
-        if (norms) key = cosine_key(key, norms[q], norms[lrow[r]]);
+        if constexpr (COS) key = cosine_key(key, norms[q], norms[lrow[r]]);
         if (t == (unsigned)r) {
             mine = c < cnt ? (unsigned long long)__double_as_longlong(key) : ~0ull;
             mine_row = rowv[r];
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
             // x / |x|, a zero row 0), so the cosine floor is half the Euclidean one, less
             // the f64 rounding of kyto's 1 - dot / (|q| |p|); a zero row's key, 1, lies
             // above half its distance to any row (0.5 or 0), so the floor still holds.
-            if (norms) floor = 0.5 * floor - 1e-15;
+            if constexpr (COS) floor = 0.5 * floor - 1e-15;
             cert = isfinite(far) && floor > kth;
         }
     }
@@ -938,7 +939,8 @@ void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t
             if (timed) KY_HIP(hipStreamSynchronize(st));
             const double t1 = now_ms();
             search_ms += t1 - t0;
-            rerank<<<div_up(count, 4), 256, 0, st>>>(
+            const auto rr = prep.norms ? rerank<true> : rerank<false>;
+            rr<<<div_up(count, 4), 256, 0, st>>>(
                 pts, n, d, r.rows, r.keys, r.count, s.precision == KY_PRECISION_BF16 ? r.thr : nullptr, round_rows,
                 round_rows ? 0 : lo, count, s.m, k, prep.slack, b.err, b.err_max, b.acc_rel,
                 s.precision == KY_PRECISION_FP32 ? 0 : 1, prep.norms, keys, rows, unc, unc_count);

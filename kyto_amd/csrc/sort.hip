@@ -2,7 +2,7 @@
 //
 // - sort_pairs: a stable LSD radix sort of (u32 key, u32 value) pairs by the low
 //   end_bit bits of the key, 8 bits a pass. Each pass: a histogram of the digit per
-//   tile of 4,096 keys (LDS counters), one exclusive scan of the digit-major counts
+//   tile of 4,096 keys (LDS counters), an exclusive scan of the digit-major counts
 //   (digit d's tiles in order, then digit d + 1's), and a scatter in which every tile
 //   walks its keys in order, 256 at a time: a key's place is its digit's offset for
 //   the tile, plus the keys of that digit earlier in the tile (earlier rounds, earlier
@@ -26,7 +26,6 @@ constexpr unsigned RS_THREADS = 256;
 constexpr unsigned RS_ITEMS = 16;
 constexpr unsigned RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr unsigned RS_BINS = 256;
-constexpr unsigned SCAN_THREADS = 1024;
 
 static unsigned rs_tiles(size_t n) { return (unsigned)((n + RS_TILE - 1) / RS_TILE); }
 
@@ -46,29 +45,95 @@ __global__ __launch_bounds__(RS_THREADS) void radix_hist(const uint32_t *__restr
     hist[(size_t)t * tiles + blockIdx.x] = cnt[t];
 }
 
-// In-place exclusive scan of m u32 values by one block: each thread sums a contiguous
-// run, the runs' sums are scanned in LDS, then each thread rewrites its run.
-__global__ __launch_bounds__(SCAN_THREADS) void scan_one_block(uint32_t *__restrict__ v, size_t m) {
-    __shared__ uint32_t part[SCAN_THREADS];
-    const unsigned t = threadIdx.x;
-    const size_t per = (m + SCAN_THREADS - 1) / SCAN_THREADS;
-    const size_t lo = std::min(m, (size_t)t * per), hi = std::min(m, lo + per);
-    uint32_t s = 0;
-    for (size_t i = lo; i < hi; i++) s += v[i];
-    part[t] = s;
+// ------------------------------------------------- the digit counts' scan ----
+//
+// In-place exclusive scan of m u32 values (m a multiple of 256): per tile of HS_TILE
+// values its sum, then one block scans the tiles' sums, then every tile scans itself
+// from its sum's offset. Each thread holds 16 consecutive values (four 16-byte loads).
+constexpr unsigned HS_THREADS = 256, HS_ITEMS = 16, HS_TILE = HS_THREADS * HS_ITEMS;
+
+// Exclusive scan of one value a thread over the block; *total gets the block's sum.
+// Called by every thread of the block.
+__device__ uint32_t block_scan(uint32_t x, uint32_t *total) {
+    __shared__ uint32_t wsum[HS_THREADS / 64];
+    const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t s = x;
+#pragma unroll
+    for (unsigned o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(s, o);
+        if (lane >= o) s += y;
+    }
+    if (lane == 63) wsum[wave] = s;
     __syncthreads();
-    for (unsigned off = 1; off < SCAN_THREADS; off <<= 1) {
-        const uint32_t add = t >= off ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += add;
-        __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (unsigned w = 0; w < HS_THREADS / 64; w++) {
+        if (w < wave) before += wsum[w];
+        all += wsum[w];
     }
-    uint32_t run = t ? part[t - 1] : 0u;
-    for (size_t i = lo; i < hi; i++) {
-        const uint32_t x = v[i];
-        v[i] = run;
-        run += x;
+    __syncthreads();
+    *total = all;
+    return before + s - x;
+}
+
+// The thread's 16 values of tile blockIdx.x (zeros past m; m is a multiple of 16).
+__device__ __forceinline__ void tile_items(const uint32_t *v, size_t m, uint32_t (&x)[HS_ITEMS]) {
+    const size_t i = (size_t)blockIdx.x * HS_TILE + (size_t)threadIdx.x * HS_ITEMS;
+#pragma unroll
+    for (unsigned q = 0; q < HS_ITEMS / 4; q++) {
+        const uint4 u = i < m ? reinterpret_cast<const uint4 *>(v + i)[q] : make_uint4(0, 0, 0, 0);
+        x[4 * q] = u.x;
+        x[4 * q + 1] = u.y;
+        x[4 * q + 2] = u.z;
+        x[4 * q + 3] = u.w;
     }
+}
+
+__global__ __launch_bounds__(HS_THREADS) void tile_sums_u32(const uint32_t *__restrict__ v, size_t m,
+                                                           uint32_t *__restrict__ sums) {
+    uint32_t x[HS_ITEMS];
+    tile_items(v, m, x);
+    uint32_t s = 0;
+#pragma unroll
+    for (unsigned k = 0; k < HS_ITEMS; k++) s += x[k];
+    uint32_t total;
+    block_scan(s, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// One block: the nb tile sums into exclusive offsets, in place.
+__global__ __launch_bounds__(HS_THREADS) void scan_sums_u32(uint32_t *__restrict__ sums, size_t nb) {
+    uint32_t carry = 0;
+    for (size_t r = 0; r < nb; r += HS_THREADS) {
+        const size_t i = r + threadIdx.x;
+        const uint32_t x = i < nb ? sums[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_scan(x, &total);
+        if (i < nb) sums[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(HS_THREADS) void tile_scan_u32(uint32_t *__restrict__ v, size_t m,
+                                                           const uint32_t *__restrict__ sums) {
+    uint32_t x[HS_ITEMS];
+    tile_items(v, m, x);
+    uint32_t s = 0;
+#pragma unroll
+    for (unsigned k = 0; k < HS_ITEMS; k++) s += x[k];
+    uint32_t total;
+    uint32_t run = sums[blockIdx.x] + block_scan(s, &total);
+    const size_t i = (size_t)blockIdx.x * HS_TILE + (size_t)threadIdx.x * HS_ITEMS;
+    if (i >= m) return;
+#pragma unroll
+    for (unsigned k = 0; k < HS_ITEMS; k++) {
+        const uint32_t t = x[k];
+        x[k] = run;
+        run += t;
+    }
+#pragma unroll
+    for (unsigned q = 0; q < HS_ITEMS / 4; q++)
+        reinterpret_cast<uint4 *>(v + i)[q] = make_uint4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
 }
 
 // The stable scatter of one tile (see the file comment). offs: the scanned counts.
@@ -120,9 +185,12 @@ __global__ __launch_bounds__(RS_THREADS) void radix_scatter(const uint32_t *__re
 }
 
 size_t sort_pairs_bytes(size_t n) {
-    // Alternate keys and values, then the digit counts (each piece 256-byte aligned).
+    // Alternate keys and values, the digit counts, then their tiles' sums (each piece
+    // 256-byte aligned).
     const size_t a = (sizeof(uint32_t) * n + 255) & ~size_t(255);
-    return 2 * a + sizeof(uint32_t) * RS_BINS * std::max<size_t>(rs_tiles(n), 1);
+    const size_t m = (size_t)RS_BINS * std::max<size_t>(rs_tiles(n), 1);
+    const size_t h = (sizeof(uint32_t) * m + 255) & ~size_t(255);
+    return 2 * a + h + sizeof(uint32_t) * ((m + HS_TILE - 1) / HS_TILE);
 }
 
 void sort_pairs(hipStream_t st, const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *vals_in,
@@ -135,6 +203,9 @@ void sort_pairs(hipStream_t st, const uint32_t *keys_in, uint32_t *keys_out, con
     uint32_t *alt_k = static_cast<uint32_t *>(tmp);
     uint32_t *alt_v = reinterpret_cast<uint32_t *>(static_cast<char *>(tmp) + a);
     uint32_t *hist = reinterpret_cast<uint32_t *>(static_cast<char *>(tmp) + 2 * a);
+    const size_t m = (size_t)RS_BINS * tiles, n_sums = (m + HS_TILE - 1) / HS_TILE;
+    uint32_t *sums = reinterpret_cast<uint32_t *>(static_cast<char *>(tmp) + 2 * a +
+                                                  ((sizeof(uint32_t) * m + 255) & ~size_t(255)));
     // Ping-pong so that the last pass writes the outputs.
     const uint32_t *sk = keys_in, *sv = vals_in;
     for (unsigned p = 0; p < passes; p++) {
@@ -144,7 +215,9 @@ void sort_pairs(hipStream_t st, const uint32_t *keys_in, uint32_t *keys_out, con
         const unsigned bits = std::min(8u, end_bit - shift);
         const uint32_t mask = bits == 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
         radix_hist<<<tiles, RS_THREADS, 0, st>>>(sk, n, shift, mask, tiles, hist);
-        scan_one_block<<<1, SCAN_THREADS, 0, st>>>(hist, (size_t)RS_BINS * tiles);
+        tile_sums_u32<<<n_sums, HS_THREADS, 0, st>>>(hist, m, sums);
+        scan_sums_u32<<<1, HS_THREADS, 0, st>>>(sums, n_sums);
+        tile_scan_u32<<<n_sums, HS_THREADS, 0, st>>>(hist, m, sums);
         radix_scatter<<<tiles, RS_THREADS, 0, st>>>(sk, sv, dk, dv, n, shift, mask, tiles, hist);
         KY_HIP(hipGetLastError());
         sk = dk;
