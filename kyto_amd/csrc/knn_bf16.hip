@@ -508,25 +508,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                             const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
                             z[g][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[g], bq[b][0], zero, 0, 0, 0);
                         }
-#pragma unroll
-                    for (int g = 0; g < MB; g++)
-#pragma unroll
-                        for (int b = 0; b < NB; b++) {
-                            const f32x4 r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[g], bq[b][1], z[g][b], 0, 0, 0);
-                            if (g == 0) acc[b].s0123 = r;
-                            else if (g == 1) acc[b].s4567 = r;
-                            else if (g == 2) acc[b].s89ab = r;
-                            else acc[b].scdef = r;
-                        }
-                    if (sb + 1 < TILE / SPB) fragments(sb + 1);
-                    __builtin_amdgcn_s_setprio(0);
-                    // One test for the super-block: the smallest of each lane's 4 MB keys of
-                    // a query (a tree of 3-input mins) against the query's threshold; the
-                    // column blocks' ballots OR'ed in scalar registers, one scalar branch.
+                    // The second halves column block by column block, each block's hit test
+                    // placed after the next block's MFMAs: the test's VALU instructions issue
+                    // while this wave's own MFMAs run (scheduling barriers keep the order).
+                    // The test: the smallest of each lane's 4 MB keys of a query (a tree of
+                    // 3-input mins) against the query's threshold; the column blocks' ballots
+                    // OR'ed in scalar registers, one scalar branch.
                     bool hit[NB];
                     uint64_t hits = 0;
-#pragma unroll
-                    for (int b = 0; b < NB; b++) {
+                    auto test_block = [&](int b) {
                         float part[(4 * MB + 2) / 3];
 #pragma unroll
                         for (int e = 0; e < 4 * MB; e += 3)
@@ -539,7 +529,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                             lo = e + 1 < (4 * MB + 2) / 3 ? fminf(fminf(lo, part[e]), part[e + 1]) : fminf(lo, part[e]);
                         hit[b] = lo <= thr[b];
                         hits |= __ballot(hit[b]);
+                    };
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+#pragma unroll
+                        for (int g = 0; g < MB; g++) {
+                            const f32x4 r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[g], bq[b][1], z[g][b], 0, 0, 0);
+                            if (g == 0) acc[b].s0123 = r;
+                            else if (g == 1) acc[b].s4567 = r;
+                            else if (g == 2) acc[b].s89ab = r;
+                            else acc[b].scdef = r;
+                        }
+                        if (b == NB - 1 && sb + 1 < TILE / SPB) fragments(sb + 1);
+                        if (b > 0) test_block(b - 1);
+                        __builtin_amdgcn_sched_barrier(0);
                     }
+                    __builtin_amdgcn_s_setprio(0);
+                    test_block(NB - 1);
                     if (!hits) continue;
                     // Appends: the 4 lanes of a query take buffer slots in lane order after
                     // its count, and stage slots after the wave's staged entries. The keys of
