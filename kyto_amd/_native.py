@@ -10,8 +10,11 @@ import os
 from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_uint8, c_uint32, c_uint64, c_void_p
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-GPU_LIB = os.path.join(HERE, "libkyto_gpu.so")
-SYNTH_LIB = os.path.join(HERE, "libkyto_synth.so")
+# KYTO_AMD_SANITIZED=1: the host code's AddressSanitizer + UBSan build (kyto_amd/build.py
+# sanitize), for CPU test runs under the sanitizer runtime (tests/sanitize.sh).
+_SAN = "_san" if os.environ.get("KYTO_AMD_SANITIZED") == "1" else ""
+GPU_LIB = os.path.join(HERE, f"libkyto_gpu{_SAN}.so")
+SYNTH_LIB = os.path.join(HERE, f"libkyto_synth{_SAN}.so")
 
 KY_OK = 0
 KY_E215 = 215

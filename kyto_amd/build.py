@@ -16,9 +16,19 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 GPU_LIB = os.path.join(HERE, "libkyto_gpu.so")
 SYNTH_LIB = os.path.join(HERE, "libkyto_synth.so")
+# The sanitizer build (python -m kyto_amd.build sanitize): the host code of every
+# source with AddressSanitizer and UndefinedBehaviorSanitizer (the device code as
+# always), for CPU test runs (tests/sanitize.sh).
+SAN_OBJ = os.path.join(HERE, "_build_san")
+SAN_GPU_LIB = os.path.join(HERE, "libkyto_gpu_san.so")
+SAN_SYNTH_LIB = os.path.join(HERE, "libkyto_synth_san.so")
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer",
+            "-Xarch_host", "-shared-libsan", "-g"]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
+CLANG = os.path.join(ROCM, "lib", "llvm", "bin", "clang")
 
 # Packed FP32 instructions are disabled (-packed-fp32-ops); no contraction, so every
 # f64 sum the exact keys make is evaluated as written (fused multiply-adds are
@@ -47,11 +57,11 @@ def _run(cmd):
     return r
 
 
-def _compile(src: str) -> str:
-    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+def _compile(src: str, obj: str = OBJ, extra=()) -> str:
+    out = os.path.join(obj, os.path.splitext(src)[0] + ".o")
     path = os.path.join(CSRC, src)
     # .cpp sources hold host code only; hipcc compiles them with the HIP headers.
-    cmd = [HIPCC, *HIP_FLAGS, *MFMA_FLAGS.get(src, []), *HOST_FLAGS.get(src, []), "-c", path, "-o", out]
+    cmd = [HIPCC, *HIP_FLAGS, *MFMA_FLAGS.get(src, []), *HOST_FLAGS.get(src, []), *extra, "-c", path, "-o", out]
     _run(cmd)
     return out
 
@@ -76,11 +86,27 @@ def build_synth() -> str:
     return SYNTH_LIB
 
 
+def build_sanitized() -> None:
+    os.makedirs(SAN_OBJ, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(GPU_SOURCES))) as pool:
+        objs = list(pool.map(lambda src: _compile(src, SAN_OBJ, SAN_HOST), GPU_SOURCES))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address",
+          "-Xarch_host", "-fsanitize=undefined", "-Xarch_host", "-shared-libsan", "-o", SAN_GPU_LIB, *objs,
+          "-lpthread", "-l:libzstd.so.1"])
+    _run([CLANG, "-O1", "-g", "-fPIC", "-march=x86-64-v3", "-std=c11", "-shared", "-Wno-unknown-pragmas",
+          "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-shared-libsan",
+          os.path.join(CSRC, "synth.c"), "-o", SAN_SYNTH_LIB, "-lm"])
+
+
 def build_all() -> None:
     build_synth()
     build_gpu()
 
 
 if __name__ == "__main__":
-    build_all()
-    print("built", GPU_LIB, SYNTH_LIB, file=sys.stderr)
+    if len(sys.argv) > 1 and sys.argv[1] == "sanitize":
+        build_sanitized()
+        print("built", SAN_GPU_LIB, SAN_SYNTH_LIB, file=sys.stderr)
+    else:
+        build_all()
+        print("built", GPU_LIB, SYNTH_LIB, file=sys.stderr)

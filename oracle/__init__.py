@@ -22,7 +22,9 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libkyto_oracle.so")
+# KYTO_AMD_SANITIZED=1 (tests/sanitize.sh): the AddressSanitizer + UBSan build.
+_SANITIZED = os.environ.get("KYTO_AMD_SANITIZED") == "1"
+_LIB_PATH = os.path.join(_HERE, "_build", "libkyto_oracle_san.so" if _SANITIZED else "libkyto_oracle.so")
 _lib = None
 
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -39,7 +41,7 @@ DROP_TOLERANCE = 1e-10
 
 def build() -> str:
     """Compile the C restatement (``make -C oracle``)."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    subprocess.run(["make", "-s", "-C", _HERE] + (["sanitize"] if _SANITIZED else []), check=True)
     return _LIB_PATH
 
 
