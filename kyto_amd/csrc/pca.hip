@@ -41,53 +41,83 @@ __device__ __forceinline__ uint64_t mix64(uint64_t v) {
 
 // ----------------------------------------------------------------- validation --
 
-// One thread per row: indices strictly increasing below n_cols, values finite and 0 or
-// within GPU_RANGE in magnitude (pca/gpu.rs:120). flags: bit 0 index, bit 1 value.
-__global__ void check_rows(const uint64_t *__restrict__ indptr, const uint32_t *__restrict__ indices,
-                           const float *__restrict__ values, uint64_t n_rows, uint32_t n_cols,
-                           unsigned *__restrict__ flags) {
-    uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (r >= n_rows) return;
-    uint64_t s = indptr[r], e = indptr[r + 1];
+// Indices strictly increasing below n_cols in every row, values finite and 0 or within
+// GPU_RANGE in magnitude (pca/gpu.rs:120). flags: bit 0 index, bit 1 value. A wavefront
+// a row at a time (rows wave, wave + waves, ...), lane j on entries s + j, s + j + 64, ...,
+// each compared with the entry before it in the row: the reads are coalesced.
+__global__ __launch_bounds__(256) void check_rows(const uint64_t *__restrict__ indptr,
+                                                  const uint32_t *__restrict__ indices,
+                                                  const float *__restrict__ values, uint64_t n_rows, uint32_t n_cols,
+                                                  unsigned *__restrict__ flags) {
+    const unsigned lane = lane_id();
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
     unsigned bad = 0;
-    uint32_t prev = 0;
-    for (uint64_t i = s; i < e; i++) {
-        uint32_t g = indices[i];
-        if (g >= n_cols || (i > s && g <= prev)) bad |= 1u;
-        prev = g;
-        float v = values[i];
-        float a = fabsf(v);
-        if (!(a == 0.0f || (a >= 1e-10f && a <= 1e10f))) bad |= 2u;
+    for (uint64_t r = blockIdx.x * (uint64_t)(blockDim.x / 64) + threadIdx.x / 64; r < n_rows; r += waves) {
+        const uint64_t s = indptr[r], e = indptr[r + 1];
+        for (uint64_t i = s + lane; i < e; i += 64) {
+            const uint32_t g = indices[i];
+            if (g >= n_cols || (i > s && g <= indices[i - 1])) bad |= 1u;
+            const float a = fabsf(values[i]);
+            if (!(a == 0.0f || (a >= 1e-10f && a <= 1e10f))) bad |= 2u;
+        }
     }
-    if (bad) atomicOr(flags, bad);
+    const unsigned any = (__ballot(bad & 1u) ? 1u : 0u) | (__ballot(bad & 2u) ? 2u : 0u);
+    if (lane == 0 && any) atomicOr(flags, any);
 }
 
 // --------------------------------------------------------- chunked columns ----
 
-// One block per chunk; `waves` waves count and place rows. LDS: waves * n_cols
-// counters then blockDim partial sums.
-__global__ void build_columns(const uint64_t *__restrict__ indptr, const uint32_t *__restrict__ indices,
-                              const float *__restrict__ values, uint64_t n_rows, uint32_t n_cols,
-                              uint32_t chunk_rows, uint32_t *__restrict__ col_ptr,
-                              uint32_t *__restrict__ col_row, float *__restrict__ col_val) {
+constexpr int BC_U = 8;  // steps of 64 entries whose loads build_columns issues together
+
+// One block per chunk; wavefront w takes a contiguous run of the chunk's rows. Each
+// wave counts its rows' genes (LDS counters of its own), the counts are scanned in
+// (gene, wave) order, and each wave then places its entries 64 at a time in entry
+// order, a row at a time within a step (genes are distinct within a row), so every
+// gene's entries land in row order. LDS: waves * n_cols counters, blockDim partial
+// sums, then each wave's row offsets (span + 1 of them, relative to the chunk's first
+// entry).
+__global__ __launch_bounds__(1024) void build_columns(const uint64_t *__restrict__ indptr,
+                                                      const uint32_t *__restrict__ indices,
+                                                      const float *__restrict__ values, uint64_t n_rows,
+                                                      uint32_t n_cols, uint32_t chunk_rows,
+                                                      uint32_t *__restrict__ col_ptr, uint32_t *__restrict__ col_row,
+                                                      float *__restrict__ col_val) {
     extern __shared__ uint32_t lds[];
     const unsigned waves = blockDim.x / 64;
-    uint32_t *cnt = lds;                         // [waves][n_cols]
+    uint32_t *cnt = lds;                            // [waves][n_cols]
     uint32_t *part = lds + (size_t)waves * n_cols;  // [blockDim]
     const unsigned c = blockIdx.x;
     const uint64_t r0 = (uint64_t)c * chunk_rows;
     const uint64_t r1 = min(r0 + chunk_rows, n_rows);
     const unsigned wave = threadIdx.x / 64, lane = lane_id();
+    const uint32_t span = (uint32_t)((r1 - r0 + waves - 1) / waves);
+    uint32_t *rp = part + blockDim.x + (size_t)wave * (span + 1);  // [waves][span + 1]
     for (size_t i = threadIdx.x; i < (size_t)waves * n_cols; i += blockDim.x) cnt[i] = 0;
+    const uint64_t wr0 = min(r0 + (uint64_t)wave * span, r1), wr1 = min(wr0 + span, r1);
+    const uint32_t nr = (uint32_t)(wr1 - wr0);
+    const uint64_t base = indptr[r0];
+    for (uint32_t i = lane; i <= nr; i += 64) rp[i] = (uint32_t)(indptr[wr0 + i] - base);
     __syncthreads();
-    const uint64_t span = (r1 - r0 + waves - 1) / waves;
-    const uint64_t wr0 = min(r0 + wave * span, r1), wr1 = min(wr0 + span, r1);
+    const uint32_t f0 = rp[0], f1 = rp[nr];
     uint32_t *mine = cnt + (size_t)wave * n_cols;
-    // Genes are distinct within a row, so no two lanes of one step touch one counter.
-    for (uint64_t r = wr0; r < wr1; r++)
-        for (uint64_t e = indptr[r] + lane; e < indptr[r + 1]; e += 64) mine[indices[e]] += 1;
+    const uint32_t *ix = indices + base;
+    const float *vx = values + base;
+    // Counts, BC_U steps of 64 entries loaded at a time (the loads do not wait on one
+    // another).
+    for (uint32_t f = f0; f < f1; f += 64 * BC_U) {
+        uint32_t g[BC_U];
+#pragma unroll
+        for (int u = 0; u < BC_U; u++) {
+            const uint32_t i = f + 64 * u + lane;
+            g[u] = i < f1 ? ix[i] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int u = 0; u < BC_U; u++)
+            if (g[u] != 0xFFFFFFFFu) atomicAdd(&mine[g[u]], 1u);
+    }
     __syncthreads();
-    // Exclusive scan over (gene, wave) in gene-major order.
+    // Exclusive scan over (gene, wave) in gene-major order: each thread sums a run of
+    // genes, the runs' sums are scanned (Hillis-Steele), then each thread writes its run.
     const unsigned per = (n_cols + blockDim.x - 1) / blockDim.x;
     const unsigned g0 = min(threadIdx.x * per, n_cols), g1 = min(g0 + per, n_cols);
     uint32_t local = 0;
@@ -95,17 +125,13 @@ __global__ void build_columns(const uint64_t *__restrict__ indptr, const uint32_
         for (unsigned w = 0; w < waves; w++) local += cnt[(size_t)w * n_cols + g];
     part[threadIdx.x] = local;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (unsigned t = 0; t < blockDim.x; t++) {
-            uint32_t x = part[t];
-            part[t] = run;
-            run += x;
-        }
+    for (unsigned off = 1; off < blockDim.x; off <<= 1) {
+        const uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
     }
-    __syncthreads();
-    const uint32_t base = (uint32_t)indptr[r0];
-    uint32_t run = base + part[threadIdx.x];
+    uint32_t run = (uint32_t)base + part[threadIdx.x] - local;
     uint32_t *cp = col_ptr + (size_t)c * (n_cols + 1);
     for (unsigned g = g0; g < g1; g++) {
         cp[g] = run;
@@ -117,13 +143,39 @@ __global__ void build_columns(const uint64_t *__restrict__ indptr, const uint32_
     }
     if (threadIdx.x == 0) cp[n_cols] = (uint32_t)indptr[r1];
     __syncthreads();
-    for (uint64_t r = wr0; r < wr1; r++) {
-        for (uint64_t e = indptr[r] + lane; e < indptr[r + 1]; e += 64) {
-            uint32_t g = indices[e];
-            uint32_t pos = mine[g];
-            mine[g] = pos + 1;
-            col_row[pos] = (uint32_t)(r - r0);
-            col_val[pos] = values[e];
+    // Placement. Lane l holds entry f + l of the step; its row (an index into rp)
+    // advances with the entry. The step's rows are taken in order, the lanes of a row
+    // together.
+    uint32_t row = 0;
+    const uint32_t row_off = (uint32_t)(wr0 - r0);
+    for (uint32_t f = f0; f < f1; f += 64 * BC_U) {
+        uint32_t g[BC_U];
+        float v[BC_U];
+#pragma unroll
+        for (int u = 0; u < BC_U; u++) {
+            const uint32_t i = f + 64 * u + lane;
+            g[u] = i < f1 ? ix[i] : 0u;
+            v[u] = i < f1 ? vx[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < BC_U; u++) {
+            const uint32_t fu = f + 64 * u;
+            if (fu >= f1) break;
+            const uint32_t i = fu + lane;
+            const bool valid = i < f1;
+            if (valid)
+                while (rp[row + 1] <= i) row++;
+            const uint32_t last = min(63u, f1 - fu - 1);
+            const uint32_t rlo = (uint32_t)__builtin_amdgcn_readfirstlane((int)row);
+            const uint32_t rhi = (uint32_t)__shfl((int)row, (int)last);
+            for (uint32_t rr = rlo; rr <= rhi; rr++) {
+                if (valid && row == rr) {
+                    const uint32_t pos = mine[g[u]];
+                    mine[g[u]] = pos + 1;
+                    col_row[pos] = row_off + rr;
+                    col_val[pos] = v[u];
+                }
+            }
         }
     }
 }
@@ -764,7 +816,8 @@ static void csr_finish(Csr &csr, const LayerMaps &maps) {
     normalize_log1p(st, csr.indptr.ptr, n, csr.values.ptr, maps.target_sum, maps.normalize, maps.log1p);
     DevBuf<unsigned> flags(1);
     KY_HIP(hipMemsetAsync(flags.ptr, 0, sizeof(unsigned), st));
-    check_rows<<<div_up(n, 256), 256, 0, st>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, n, csr.n_cols,
+    check_rows<<<(unsigned)std::min<uint64_t>(div_up(n, 4), 8u * 256u * 4u), 256, 0, st>>>(
+        csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, n, csr.n_cols,
                                               flags.ptr);
     KY_HIP(hipGetLastError());
     unsigned hflags = 0;
@@ -778,8 +831,13 @@ static void csr_finish(Csr &csr, const LayerMaps &maps) {
     // Chunked columns.
     csr.chunk_rows = CHUNK_ROWS;
     csr.n_chunks = div_up(n, csr.chunk_rows);
-    unsigned waves = std::min<unsigned>(16, std::max<unsigned>(1, lds_limit(ctx) / (4u * csr.n_cols + 256)));
-    size_t lds = sizeof(uint32_t) * ((size_t)waves * csr.n_cols + 64 * waves);
+    // Per wave: n_cols counters, 64 partial sums and two row offsets beyond its share of
+    // the chunk's rows (chunk_rows offsets in all).
+    const size_t fixed = sizeof(uint32_t) * csr.chunk_rows;
+    const size_t limit = lds_limit(ctx);
+    unsigned waves = std::min<unsigned>(
+        16, std::max<unsigned>(1, (unsigned)((limit > fixed ? limit - fixed : 0) / (4u * csr.n_cols + 264))));
+    size_t lds = sizeof(uint32_t) * ((size_t)waves * csr.n_cols + 64 * waves + csr.chunk_rows + 2 * waves);
     KY_CHECK(lds <= ctx.props.sharedMemPerBlock, KY_E215,
              "csr: too many genes for the device's column copy (" + std::to_string(csr.n_cols) + ")");
     csr.col_ptr.alloc((size_t)csr.n_chunks * (csr.n_cols + 1));

@@ -138,6 +138,33 @@ def test_pca_refusals(device):
         device.pca(one, 1, seed=0)  # fewer than 2 cells
 
 
+def test_row_checks_at_lane_and_row_boundaries(device):
+    """The upload's checks: a row may start below the previous row's last gene; a
+    repeat or a decrease is refused wherever it falls in a long row (entries 63/64 and
+    127/128 sit on either side of a 64-entry step), and so is a gene past the last."""
+    g = 400
+    long_row = np.arange(2, 2 * 151, 2, dtype=np.uint32)  # 150 increasing genes, 2 .. 300
+    rows = [long_row, np.array([0, 1, 5], np.uint32), long_row]
+
+    def upload(rs):
+        ip = np.zeros(len(rs) + 1, np.uint64)
+        ip[1:] = np.cumsum([len(r) for r in rs])
+        ix = np.concatenate(rs)
+        return device.upload_csr(ip, ix, np.ones(len(ix), np.float32), g)
+
+    upload(rows).free()
+    for at in (1, 63, 64, 65, 127, 128, 149):
+        for kind in ("repeat", "decrease"):
+            bad = long_row.copy()
+            bad[at] = bad[at - 1] if kind == "repeat" else bad[at - 1] - 1
+            with pytest.raises(N.KytoError, match="KY-E215"):
+                upload([rows[1], bad, rows[1]])
+    past = long_row.copy()
+    past[-1] = g
+    with pytest.raises(N.KytoError, match="KY-E215"):
+        upload([rows[1], past])
+
+
 def test_pca_with_empty_rows_and_constant_genes(device):
     rng = np.random.default_rng(3)
     n, g = 9000, 40
