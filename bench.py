@@ -4,7 +4,10 @@ A step is one pass of the path over the resident layer: the streamed block-Krylo
 PCA (q product passes, FP64 dense steps on the host, embedding left on the device),
 then the kNN graph of the embedding with BF16 MFMA candidates, exact re-rank and
 certificate, downloaded to the host. The layer is the synthetic 1,000,000 cells x
-2,000 genes log1p CSR (20 cell types, ~8% density), uploaded before the timed region.
+2,000 genes log1p CSR (20 cell types, ~8% density), uploaded before the timed region:
+`value` is the step with the layer resident in HBM. The "end_to_end" field times the
+same step from the CSR in pageable host memory (upload, row checks and column copy
+inside the clock), the PCIe-inclusive rate.
 
 The metric compares the learned + BF16 search with the plain FP32 GPU path at
 recall@15 >= 0.99. On this workload the learned router (an IVF probe of 8 of 4,096
