@@ -117,6 +117,8 @@ struct SizeArena {
 
 // Fill `count` 32-bit words with `value` (a kernel: usable where a memset is not).
 void fill_words(hipStream_t st, void *dst, size_t count, uint32_t value);
+// dst[i] = first + i for i < count.
+void iota_words(hipStream_t st, uint32_t *dst, size_t count, uint32_t first);
 
 struct Context {
     int device = 0;

@@ -166,11 +166,6 @@ __global__ void zero_rho_rows(const uint32_t *__restrict__ indices, const float 
     for (uint32_t j = 0; j < k; j++) w[base + j] = membership(r, indices[base + j], dist[base + j], s, 0.0f);
 }
 
-__global__ void iota_u32(uint32_t *__restrict__ out, uint64_t count) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) out[i] = (uint32_t)i;
-}
-
 // Column offsets of the entries sorted by column: tptr[c] is the first sorted position
 // of column c (tptr[n] = count). Position i writes the columns in (col[i-1], col[i]].
 __global__ void column_offsets(const uint32_t *__restrict__ cols, uint64_t count, uint64_t n,
@@ -299,7 +294,7 @@ void connectivities_run(Context &ctx, uint64_t n, uint32_t n_neighbors, const ui
     // The transpose: entries sorted by column, stable (rows increase within a column).
     DevBuf<uint32_t> ent(ne), scol(ne), sent(ne);
     DevBuf<unsigned char> sort_tmp;
-    iota_u32<<<div_up(ne, 256), 256, 0, st>>>(ent.ptr, ne);
+    iota_words(st, ent.ptr, ne, 0u);
     KY_HIP(hipGetLastError());
     unsigned bits = 1;
     while (bits < 32 && (1ull << bits) < n) bits++;

@@ -219,95 +219,228 @@ __global__ void label_mass(const uint32_t *__restrict__ top_idx, uint64_t rows, 
     mass[r] = s;
 }
 
-// The probed lists' points of each row, exact keys (kyto's 8-lane f64 key) and the
-// k smallest (key, row) pairs, the row itself excluded. One wavefront a row, lane =
-// point; a buffer per wave in LDS compacted by the wave-level sort (device.hpp).
-__global__ __launch_bounds__(256) void ivf_scan(const float *__restrict__ pts, uint32_t d, uint64_t n,
-                                                const uint32_t *__restrict__ list_ptr,
-                                                const uint32_t *__restrict__ list_rows,
-                                                const float *__restrict__ list_pts, const uint32_t *__restrict__ probes,
-                                                uint32_t P, uint32_t k, double *__restrict__ out_keys,
-                                                uint32_t *__restrict__ out_rows, uint32_t *__restrict__ out_count) {
-    constexpr unsigned CAP = 128;
-    __shared__ unsigned long long bkey[4][CAP];
-    __shared__ uint32_t brow[4][CAP];
-    __shared__ float qv[4][64];
+// ------------------------------------------------------------ list-major scan ----
+//
+// The probed lists' exact keys, list by list: the (query, probe) pairs are sorted by
+// list, and a block takes up to PAIRS_A_BLOCK pairs of one list and stages the list's
+// points in LDS once (a chunk of LIST_CHUNK points at a time), so a list's points
+// cross from L2 once per block instead of once per query. Each wavefront takes the
+// block's pairs in turn: lane = point, kyto's exact f64 key, the hits below the pair's
+// current k-th pair appended to the wave's LDS buffer and compacted (device.hpp), the
+// query itself left out; each pair's k smallest (key, row) pairs, ascending, go to
+// out[(q * P + p) * k]. merge_probes then keeps each query's k smallest of its P
+// lists' (every probed point at most once: the lists are disjoint).
+constexpr unsigned LIST_CHUNK = 256;     // points staged in LDS at a time
+constexpr unsigned PAIRS_A_BLOCK = 128;  // pairs a block (32 a wavefront)
+constexpr unsigned PAIR_CAP = 128;       // a wavefront's candidate buffer
+
+// kyto's exact key (kernels.rs:235-247, 290-299, 422-448; exact_sq_distance) of the
+// query qd (f64, zero-padded to 8 DC components) and the point p (f32, zero-padded):
+// a padding component adds (0 - 0)^2 = +0.0 to its lane, which changes no sum.
+template <int DC>
+__device__ __forceinline__ double padded_sq_distance(const double *qd, const float *p) {
+    double l[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < DC; c++)
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const double t = qd[8 * c + u] - (double)p[8 * c + u];
+            l[u] += t * t;
+        }
+    const double h0 = l[0] + l[4], h1 = l[1] + l[5], h2 = l[2] + l[6], h3 = l[3] + l[7];
+    return (h0 + h2) + (h1 + h3);
+}
+
+// units[b] = (list, first pair, pair count) of block b. Lists of more than LIST_CHUNK
+// points are staged a chunk at a time, again for every 4 pairs.
+template <int DC>
+__global__ __launch_bounds__(256) void ivf_pair_scan(const float *__restrict__ pts, uint32_t d,
+                                                     const uint32_t *__restrict__ list_ptr,
+                                                     const uint32_t *__restrict__ list_rows,
+                                                     const float *__restrict__ list_pts,
+                                                     const uint32_t *__restrict__ pair_code,
+                                                     const uint4 *__restrict__ units, uint32_t P, uint32_t k,
+                                                     double *__restrict__ out_keys, uint32_t *__restrict__ out_rows,
+                                                     uint32_t *__restrict__ out_count) {
+    constexpr unsigned DP = 8 * DC, PS = DP + 1;  // padded width; odd row stride (lanes' rows on different banks)
+    __shared__ float chunk[LIST_CHUNK * PS];
+    __shared__ double qd[4][DP];
+    __shared__ unsigned long long bkey[4][PAIR_CAP];
+    __shared__ uint32_t brow[4][PAIR_CAP];
     __shared__ uint32_t bcnt[4], trow[4];
     __shared__ unsigned long long tkey[4];
-    // The wave's 64 points of the current chunk, rows `ps` floats apart (odd: lanes
-    // reading their own rows fall on different banks).
-    extern __shared__ float chunk_lds[];
     const unsigned wave = threadIdx.x / 64, lane = lane_id();
-    const uint32_t ps = d | 1u;
-    float *cp = chunk_lds + wave * 64 * ps;
-    const uint64_t q = (uint64_t)blockIdx.x * (blockDim.x / 64) + wave;
-    if (q >= n) return;
-    if (lane < d) qv[wave][lane] = pts[q * d + lane];
-    if (lane == 0) {
-        bcnt[wave] = 0;
-        tkey[wave] = ~0ull;
-        trow[wave] = ~0u;
-    }
-    wave_sync();
-    for (uint32_t p = 0; p < P; p++) {
-        const uint32_t L = probes[q * P + p];
-        const uint32_t s = list_ptr[L], e = list_ptr[L + 1];
-        for (uint32_t e0 = s; e0 < e; e0 += 64) {
-            if (bcnt[wave] > CAP - 64)
-                compact<unsigned long long, CAP>(bkey[wave], brow[wave], &bcnt[wave], &tkey[wave], &trow[wave], k);
-            const uint32_t idx = e0 + lane;
-            const bool valid = idx < e;
-            // The chunk's points (contiguous in list order) staged through LDS with
-            // coalesced loads.
-            const uint32_t cnt = min(64u, e - e0);
-            const float *src = list_pts + (uint64_t)e0 * d;
-            const uint32_t total = cnt * d;
-            // Sixteen loads a lane in flight at a time, then their stores.
-            for (uint32_t t0 = 0; t0 < total; t0 += 64 * 16) {
-                float v[16];
+    const uint4 u = units[blockIdx.x];
+    const uint32_t L = u.x, first = u.y, cnt = u.z;
+    const uint32_t s = list_ptr[L], e = list_ptr[L + 1], np = e - s;
+    const uint32_t n_chunks = (np + LIST_CHUNK - 1) / LIST_CHUNK;
+    // The chunk's points staged by the whole block, 8 loads a thread in flight.
+    auto stage = [&](uint32_t c) {
+        const uint32_t c0 = c * LIST_CHUNK, m = min(LIST_CHUNK, np - c0);
+        const float *src = list_pts + (uint64_t)(s + c0) * d;
+        const uint32_t total = m * DP;
+        for (uint32_t t0 = threadIdx.x; t0 < total; t0 += 8 * blockDim.x) {
+            float v[8];
 #pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    const uint32_t t = t0 + u * 64 + lane;
-                    v[u] = t < total ? src[t] : 0.0f;
-                }
+            for (int w = 0; w < 8; w++) {
+                const uint32_t t = t0 + w * blockDim.x, r = t / DP, j = t % DP;
+                v[w] = t < total && j < d ? src[(uint64_t)r * d + j] : 0.0f;
+            }
 #pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    const uint32_t t = t0 + u * 64 + lane;
-                    if (t < total) cp[(t / d) * ps + t % d] = v[u];
-                }
-            }
-            wave_sync();
-            uint32_t row = 0xFFFFFFFFu;
-            unsigned long long kb = ~0ull;
-            if (valid) {
-                row = list_rows[idx];
-                kb = (unsigned long long)__double_as_longlong(exact_sq_distance(qv[wave], cp + lane * ps, d));
-            }
-            wave_sync();
-            const unsigned long long tk = tkey[wave];
-            const uint32_t tr = trow[wave];
-            const bool hit = valid && row != q && (kb < tk || (kb == tk && row < tr));
-            const uint64_t mask = ballot(hit);
-            if (mask) {
-                const unsigned base = bcnt[wave];
-                if (hit) {
-                    unsigned pos = base + mbcnt(mask);
-                    bkey[wave][pos] = kb;
-                    brow[wave][pos] = row;
-                }
-                wave_sync();
-                if (lane == 0) bcnt[wave] = base + (unsigned)__popcll(mask);
-                wave_sync();
+            for (int w = 0; w < 8; w++) {
+                const uint32_t t = t0 + w * blockDim.x;
+                if (t < total) chunk[(t / DP) * PS + t % DP] = v[w];
             }
         }
+    };
+    if (n_chunks == 1) stage(0);
+    __syncthreads();
+    const uint32_t slots = (cnt + 3) / 4;
+    for (uint32_t slot = 0; slot < slots; slot++) {
+        const uint32_t pi = slot * 4 + wave;
+        const bool live = pi < cnt;
+        const uint32_t code = live ? pair_code[first + pi] : 0u;
+        const uint64_t q = code / P;
+        if (live) {
+            if (lane < DP) qd[wave][lane] = lane < d ? (double)pts[q * d + lane] : 0.0;
+            if (lane == 0) {
+                bcnt[wave] = 0;
+                tkey[wave] = ~0ull;
+                trow[wave] = ~0u;
+            }
+        }
+        wave_sync();
+        bool started = false;  // the pair's first 64 points went straight to its k smallest
+        for (uint32_t c = 0; c < n_chunks; c++) {
+            if (n_chunks > 1) {
+                __syncthreads();
+                stage(c);
+                __syncthreads();
+            }
+            if (!live) continue;
+            const uint32_t c0 = c * LIST_CHUNK, m = min(LIST_CHUNK, np - c0);
+            for (uint32_t e0 = 0; e0 < m; e0 += 64) {
+                const uint32_t j = e0 + lane;
+                uint32_t row = 0xFFFFFFFFu;
+                unsigned long long kb = ~0ull;
+                if (j < m) {
+                    row = list_rows[s + c0 + j];
+                    kb = (unsigned long long)__double_as_longlong(padded_sq_distance<DC>(qd[wave], chunk + j * PS));
+                }
+                const bool valid = j < m && row != (uint32_t)q;
+                if (!started) {
+                    // The first round: its k smallest pairs sorted in registers.
+                    if (!valid) {
+                        kb = ~0ull;
+                        row = 0xFFFFFFFFu;
+                    }
+                    const unsigned n_valid = (unsigned)__popcll(ballot(valid));
+                    wave_sort64(kb, row);
+                    const unsigned kept = min(k, n_valid);
+                    if (lane < kept) {
+                        bkey[wave][lane] = kb;
+                        brow[wave][lane] = row;
+                    }
+                    const unsigned long long tk = __shfl(kb, (int)k - 1);
+                    const uint32_t tr = __shfl(row, (int)k - 1);
+                    if (lane == 0) {
+                        bcnt[wave] = kept;
+                        if (kept == k) {
+                            tkey[wave] = tk;
+                            trow[wave] = tr;
+                        }
+                    }
+                    wave_sync();
+                    started = true;
+                    continue;
+                }
+                if (bcnt[wave] > PAIR_CAP - 64)
+                    compact<unsigned long long, PAIR_CAP>(bkey[wave], brow[wave], &bcnt[wave], &tkey[wave],
+                                                          &trow[wave], k);
+                const unsigned long long tk = tkey[wave];
+                const uint32_t tr = trow[wave];
+                const bool hit = valid && (kb < tk || (kb == tk && row < tr));
+                const uint64_t mask = ballot(hit);
+                if (mask) {
+                    const unsigned base = bcnt[wave];
+                    if (hit) {
+                        const unsigned pos = base + mbcnt(mask);
+                        bkey[wave][pos] = kb;
+                        brow[wave][pos] = row;
+                    }
+                    wave_sync();
+                    if (lane == 0) bcnt[wave] = base + (unsigned)__popcll(mask);
+                    wave_sync();
+                }
+            }
+        }
+        if (live) {
+            // The pair's k smallest, sorted: in registers when the buffer holds at most
+            // 64, else by the LDS sort of the whole buffer.
+            unsigned have = bcnt[wave];
+            unsigned long long kb = ~0ull;
+            uint32_t row = 0xFFFFFFFFu;
+            if (have <= 64) {
+                if (lane < have) {
+                    kb = bkey[wave][lane];
+                    row = brow[wave][lane];
+                }
+                wave_sort64(kb, row);
+            } else {
+                compact<unsigned long long, PAIR_CAP>(bkey[wave], brow[wave], &bcnt[wave], &tkey[wave], &trow[wave],
+                                                      k);
+                have = bcnt[wave];
+                if (lane < have) {
+                    kb = bkey[wave][lane];
+                    row = brow[wave][lane];
+                }
+            }
+            const unsigned kept = min(k, have);
+            if (lane < k) {
+                out_keys[(uint64_t)code * k + lane] = lane < kept ? __longlong_as_double((long long)kb) : INFINITY;
+                out_rows[(uint64_t)code * k + lane] = lane < kept ? row : 0xFFFFFFFFu;
+            }
+            if (lane == 0) out_count[code] = kept;
+        }
+        wave_sync();
     }
-    compact<unsigned long long, CAP>(bkey[wave], brow[wave], &bcnt[wave], &tkey[wave], &trow[wave], k);
-    const unsigned cnt = bcnt[wave];
-    for (unsigned e = lane; e < k; e += 64) {
-        out_keys[q * k + e] = e < cnt ? __longlong_as_double((long long)bkey[wave][e]) : INFINITY;
-        out_rows[q * k + e] = e < cnt ? brow[wave][e] : 0xFFFFFFFFu;
+}
+
+// Each query's k smallest (key, row) pairs of its P lists' sorted lists (one thread a
+// query, a P-way merge).
+__global__ void merge_probes(const double *__restrict__ pk, const uint32_t *__restrict__ pr,
+                             const uint32_t *__restrict__ pc, uint64_t n, uint32_t P, uint32_t k,
+                             double *__restrict__ keys, uint32_t *__restrict__ rows, uint32_t *__restrict__ count) {
+    const uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    uint32_t head[8];
+    for (uint32_t p = 0; p < P; p++) head[p] = 0;
+    uint32_t out = 0;
+    for (; out < k; out++) {
+        int best = -1;
+        double bk = 0.0;
+        uint32_t br = 0;
+        for (uint32_t p = 0; p < P; p++) {
+            const uint64_t code = q * P + p;
+            if (head[p] >= pc[code]) continue;
+            const double key = pk[code * k + head[p]];
+            const uint32_t row = pr[code * k + head[p]];
+            if (best < 0 || key < bk || (key == bk && row < br)) {
+                best = (int)p;
+                bk = key;
+                br = row;
+            }
+        }
+        if (best < 0) break;
+        head[best]++;
+        keys[q * k + out] = bk;
+        rows[q * k + out] = br;
     }
-    if (lane == 0) out_count[q] = cnt;
+    for (uint32_t t = out; t < k; t++) {
+        keys[q * k + t] = INFINITY;
+        rows[q * k + t] = 0xFFFFFFFFu;
+    }
+    count[q] = out;
 }
 
 // ==================================================================== host ====
@@ -640,12 +773,42 @@ void ivf_search(Context &ctx, const ky_ivf &ivf, const Dense &points, uint32_t n
     timer.start();
     DevBuf<double> keys(n * k);
     DevBuf<uint32_t> rows(n * k), count(n);
-    // Four wavefronts a block when their point chunks fit 48 KB of LDS, else two.
-    const unsigned scan_waves = sizeof(float) * 4 * 64 * (d | 1u) <= 48 * 1024 + 4096 ? 4 : 2;
-    ivf_scan<<<div_up(n, scan_waves), 64 * scan_waves, sizeof(float) * scan_waves * 64 * (d | 1u), st>>>(
-        x, d, n, ivf.list_ptr.ptr, ivf.list_rows.ptr, ivf.list_pts.ptr,
-                                           probes.ptr, P, k, keys.ptr, rows.ptr, count.ptr);
-    KY_HIP(hipGetLastError());
+    {
+        // The (query, probe) pairs sorted by list, a block's worth of one list a unit.
+        const uint64_t np = n * P;
+        DevBuf<uint32_t> codes(np), s_lists(np), s_codes(np), start(ivf.n_lists), end(ivf.n_lists);
+        DevBuf<unsigned char> sort_tmp;
+        iota_words(st, codes.ptr, np, 0u);
+        sort_pairs(st, probes.ptr, s_lists.ptr, codes.ptr, s_codes.ptr, np, bits_for(ivf.n_lists), sort_tmp);
+        fill_words(st, start.ptr, ivf.n_lists, 0u);
+        fill_words(st, end.ptr, ivf.n_lists, 0u);
+        segment_bounds<<<div_up(np, 256), 256, 0, st>>>(s_lists.ptr, np, start.ptr, end.ptr);
+        KY_HIP(hipGetLastError());
+        std::vector<uint32_t> hs(ivf.n_lists), he(ivf.n_lists);
+        KY_HIP(hipMemcpyAsync(hs.data(), start.ptr, sizeof(uint32_t) * ivf.n_lists, hipMemcpyDeviceToHost, st));
+        KY_HIP(hipMemcpyAsync(he.data(), end.ptr, sizeof(uint32_t) * ivf.n_lists, hipMemcpyDeviceToHost, st));
+        KY_HIP(hipStreamSynchronize(st));
+        std::vector<uint4> hunits;
+        for (uint32_t l = 0; l < ivf.n_lists; l++)
+            for (uint32_t f = hs[l]; f < he[l]; f += PAIRS_A_BLOCK)
+                hunits.push_back(make_uint4(l, f, std::min<uint32_t>(PAIRS_A_BLOCK, he[l] - f), 0));
+        DevBuf<uint4> units(std::max<size_t>(hunits.size(), 1));
+        KY_HIP(hipMemcpyAsync(units.ptr, hunits.data(), sizeof(uint4) * hunits.size(), hipMemcpyHostToDevice, st));
+        DevBuf<double> pk(np * k);
+        DevBuf<uint32_t> pr(np * k), pc(np);
+        if (!hunits.empty()) {
+            const unsigned dc = (d + 7) / 8;
+            const auto scan = dc == 1 ? ivf_pair_scan<1> : dc == 2 ? ivf_pair_scan<2> : dc == 3 ? ivf_pair_scan<3>
+                            : dc == 4 ? ivf_pair_scan<4> : dc == 5 ? ivf_pair_scan<5> : dc == 6 ? ivf_pair_scan<6>
+                            : dc == 7 ? ivf_pair_scan<7> : ivf_pair_scan<8>;
+            scan<<<(unsigned)hunits.size(), 256, 0, st>>>(x, d, ivf.list_ptr.ptr, ivf.list_rows.ptr, ivf.list_pts.ptr,
+                                                          s_codes.ptr, units.ptr, P, k, pk.ptr, pr.ptr, pc.ptr);
+        }
+        KY_HIP(hipGetLastError());
+        merge_probes<<<div_up(n, 256), 256, 0, st>>>(pk.ptr, pr.ptr, pc.ptr, n, P, k, keys.ptr, rows.ptr, count.ptr);
+        KY_HIP(hipGetLastError());
+        KY_HIP(hipStreamSynchronize(st));
+    }
     std::vector<float> hconf(n);
     std::vector<uint32_t> hcount(n);
     KY_HIP(hipMemcpyAsync(hconf.data(), conf.ptr, sizeof(float) * n, hipMemcpyDeviceToHost, st));

@@ -817,11 +817,6 @@ size_t knn_scratch_bytes(const KnnShape &s, bool graph, bool packed) {
     return a.high;
 }
 
-__global__ void iota_words(uint32_t *__restrict__ out, uint64_t count, uint32_t first) {
-    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i < count) out[i] = first + (uint32_t)i;
-}
-
 // Exact search of the listed rows against every row; results into keys/rows (n x k).
 static void knn_exact_rows(Context &ctx, Arena &a, const KnnShape &s, const float *pts, const double *norms,
                            const uint32_t *d_list, uint64_t count, double *d_keys, uint32_t *d_rows, bool seeded) {
@@ -959,7 +954,7 @@ void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t
         if (q_rows)
             KY_HIP(hipMemcpyAsync(unc, q_rows, sizeof(uint32_t) * nq, hipMemcpyDeviceToDevice, st));
         else
-            iota_words<<<div_up(nq, 256), 256, 0, st>>>(unc, nq, 0u);
+            iota_words(st, unc, nq, 0u);
         KY_HIP(hipGetLastError());
         total = (uint32_t)nq;
         search_ms = now_ms() - t_start;

@@ -336,4 +336,15 @@ void fill_words(hipStream_t st, void *dst, size_t count, uint32_t value) {
     KY_HIP(hipGetLastError());
 }
 
+__global__ void iota_words_kernel(uint32_t *__restrict__ dst, size_t count, uint32_t first) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = first + (uint32_t)i;
+}
+
+void iota_words(hipStream_t st, uint32_t *dst, size_t count, uint32_t first) {
+    if (count == 0) return;
+    iota_words_kernel<<<div_up(count, 256), 256, 0, st>>>(dst, count, first);
+    KY_HIP(hipGetLastError());
+}
+
 }  // namespace ky
