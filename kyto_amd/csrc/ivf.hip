@@ -183,12 +183,23 @@ __global__ __launch_bounds__(256) void row_dot(const float *__restrict__ dL, con
     if (lane == 0) out[r] = acc;
 }
 
-// gamma's gradient: alpha times the rows' sums added in order (one thread).
-__global__ void gamma_grad(const float *__restrict__ row_sums, uint64_t rows, float *__restrict__ gamma) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// gamma's gradient: alpha times the rows' sum. One block of 256 threads: thread t sums
+// its contiguous run of rows in order, then the runs' sums are added by a fixed
+// binary tree (the same bits on every run).
+__global__ __launch_bounds__(256) void gamma_grad(const float *__restrict__ row_sums, uint64_t rows,
+                                                  float *__restrict__ gamma) {
+    __shared__ float part[256];
+    const unsigned t = threadIdx.x;
+    const uint64_t per = (rows + 255) / 256, r0 = min<uint64_t>(rows, t * per), r1 = min<uint64_t>(rows, r0 + per);
     float s = 0.0f;
-    for (uint64_t r = 0; r < rows; r++) s += row_sums[r];
-    gamma[1] = expf(gamma[0]) * s;
+    for (uint64_t r = r0; r < r1; r++) s += row_sums[r];
+    part[t] = s;
+    __syncthreads();
+    for (unsigned h = 128; h > 0; h >>= 1) {
+        if (t < h) part[t] += part[t + h];
+        __syncthreads();
+    }
+    if (t == 0) gamma[1] = expf(gamma[0]) * part[0];
 }
 
 // confidence = sum over the probed lists of exp(logit - lse).
@@ -654,7 +665,7 @@ ky_ivf *ivf_build(Context &ctx, const Dense &points, const ky_ivf_params &p, ky_
             R.backward(st, xb.ptr, B, h1.ptr, h2.ptr, lg.ptr, dh2.ptr, dh1.ptr);
             // gamma: dLoss/dgamma = alpha sum dL S.
             row_dot<<<div_up(B, 4), 256, 0, st>>>(lg.ptr, sc.ptr, B, p.n_lists, rsum.ptr);
-            gamma_grad<<<1, 1, 0, st>>>(rsum.ptr, B, ivf->gamma.ptr);
+            gamma_grad<<<1, 256, 0, st>>>(rsum.ptr, B, ivf->gamma.ptr);
             KY_HIP(hipGetLastError());
             R.adam_step(st, p.lr);
             adam(st, ivf->gamma.ptr, ivf->gamma.ptr + 1, ivf->gamma.ptr + 2, ivf->gamma.ptr + 3, 1, p.lr, R.step);

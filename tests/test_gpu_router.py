@@ -202,3 +202,27 @@ def test_router_gains_over_centroid_probing(device):
     print(f"recall@15 router {r_router:.4f} centroid probing {r_centroid:.4f} alpha {alpha:.4g} "
           f"label mass in top 8 {index.info['label_recall_top']:.4f}")
     assert r_router > r_centroid
+
+
+@pytest.mark.parametrize("d, k", [(20, 15), (50, 31)])
+def test_ivf_search_of_every_list_is_the_exact_graph(device, d, k):
+    """With as many probes as lists every point is scored, so the list-by-list scan and
+    the per-row merge must give the exact graph bit for bit (kyto's f64 key and (key,
+    row) order); 8,000 rows in 8 lists of ~1,000 points are staged in LDS a chunk at a
+    time."""
+    pts = clustered(5, 8000, d, 12, 0.0)
+    m = device.upload_dense(pts)
+    try:
+        exact = device.neighbors(m, k, precision="fp32")
+        index = device.ivf_build(m, n_lists=8, kmeans_iters=3, seed=0, hidden=32, train_queries=2000,
+                                 batch=512, epochs=1, lr=1e-3, label_k=k - 1)
+        try:
+            for use_router in (False, True):
+                g = index.search(m, k, n_probe=8, use_router=use_router, confidence=0.0)
+                assert g.info["fallback"] == 0
+                assert np.array_equal(g.rows(), exact.rows())
+                assert np.array_equal(g.distances.view(np.uint32), exact.distances.view(np.uint32))
+        finally:
+            index.free()
+    finally:
+        m.free()
