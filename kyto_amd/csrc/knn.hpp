@@ -61,6 +61,10 @@ struct KnnRound {
     uint32_t *seg_rows = nullptr, *seg_count = nullptr;
     float *seg_keys = nullptr, *seg_thr = nullptr;
     unsigned main_blocks = 0, n_seg = 1, seg_tiles = 0;
+    // The symmetric BF16 scan: each row's list of keys from earlier rows.
+    bool sym = false;
+    uint32_t *tcount = nullptr;
+    void *tlist = nullptr;
 };
 
 uint32_t candidates_for(uint32_t k, uint64_t n, int precision);

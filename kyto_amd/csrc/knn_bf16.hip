@@ -352,6 +352,7 @@ constexpr unsigned MERGE_CAP = MAX_TAIL_SEG * 64;    // a tail query's merged li
 // Blocks are mapped to queries XCD by XCD (block b runs on XCD b % 8): the blocks
 // resident on one XCD hold consecutive queries, start their scans at nearby tiles
 // and read the same tiles through that XCD's L2 at about the same time.
+template <bool SYM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void search_bf16(
     const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q, uint64_t np, uint32_t pstride,
     const uint32_t *__restrict__ qpos, uint64_t qoff, uint64_t nq,
@@ -359,20 +360,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     uint32_t *__restrict__ out_rows, float *__restrict__ out_keys, uint32_t *__restrict__ out_count,
     float *__restrict__ out_thr, uint32_t main_blocks, uint32_t n_seg, uint32_t seg_tiles,
     uint32_t *__restrict__ seg_rows, float *__restrict__ seg_keys, uint32_t *__restrict__ seg_count,
-    float *__restrict__ seg_thr) {
+    float *__restrict__ seg_thr, uint32_t *__restrict__ tcount, Entry *__restrict__ tlist, uint32_t tcap) {
     __shared__ __attribute__((aligned(16))) unsigned char tile[3][TILE * ROW_BYTES];
+    // Symmetric scan: the starting thresholds of the points of this tile and the next.
+    __shared__ __attribute__((aligned(16))) float tile_thr[SYM ? 2 : 1][SYM ? TILE : 4];
     // Per wave: the append stage, which a compaction's sort scratch overlays.
     __shared__ __attribute__((aligned(16))) uint32_t stage_key[4][SCAP];
     __shared__ __attribute__((aligned(16))) uint32_t stage_row[4][SCAP];
     __shared__ uint16_t stage_dst[4][SCAP];
-    __shared__ uint32_t qrow_lds[QPB];
+    __shared__ uint32_t qrow_lds[SYM ? 1 : QPB];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
     // Blocks below main_blocks scan every point for query block lblk (XCD-aware order);
     // the others are the tail: query block main_blocks + t / n_seg over point segment
-    // t % n_seg (seg_tiles tiles), with its own buffers and outputs.
-    const bool tail = blockIdx.x >= main_blocks;
+    // t % n_seg (seg_tiles tiles), with its own buffers and outputs. The symmetric scan
+    // has no tail; block b is query block b (the longest first).
+    const bool tail = !SYM && blockIdx.x >= main_blocks;
     unsigned lblk, seg = 0;
-    if (!tail) {
+    if (SYM) {
+        lblk = blockIdx.x;
+    } else if (!tail) {
         const unsigned nblk = main_blocks, xcd = blockIdx.x % 8, per = nblk / 8, extra = nblk % 8;
         lblk = xcd * per + min(xcd, extra) + blockIdx.x / 8;
     } else {
@@ -384,7 +390,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     // Buffer (and tail output) slot of query i.
     auto slot = [&](uint64_t i) { return tail ? main_q + (i - main_q) * n_seg + seg : i; };
     const uint64_t qbase = (uint64_t)lblk * QPB + (uint64_t)wave * QPW;  // this wave's first query
-    uint32_t *qrow = qrow_lds + wave * QPW;  // scan-order row of each of the wave's queries
+    // The scan-order row of each of the wave's queries (the symmetric scan's are
+    // consecutive).
+    uint32_t *qrow = qrow_lds + (SYM ? 0 : wave * QPW);
+    auto query_row = [&](unsigned j) { return SYM ? (uint32_t)(qbase + j) : qrow[j]; };
     // B fragments (the wave's queries, loaded once), and per column block b the
     // threshold and buffer count of this lane's query 16 b + l % 16.
     bf16x8 bq[NB][2];
@@ -395,7 +404,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         const uint64_t i = qbase + 16 * b + (lane % 16);
         const uint64_t ic = min<uint64_t>(i, nq - 1);
         const uint32_t r = qpos ? qpos[ic] : (uint32_t)(qoff + ic);
-        if (lane < 16) qrow[16 * b + lane] = r;
+        if (!SYM && lane < 16) qrow[16 * b + lane] = r;
         const bf16x8 *src = reinterpret_cast<const bf16x8 *>(Q + (uint64_t)r * KDIM);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) bq[b][ks] = src[4 * ks + lane / 16];
@@ -405,12 +414,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     }
     unsigned staged = 0;  // wave-uniform
     const bool wave_live = qbase < nq;
-    // The staged appends to the buffers (one store instruction).
+    // The staged appends to the buffers (one store instruction). In the symmetric scan
+    // a stage entry with bit 15 of its destination set is a key for the point's own row
+    // (row = the point, low bits = the wave's query): it takes a slot of the point's
+    // list by an atomic (slots past tcap are counted, not written).
     auto flush = [&](const uint32_t *sk, const uint32_t *sr, const uint16_t *sd) {
         wave_sync();
         if (lane < staged) {
             const unsigned dst = sd[lane];
-            buf[slot(qbase + (dst >> 9)) * BCAP + (dst & (BCAP - 1))] = make_uint2(sk[lane], sr[lane]);
+            if (SYM && (dst & 0x8000u)) {
+                const uint32_t p = sr[lane];
+                const unsigned at = atomicAdd(tcount + p, 1u);
+                if (at < tcap) tlist[(uint64_t)p * tcap + at] = make_uint2(sk[lane], query_row(dst & (QPW - 1)));
+            } else {
+                buf[slot(qbase + (dst >> 9)) * BCAP + (dst & (BCAP - 1))] = make_uint2(sk[lane], sr[lane]);
+            }
         }
         staged = 0;
         wave_sync();
@@ -435,9 +453,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     // every block starts at tile 0, so the blocks resident on an XCD read the same
     // tiles at about the same time through its L2.
     // A tail block scans its segment only.
-    const uint64_t first_tile =
-        tail ? (uint64_t)seg * seg_tiles : thr_init ? 0 : (min<uint64_t>(own, n_tiles - 1) + n_tiles - lead) % n_tiles;
-    const uint64_t n_scan = tail ? min<uint64_t>(seg_tiles, n_tiles - first_tile) : n_tiles;
+    // The symmetric scan: query block b scans from its own tiles to the last (each pair
+    // of rows once, by the block of the earlier row).
+    static_assert(QPB % TILE == 0, "a query block spans whole tiles");
+    const uint64_t first_tile = SYM    ? (uint64_t)lblk * (QPB / TILE)
+                                : tail ? (uint64_t)seg * seg_tiles
+                                : thr_init ? 0
+                                           : (min<uint64_t>(own, n_tiles - 1) + n_tiles - lead) % n_tiles;
+    const uint64_t n_scan = SYM ? n_tiles - first_tile : tail ? min<uint64_t>(seg_tiles, n_tiles - first_tile) : n_tiles;
     // Tiles go straight from global memory to LDS (global_load_lds_dwordx4) into three
     // buffers: tile t + 2 is requested once tile t's keys are done, so it has the
     // whole of tile t + 1 to arrive. A tile is 128 rows of 128 bytes with the 16-byte
@@ -457,10 +480,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                                              16, 0, 0);
         }
     };
+    // The symmetric scan's thresholds of a tile's points (thr_init holds -inf past the
+    // last point), 64 by each of waves 0 and 1: those of tile t + 1 are requested as
+    // tile t starts, so the wait for tile t + 1 at its end covers them.
+    auto fetch_thr = [&](uint64_t t0, float *__restrict__ dthr) {
+        if (SYM && wave < 2)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(thr_init + t0 + 64 * wave + lane),
+                                             (__attribute__((address_space(3))) void *)(dthr + 64 * wave), 4, 0, 0);
+    };
     // Tile indices advance by one and wrap (no division in the loop).
     auto wrap_next = [&](uint32_t t) { return t + 1 == (uint32_t)n_tiles ? 0u : t + 1; };
     uint32_t tcur = (uint32_t)first_tile, tfetch = wrap_next(tcur);
     fetch((uint64_t)tcur * TILE, tile[0]);
+    fetch_thr((uint64_t)tcur * TILE, tile_thr[0]);
     if (n_scan > 1) fetch((uint64_t)tfetch * TILE, tile[1]);
     tfetch = wrap_next(tfetch);
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
@@ -473,7 +505,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         // restrict qualifiers tell the compiler that the stage and the tile reads do
         // not touch the buffer in flight, so it does not wait for it before them.
         auto body = [&](const unsigned char *__restrict__ tb, unsigned char *__restrict__ nx,
-                        uint32_t *__restrict__ sk, uint32_t *__restrict__ sr, uint16_t *__restrict__ sd) {
+                        const float *__restrict__ tt, float *__restrict__ ntt, uint32_t *__restrict__ sk,
+                        uint32_t *__restrict__ sr, uint16_t *__restrict__ sd) {
+            // Past the block's own two tiles, the symmetric scan tests each key against the
+            // point's threshold as well.
+            const bool cross = SYM && it >= QPB / TILE;
+            if (SYM && it + 1 < n_scan) fetch_thr((uint64_t)(tcur + 1) * TILE, ntt);
             if (wave_live) {
                 // The point fragments of super-block sb (row blocks g of 16 points, the two
                 // halves of K); those of the next super-block are read from LDS while this
@@ -546,7 +583,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     }
                     __builtin_amdgcn_s_setprio(0);
                     test_block(NB - 1);
-                    if (!hits) continue;
+                    // The symmetric scan's second test: the smallest of the lane's NB keys of
+                    // each of its points against the point's threshold.
+                    // psites: bit e set when some lane's key e is at or below its point's threshold.
+                    uint32_t psites = 0;
+                    auto point_thresholds = [&](float (&pt)[4 * MB]) {
+#pragma unroll
+                        for (int g = 0; g < MB; g++) {
+                            const float4 t4 = *reinterpret_cast<const float4 *>(tt + sb * SPB + 16 * g + 4 * (lane / 16));
+                            pt[4 * g] = t4.x;
+                            pt[4 * g + 1] = t4.y;
+                            pt[4 * g + 2] = t4.z;
+                            pt[4 * g + 3] = t4.w;
+                        }
+                    };
+                    if (cross) {
+                        float pthr[4 * MB];
+                        point_thresholds(pthr);
+#pragma unroll
+                        for (int e = 0; e < 4 * MB; e++) {
+                            static_assert(NB == 4, "the point test takes the minimum of 4 column blocks");
+                            const float lo = fminf(fminf(fminf(acc[0][e], acc[1][e]), acc[2][e]), acc[3][e]);
+                            if (__ballot(lo <= pthr[e])) psites |= 1u << e;
+                        }
+                    }
+                    if (!hits && !psites) continue;
                     // Appends: the 4 lanes of a query take buffer slots in lane order after
                     // its count, and stage slots after the wave's staged entries. The keys of
                     // a column block are walked with a uniform index into a 16-register
@@ -563,7 +624,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 #pragma unroll
                         for (int e = 0; e < 4 * MB; e++)
                             if (__ballot(acc[b][e] <= thr[b])) sites |= 1u << e;
-                        const uint32_t self = qrow[16 * b + (lane % 16)];
+                        const uint32_t self = query_row(16 * b + (lane % 16));
                         const uint64_t same = 0x0001000100010001ull << (lane % 16);
                         while (sites) {
                             const int e = __builtin_ctz(sites);
@@ -583,6 +644,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                             }
                             staged += (unsigned)__popcll(mh);
                             cb[b] += (unsigned)__popcll(mq);
+                        }
+                    }
+                    // Keys at or below their point's threshold, staged for the points' lists
+                    // (walked key by key, then column block by column block).
+                    if (psites) {
+                        float pthr[4 * MB];
+                        point_thresholds(pthr);
+#pragma unroll
+                        for (int e = 0; e < 4 * MB; e++) {
+                            if (!(psites & (1u << e))) continue;
+                            const uint32_t p = (uint32_t)(pbase + 16 * (e / 4) + (e % 4));
+#pragma unroll
+                            for (int b = 0; b < NB; b++) {
+                                const float x = acc[b][e];
+                                const bool h = x <= pthr[e];
+                                const uint64_t mh = __ballot(h);
+                                if (!mh) continue;
+                                if (staged + (unsigned)__popcll(mh) > SCAP) flush(sk, sr, sd);
+                                if (h) {
+                                    const unsigned at = staged + mbcnt(mh);
+                                    sk[at] = __float_as_uint(x);
+                                    sr[at] = p;
+                                    sd[at] = (uint16_t)(0x8000u | (16 * b + (lane % 16)));
+                                }
+                                staged += (unsigned)__popcll(mh);
+                            }
                         }
                     }
                 }
@@ -617,7 +704,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
             }
         };
-        body(tile[cur], tile[after], stage_key[wave], stage_row[wave], stage_dst[wave]);
+        body(tile[cur], tile[after], tile_thr[SYM ? it % 2 : 0], tile_thr[SYM ? (it + 1) % 2 : 0], stage_key[wave],
+             stage_row[wave], stage_dst[wave]);
         // A bare barrier: each wave has waited for its own part of tile t + 1 above,
         // and the LDS the block shares in this loop is written by those loads only (a
         // __syncthreads() fence would also wait for tile t + 2). The signal fences keep
@@ -648,9 +736,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         uint32_t *const o_rows = tail ? seg_rows : out_rows;
         float *const o_keys = tail ? seg_keys : out_keys;
         if (o_rows) {
+            // The symmetric scan leaves scan-order rows (its merge maps them).
             for (unsigned e = lane; e < m; e += 64) {
                 const uint32_t row = stage_row[wave][e];
-                o_rows[o * m + e] = e < kept && row < 0xFFFFFFFFu ? perm[row] : 0xFFFFFFFFu;
+                o_rows[o * m + e] = e < kept && row < 0xFFFFFFFFu ? (SYM ? row : perm[row]) : 0xFFFFFFFFu;
                 o_keys[o * m + e] = e < kept ? unorder_f32(stage_key[wave][e]) : INFINITY;
             }
         }
@@ -715,6 +804,72 @@ __global__ __launch_bounds__(64) void merge_tail(const uint32_t *__restrict__ se
     if (lane == 0) {
         out_count[o] = kept;
         out_thr[o] = total >= m ? unorder_f32(skey[m - 1]) : T;
+    }
+}
+
+// The symmetric scan's lists joined (one wavefront a row, four a block): row r's own
+// scan kept every point of the later tiles at or below its threshold T (its m-th key,
+// or the starting one), and its list holds every earlier row at or below the starting
+// threshold, which is at least T; below T the union is whole. Its m smallest (key,
+// scan-order row) pairs at or below T replace the row's candidates, as rows (perm[.]).
+// A list that overflowed tcap leaves the row without candidates (the exact search takes
+// it).
+constexpr uint32_t SYM_TCAP = 320;     // list entries a row
+constexpr unsigned SYM_MERGE = 512;    // m + SYM_TCAP, rounded up to a power of two
+
+__global__ __launch_bounds__(256) void merge_sym(const uint32_t *__restrict__ perm, uint64_t n, uint32_t m,
+                                                 const uint32_t *__restrict__ tcount, const Entry *__restrict__ tlist,
+                                                 uint32_t *__restrict__ rows, float *__restrict__ keys,
+                                                 uint32_t *__restrict__ count, float *__restrict__ thr) {
+    __shared__ uint32_t skey[4][SYM_MERGE];
+    __shared__ uint32_t srow[4][SYM_MERGE];
+    const unsigned lane = lane_id(), wave = threadIdx.x / 64;
+    const uint64_t r = (uint64_t)blockIdx.x * 4 + wave;
+    if (r >= n) return;
+    uint32_t *sk = skey[wave], *sr = srow[wave];
+    const unsigned own = count[r], listed = tcount[r];
+    const float T = thr[r];
+    if (listed > SYM_TCAP) {
+        for (unsigned e = lane; e < m; e += 64) {
+            rows[r * m + e] = 0xFFFFFFFFu;
+            keys[r * m + e] = INFINITY;
+        }
+        if (lane == 0) count[r] = 0;
+        return;
+    }
+    for (unsigned e = lane; e < own; e += 64) {
+        sk[e] = order_f32(keys[r * m + e]);
+        sr[e] = rows[r * m + e];
+    }
+    unsigned total = own;  // wave-uniform
+    for (unsigned e0 = 0; e0 < listed; e0 += 64) {
+        const unsigned e = e0 + lane;
+        const Entry v = e < listed ? tlist[r * SYM_TCAP + e] : make_uint2(0, 0);
+        const bool keep = e < listed && __uint_as_float(v.x) <= T;
+        const uint64_t mask = __ballot(keep);
+        if (keep) {
+            const unsigned at = total + mbcnt(mask);
+            sk[at] = order_f32(__uint_as_float(v.x));
+            sr[at] = v.y;
+        }
+        total += (unsigned)__popcll(mask);
+    }
+    unsigned N = 64;
+    while (N < total) N <<= 1;
+    for (unsigned e = total + lane; e < N; e += 64) {
+        sk[e] = 0xFFFFFFFFu;
+        sr[e] = 0xFFFFFFFFu;
+    }
+    wave_sync();
+    wave_bitonic<uint32_t>(sk, sr, N);
+    const unsigned kept = min(total, m);
+    for (unsigned e = lane; e < m; e += 64) {
+        rows[r * m + e] = e < kept ? perm[sr[e]] : 0xFFFFFFFFu;
+        keys[r * m + e] = e < kept ? unorder_f32(sk[e]) : INFINITY;
+    }
+    if (lane == 0) {
+        count[r] = kept;
+        thr[r] = total >= m ? unorder_f32(sk[m - 1]) : T;
     }
 }
 
@@ -865,7 +1020,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 
 unsigned bf16_resident(const Context &ctx) {
     int per_cu = 0;
-    KY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, search_bf16, 256, 0));
+    KY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, search_bf16<false>, 256, 0));
     return (unsigned)std::max(per_cu, 1) * (unsigned)std::max(ctx.n_cu, 1);
 }
 
@@ -881,7 +1036,9 @@ void bf16_take_points(A &a, const KnnShape &s, Bf16Points &b) {
     b.P = a.template take<__bf16>((s.n + PAD_ROWS) * KDIM);
     b.Q = a.template take<__bf16>(s.n * KDIM);
     b.err = a.template take<double>(2 * s.n);
-    b.thr_est = s.n >= EST_MIN_ROWS ? a.template take<float>(s.nq) : nullptr;
+    // The thresholds, then -inf for a tile's worth past the last (the symmetric scan
+    // reads whole tiles of them).
+    b.thr_est = s.n >= EST_MIN_ROWS ? a.template take<float>(s.nq + TILE) : nullptr;
     b.small = a.template take<unsigned long long>(4);
     if (s.list) {
         b.qpos = a.template take<uint32_t>(s.nq);
@@ -937,8 +1094,24 @@ static void bf16_split(const KnnShape &s, uint64_t count, unsigned &main_blocks,
     }
 }
 
+// The symmetric scan (each pair of rows keyed once, for both rows) runs when a round
+// holds every row, with starting thresholds; it needs a list of SYM_TCAP entries a row.
+static bool bf16_symmetric(const KnnShape &s, uint64_t count) {
+    return !s.list && count == s.n && s.n >= EST_MIN_ROWS;
+}
+
 template <class A>
 void bf16_take_round(A &a, const KnnShape &s, uint64_t count, KnnRound &r) {
+    r.sym = bf16_symmetric(s, count);
+    if (r.sym) {
+        r.main_blocks = div_up(count, QPB);
+        r.n_seg = 1;
+        r.seg_tiles = 0;
+        r.buf = a.template take<Entry>(count * BCAP);
+        r.tcount = a.template take<uint32_t>(count);
+        r.tlist = a.template take<Entry>(count * SYM_TCAP);
+        return;
+    }
     bf16_split(s, count, r.main_blocks, r.n_seg, r.seg_tiles);
     const uint64_t main_q = std::min<uint64_t>(count, (uint64_t)r.main_blocks * QPB);
     const uint64_t tail_q = count - main_q;
@@ -1043,13 +1216,29 @@ void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t l
     const uint32_t *qpos = s.list ? b.qpos + lo : nullptr;
     const float *thr = b.thr_est ? b.thr_est + lo : nullptr;
     hipEvent_t ev[2] = {ctx.timer[0], ctx.timer[1]};
-    if (scan_ms) KY_HIP(hipEventRecord(ev[0], st));
-    search_bf16<<<grid, 256, 0, st>>>(static_cast<const __bf16 *>(b.P), static_cast<const __bf16 *>(b.Q), s.n, 1,
-                                      qpos, lo, count, b.perm, s.m, thr, static_cast<Entry *>(r.buf), r.rows,
-                                      r.keys, r.count, r.thr, r.main_blocks, r.n_seg, r.seg_tiles, r.seg_rows,
-                                      r.seg_keys, r.seg_count, r.seg_thr);
-    KY_HIP(hipGetLastError());
-    if (scan_ms) KY_HIP(hipEventRecord(ev[1], st));
+    if (r.sym) {
+        KY_CHECK(thr && lo == 0 && count == s.n, KY_E505, "neighbors: a symmetric scan of a partial round (internal)");
+        fill_words(st, r.tcount, count, 0u);
+        fill_words(st, reinterpret_cast<uint32_t *>(b.thr_est + count), TILE, 0xFF800000u);  // -inf
+        if (scan_ms) KY_HIP(hipEventRecord(ev[0], st));
+        search_bf16<true><<<blocks, 256, 0, st>>>(
+            static_cast<const __bf16 *>(b.P), static_cast<const __bf16 *>(b.Q), s.n, 1, nullptr, 0, count, b.perm, s.m,
+            thr, static_cast<Entry *>(r.buf), r.rows, r.keys, r.count, r.thr, blocks, 1, 0, nullptr, nullptr, nullptr,
+            nullptr, r.tcount, static_cast<Entry *>(r.tlist), SYM_TCAP);
+        KY_HIP(hipGetLastError());
+        if (scan_ms) KY_HIP(hipEventRecord(ev[1], st));
+        merge_sym<<<div_up(count, 4), 256, 0, st>>>(b.perm, count, s.m, r.tcount, static_cast<const Entry *>(r.tlist),
+                                                     r.rows, r.keys, r.count, r.thr);
+        KY_HIP(hipGetLastError());
+    } else {
+        if (scan_ms) KY_HIP(hipEventRecord(ev[0], st));
+        search_bf16<false><<<grid, 256, 0, st>>>(
+            static_cast<const __bf16 *>(b.P), static_cast<const __bf16 *>(b.Q), s.n, 1, qpos, lo, count, b.perm, s.m,
+            thr, static_cast<Entry *>(r.buf), r.rows, r.keys, r.count, r.thr, r.main_blocks, r.n_seg, r.seg_tiles,
+            r.seg_rows, r.seg_keys, r.seg_count, r.seg_thr, nullptr, nullptr, 0);
+        KY_HIP(hipGetLastError());
+        if (scan_ms) KY_HIP(hipEventRecord(ev[1], st));
+    }
     if (tail_q) {
         merge_tail<<<(unsigned)tail_q, 64, 0, st>>>(r.seg_rows, r.seg_keys, r.seg_count, r.seg_thr, tail_q, r.n_seg,
                                                     s.m, main_q, r.rows, r.keys, r.count, r.thr);

@@ -164,19 +164,55 @@ def test_large_input_threshold_estimate(device):
 
 
 def test_scan_tail_split_over_point_segments(device):
-    # 200,000 rows: 782 query blocks against 768 resident ones, so the last 14 blocks'
-    # queries are scanned as 16 point segments each and their lists merged. The graph
-    # must still be the FP32 path's on every row, and the oracle's on sampled rows.
-    pts = _large_clustered(200_000, 50, 16, seed=23)
-    g16 = _exact(device, pts, 15, "bf16")
-    g32 = _exact(device, pts, 15, "fp32")
-    assert np.array_equal(g16.rows(), g32.rows())
-    assert np.array_equal(g16.distances.view(np.uint32), g32.distances.view(np.uint32))
+    # Rounds of 782 query blocks (and 781) against 768 resident ones, so the last few
+    # blocks' queries of each round are scanned as point segments and their lists
+    # merged. The graph must be the one-round (symmetric) scan's on every row, and the
+    # oracle's on sampled rows.
+    pts = _large_clustered(400_000, 50, 16, seed=23)
+    m = device.upload_dense(pts)
+    try:
+        whole = device.neighbors(m, 15, precision="bf16")
+        g16 = device.neighbors(m, 15, precision="bf16", round_rows=782 * 256)
+    finally:
+        m.free()
+    assert g16.info["rounds"] == 2
+    assert np.array_equal(g16.rows(), whole.rows())
+    assert np.array_equal(g16.distances.view(np.uint32), whole.distances.view(np.uint32))
     assert g16.info["fallback"] < 1000
-    first, count = 199_000, 400
-    keys, rows, _ = oracle.knn_exact_range(pts, 14, first, count)
-    o = np.argsort(rows, axis=1)
-    assert np.array_equal(g16.rows()[first:first + count], np.take_along_axis(rows, o, 1))
+    for first in (199_000, 399_600):
+        keys, rows, _ = oracle.knn_exact_range(pts, 14, first, 400)
+        o = np.argsort(rows, axis=1)
+        assert np.array_equal(g16.rows()[first:first + 400], np.take_along_axis(rows, o, 1))
+
+
+def test_symmetric_scan_equals_the_exact_graph(device):
+    # One round over every row (above 2^17 rows) keys each pair of rows once, for both
+    # rows: each row's candidates join its own scan of the later rows with the keys the
+    # earlier rows' blocks found at or below its threshold. 150,100 rows end in a partial
+    # query block whose one tile is its own; duplicated rows tie keys across the join.
+    # The graph is the FP32 path's on every row, the rounds' (the one-sided scan) on
+    # every row, and the oracle's on sampled rows.
+    pts = _large_clustered(150_100, 50, 12, seed=37)
+    pts[1::1001] = pts[::1001][:len(pts[1::1001])]
+    pts[149_990:] = pts[70_000:70_110]
+    m = device.upload_dense(pts)
+    try:
+        whole = device.neighbors(m, 15, precision="bf16")
+        rounds = device.neighbors(m, 15, precision="bf16", round_rows=76_800)
+        g32 = device.neighbors(m, 15, precision="fp32")
+    finally:
+        m.free()
+    assert whole.info["rounds"] == 1 and rounds.info["rounds"] == 2
+    assert whole.info["fallback"] < 200
+    for g in (rounds, g32):
+        assert np.array_equal(g.rows(), whole.rows())
+        assert np.array_equal(g.distances.view(np.uint32), whole.distances.view(np.uint32))
+    rows = np.unique(np.concatenate([np.arange(0, 150_100, 1499), np.arange(149_980, 150_100)])).astype(np.uint32)
+    keys, want, _ = oracle.knn_exact_rows(pts, 14, rows)
+    o = np.argsort(want, axis=1, kind="stable")
+    assert np.array_equal(whole.rows()[rows], np.take_along_axis(want, o, 1))
+    want_dist = np.sqrt(np.take_along_axis(keys, o, 1)).astype(np.float32)
+    assert np.array_equal(whole.distances.reshape(-1, 14)[rows].view(np.uint32), want_dist.view(np.uint32))
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
