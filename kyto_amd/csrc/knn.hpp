@@ -65,6 +65,7 @@ struct KnnRound {
     bool sym = false;
     uint32_t *tcount = nullptr;
     void *tlist = nullptr;
+    float *thr_max = nullptr;  // the largest starting threshold of each super-block of points
 };
 
 uint32_t candidates_for(uint32_t k, uint64_t n, int precision);

@@ -319,6 +319,14 @@ __device__ unsigned compact_buffer(Entry *__restrict__ buf, unsigned cnt, unsign
     return kept;
 }
 
+// The lane's index as an opaque value the compiler cannot hoist out of a loop or keep
+// live across it (ALU only).
+__device__ __forceinline__ uint32_t fresh_lane() {
+    uint32_t l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
+}
+
 constexpr int LEAD_TILES = 16;        // tiles scanned before a block's own position
 constexpr uint64_t EST_MIN_ROWS = 1 << 17;  // threshold estimates from a sample above this size
 // The sample: every 12th row of the scan order. With 32 subsets of n / 384 sampled
@@ -331,8 +339,7 @@ constexpr unsigned SCAP = 64;  // staged appends per wavefront
 constexpr unsigned MAX_TAIL_SEG = 16;                // point segments of a tail query block
 constexpr unsigned MERGE_CAP = MAX_TAIL_SEG * 64;    // a tail query's merged lists (m <= 64)
 
-// The candidate scan. Points: np rows, point p being row p * pstride of the scan
-// order (P'). Queries: i < nq, query i being row qpos[i] of the scan order (row
+// The candidate scan. Points: the np rows of the scan order (P'). Queries: i < nq, query i being row qpos[i] of the scan order (row
 // qoff + i when qpos is null; ascending in i), outputs at slot o = i (a round of
 // queries has slots of its own).
 //
@@ -354,16 +361,18 @@ constexpr unsigned MERGE_CAP = MAX_TAIL_SEG * 64;    // a tail query's merged li
 // and read the same tiles through that XCD's L2 at about the same time.
 template <bool SYM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void search_bf16(
-    const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q, uint64_t np, uint32_t pstride,
+    const __bf16 *__restrict__ P, const __bf16 *__restrict__ Q, uint64_t np,
     const uint32_t *__restrict__ qpos, uint64_t qoff, uint64_t nq,
     const uint32_t *__restrict__ perm, uint32_t m, const float *__restrict__ thr_init, Entry *__restrict__ buf,
     uint32_t *__restrict__ out_rows, float *__restrict__ out_keys, uint32_t *__restrict__ out_count,
     float *__restrict__ out_thr, uint32_t main_blocks, uint32_t n_seg, uint32_t seg_tiles,
     uint32_t *__restrict__ seg_rows, float *__restrict__ seg_keys, uint32_t *__restrict__ seg_count,
-    float *__restrict__ seg_thr, uint32_t *__restrict__ tcount, Entry *__restrict__ tlist, uint32_t tcap) {
+    float *__restrict__ seg_thr, uint32_t *__restrict__ tcount, Entry *__restrict__ tlist, uint32_t tcap,
+    const float *__restrict__ thr_max) {
     __shared__ __attribute__((aligned(16))) unsigned char tile[3][TILE * ROW_BYTES];
-    // Symmetric scan: the starting thresholds of the points of this tile and the next.
-    __shared__ __attribute__((aligned(16))) float tile_thr[SYM ? 2 : 1][SYM ? TILE : 4];
+    // Symmetric scan: the starting thresholds of the points of each tile buffer's tile,
+    // then the largest of each of its super-blocks'.
+    __shared__ __attribute__((aligned(16))) float tile_thr[SYM ? 3 : 1][SYM ? TILE + TILE / SPB : 4];
     // Per wave: the append stage, which a compaction's sort scratch overlays.
     __shared__ __attribute__((aligned(16))) uint32_t stage_key[4][SCAP];
     __shared__ __attribute__((aligned(16))) uint32_t stage_row[4][SCAP];
@@ -446,7 +455,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     };
     const uint64_t n_tiles = (np + TILE - 1) / TILE;
     const uint64_t q0 = min<uint64_t>((uint64_t)lblk * QPB, nq - 1);
-    const uint64_t own = ((qpos ? (uint64_t)qpos[q0] : qoff + q0) / pstride) / TILE;
+    const uint64_t own = (qpos ? (uint64_t)qpos[q0] : qoff + q0) / TILE;
     const uint64_t lead = min<uint64_t>(LEAD_TILES, n_tiles - 1);
     // Without a starting threshold a block starts a few tiles before its own queries'
     // position (their near neighbours first tighten the thresholds early); with one,
@@ -468,32 +477,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     // fragment read touches fall on different banks. Each wave instruction fills 8 rows
     // (lane l: row l / 8, stored chunk l % 8). Rows past the last point read the padding
     // rows after P' (keys 1e30, never kept).
+    // Each lane's source offset in the tile is one 32-bit value (row % 8 = lane / 8 for
+    // every instruction, whose rows are 8 apart), instruction s s KiB further. The
+    // offset is computed from a fresh lane index at each fetch: an address kept live
+    // across the scan loop is spilled, and a spill's reload waits for every load in
+    // flight, the next tile's included.
     auto fetch = [&](uint64_t t0, unsigned char *__restrict__ dst) {
-        const __bf16 *tp = P + t0 * pstride * KDIM;
+        const uint32_t fl = fresh_lane();
+        const uint32_t fetch_off = ((wave * (TILE / 32) * 8 + fl / 8) * KDIM + (((fl % 8) ^ (fl / 8)) * 8)) * 2;
+        const unsigned char *tp = reinterpret_cast<const unsigned char *>(P + t0 * KDIM) + fetch_off;
 #pragma unroll
-        for (int s = 0; s < TILE / 32; s++) {
-            const unsigned row = (wave * (TILE / 32) + s) * 8 + lane / 8;
-            const unsigned chunk = (lane % 8) ^ (row % 8);
-            const __bf16 *src = tp + (row * pstride * KDIM + chunk * 8);
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+        for (int s = 0; s < TILE / 32; s++)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(tp + s * 8 * ROW_BYTES),
                                              (__attribute__((address_space(3))) void *)(dst + (wave * (TILE / 32) + s) * 1024),
                                              16, 0, 0);
-        }
     };
     // The symmetric scan's thresholds of a tile's points (thr_init holds -inf past the
-    // last point), 64 by each of waves 0 and 1: those of tile t + 1 are requested as
-    // tile t starts, so the wait for tile t + 1 at its end covers them.
+    // last point), 64 by each of waves 0 and 1, and the super-blocks' largest (thr_max)
+    // by 4 lanes of wave 2, requested with the tile (one more load in flight for those
+    // waves).
     auto fetch_thr = [&](uint64_t t0, float *__restrict__ dthr) {
-        if (SYM && wave < 2)
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(thr_init + t0 + 64 * wave + lane),
-                                             (__attribute__((address_space(3))) void *)(dthr + 64 * wave), 4, 0, 0);
+        if (!SYM) return;
+        const uint32_t fl = fresh_lane();
+        if (wave < 2)
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)(reinterpret_cast<const unsigned char *>(thr_init + t0 + 64 * wave) + 4 * fl),
+                (__attribute__((address_space(3))) void *)(dthr + 64 * wave), 4, 0, 0);
+        else if (wave == 2 && fl < TILE / SPB)
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)(reinterpret_cast<const unsigned char *>(thr_max + t0 / SPB) + 4 * fl),
+                (__attribute__((address_space(3))) void *)(dthr + TILE), 4, 0, 0);
     };
     // Tile indices advance by one and wrap (no division in the loop).
     auto wrap_next = [&](uint32_t t) { return t + 1 == (uint32_t)n_tiles ? 0u : t + 1; };
     uint32_t tcur = (uint32_t)first_tile, tfetch = wrap_next(tcur);
     fetch((uint64_t)tcur * TILE, tile[0]);
     fetch_thr((uint64_t)tcur * TILE, tile_thr[0]);
-    if (n_scan > 1) fetch((uint64_t)tfetch * TILE, tile[1]);
+    if (n_scan > 1) {
+        fetch((uint64_t)tfetch * TILE, tile[1]);
+        fetch_thr((uint64_t)tfetch * TILE, tile_thr[SYM ? 1 : 0]);
+    }
     tfetch = wrap_next(tfetch);
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
@@ -510,13 +533,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             // Past the block's own two tiles, the symmetric scan tests each key against the
             // point's threshold as well.
             const bool cross = SYM && it >= QPB / TILE;
-            if (SYM && it + 1 < n_scan) fetch_thr((uint64_t)(tcur + 1) * TILE, ntt);
             if (wave_live) {
                 // The point fragments of super-block sb (row blocks g of 16 points, the two
                 // halves of K); those of the next super-block are read from LDS while this
                 // one's MFMAs and hit test run.
                 bf16x8 a0[MB], a1[MB];
+                // Symmetric scan: the largest threshold of the super-block's points, read
+                // with its fragments.
+                float pmax_next = 0.0f;
                 auto fragments = [&](int sb) {
+                    if (SYM) pmax_next = tt[TILE + sb];
 #pragma unroll
                     for (int g = 0; g < MB; g++) {
                         const unsigned prow = sb * SPB + g * 16 + (lane % 16);
@@ -534,6 +560,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     // The first halves of K of every (g, b), then the second halves: each
                     // accumulator's two MFMAs are MB * NB apart.
                     f32x4 z[MB][NB];
+                    const float pmax = pmax_next;
                     // Raised wave priority while this wave issues its MFMAs (and reads the
                     // next fragments), back to normal for the hit test: the SIMD's MFMA
                     // pipe is fed first (0.9 ms a launch, A/B on one box).
@@ -552,6 +579,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     // 3-input mins) against the query's threshold; the column blocks' ballots
                     // OR'ed in scalar registers, one scalar branch.
                     bool hit[NB];
+                    float lo_b[NB];
                     uint64_t hits = 0;
                     auto test_block = [&](int b) {
                         float part[(4 * MB + 2) / 3];
@@ -564,6 +592,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 #pragma unroll
                         for (int e = 1; e < (4 * MB + 2) / 3; e += 2)
                             lo = e + 1 < (4 * MB + 2) / 3 ? fminf(fminf(lo, part[e]), part[e + 1]) : fminf(lo, part[e]);
+                        lo_b[b] = lo;
                         hit[b] = lo <= thr[b];
                         hits |= __ballot(hit[b]);
                     };
@@ -584,7 +613,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     __builtin_amdgcn_s_setprio(0);
                     test_block(NB - 1);
                     // The symmetric scan's second test: the smallest of the lane's NB keys of
-                    // each of its points against the point's threshold.
+                    // each of its points against the point's threshold, made only when the
+                    // smallest of all the lane's keys (from the minima above) is at or below
+                    // the largest threshold of the super-block's points in some lane; most
+                    // super-blocks end at that one comparison.
                     // psites: bit e set when some lane's key e is at or below its point's threshold.
                     uint32_t psites = 0;
                     auto point_thresholds = [&](float (&pt)[4 * MB]) {
@@ -597,12 +629,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                             pt[4 * g + 3] = t4.w;
                         }
                     };
-                    if (cross) {
+                    static_assert(NB == 4, "the point tests take the minimum of 4 column blocks");
+                    if (cross && __ballot(fminf(fminf(lo_b[0], lo_b[1]), fminf(lo_b[2], lo_b[3])) <= pmax)) {
                         float pthr[4 * MB];
                         point_thresholds(pthr);
 #pragma unroll
                         for (int e = 0; e < 4 * MB; e++) {
-                            static_assert(NB == 4, "the point test takes the minimum of 4 column blocks");
                             const float lo = fminf(fminf(fminf(acc[0][e], acc[1][e]), acc[2][e]), acc[3][e]);
                             if (__ballot(lo <= pthr[e])) psites |= 1u << e;
                         }
@@ -631,7 +663,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                             sites &= sites - 1;
                             const float x = acc[b][e];
                             const uint64_t p = pbase + 16 * (e / 4) + (e % 4);
-                            const bool h = x <= thr[b] && p < np && p * pstride != self;
+                            const bool h = x <= thr[b] && p < np && p != self;
                             const uint64_t mh = __ballot(h);
                             if (!mh) continue;
                             if (staged + (unsigned)__popcll(mh) > SCAP) flush(sk, sr, sd);
@@ -639,7 +671,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                             if (h) {
                                 const unsigned slot = staged + mbcnt(mh);
                                 sk[slot] = __float_as_uint(x);
-                                sr[slot] = (uint32_t)(p * pstride);
+                                sr[slot] = (uint32_t)p;
                                 sd[slot] = (uint16_t)(((16 * b + (lane % 16)) << 9) | (cb[b] + mbcnt(mq)));
                             }
                             staged += (unsigned)__popcll(mh);
@@ -698,13 +730,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 }
             }
             if (it + 2 < n_scan) {
+                // The thresholds first: any wait their addresses need then waits for tile
+                // t + 1's loads only.
+                fetch_thr((uint64_t)tfetch * TILE, ntt);
                 fetch((uint64_t)tfetch * TILE, nx);
-                __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4): all but tile t + 2 (tile t + 1 is in LDS)
+                // All but tile t + 2's loads (tile t + 1 is in LDS).
+                if (SYM && wave < 3)
+                    __builtin_amdgcn_s_waitcnt(0x0F75);  // vmcnt(5)
+                else
+                    __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4)
             } else {
                 __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
             }
         };
-        body(tile[cur], tile[after], tile_thr[SYM ? it % 2 : 0], tile_thr[SYM ? (it + 1) % 2 : 0], stage_key[wave],
+        body(tile[cur], tile[after], tile_thr[SYM ? cur : 0], tile_thr[SYM ? after : 0], stage_key[wave],
              stage_row[wave], stage_dst[wave]);
         // A bare barrier: each wave has waited for its own part of tile t + 1 above,
         // and the LDS the block shares in this loop is written by those loads only (a
@@ -805,6 +844,18 @@ __global__ __launch_bounds__(64) void merge_tail(const uint32_t *__restrict__ se
         out_count[o] = kept;
         out_thr[o] = total >= m ? unorder_f32(skey[m - 1]) : T;
     }
+}
+
+// The largest starting threshold of each super-block of SPB points (thr holds -inf
+// past the last point, through the last tile).
+__global__ __launch_bounds__(256) void superblock_max(const float *__restrict__ thr, uint64_t n_sb,
+                                                      float *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_sb) return;
+    float v = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < SPB; j++) v = fmaxf(v, thr[i * SPB + j]);
+    out[i] = v;
 }
 
 // The symmetric scan's lists joined (one wavefront a row, four a block): row r's own
@@ -1110,6 +1161,7 @@ void bf16_take_round(A &a, const KnnShape &s, uint64_t count, KnnRound &r) {
         r.buf = a.template take<Entry>(count * BCAP);
         r.tcount = a.template take<uint32_t>(count);
         r.tlist = a.template take<Entry>(count * SYM_TCAP);
+        r.thr_max = a.template take<float>(div_up(count, TILE) * (TILE / SPB));
         return;
     }
     bf16_split(s, count, r.main_blocks, r.n_seg, r.seg_tiles);
@@ -1220,11 +1272,13 @@ void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t l
         KY_CHECK(thr && lo == 0 && count == s.n, KY_E505, "neighbors: a symmetric scan of a partial round (internal)");
         fill_words(st, r.tcount, count, 0u);
         fill_words(st, reinterpret_cast<uint32_t *>(b.thr_est + count), TILE, 0xFF800000u);  // -inf
+        const uint64_t n_sb = div_up(count, TILE) * (TILE / SPB);
+        superblock_max<<<div_up(n_sb, 256), 256, 0, st>>>(thr, n_sb, r.thr_max);
         if (scan_ms) KY_HIP(hipEventRecord(ev[0], st));
         search_bf16<true><<<blocks, 256, 0, st>>>(
-            static_cast<const __bf16 *>(b.P), static_cast<const __bf16 *>(b.Q), s.n, 1, nullptr, 0, count, b.perm, s.m,
+            static_cast<const __bf16 *>(b.P), static_cast<const __bf16 *>(b.Q), s.n, nullptr, 0, count, b.perm, s.m,
             thr, static_cast<Entry *>(r.buf), r.rows, r.keys, r.count, r.thr, blocks, 1, 0, nullptr, nullptr, nullptr,
-            nullptr, r.tcount, static_cast<Entry *>(r.tlist), SYM_TCAP);
+            nullptr, r.tcount, static_cast<Entry *>(r.tlist), SYM_TCAP, r.thr_max);
         KY_HIP(hipGetLastError());
         if (scan_ms) KY_HIP(hipEventRecord(ev[1], st));
         merge_sym<<<div_up(count, 4), 256, 0, st>>>(b.perm, count, s.m, r.tcount, static_cast<const Entry *>(r.tlist),
@@ -1233,9 +1287,9 @@ void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t l
     } else {
         if (scan_ms) KY_HIP(hipEventRecord(ev[0], st));
         search_bf16<false><<<grid, 256, 0, st>>>(
-            static_cast<const __bf16 *>(b.P), static_cast<const __bf16 *>(b.Q), s.n, 1, qpos, lo, count, b.perm, s.m,
+            static_cast<const __bf16 *>(b.P), static_cast<const __bf16 *>(b.Q), s.n, qpos, lo, count, b.perm, s.m,
             thr, static_cast<Entry *>(r.buf), r.rows, r.keys, r.count, r.thr, r.main_blocks, r.n_seg, r.seg_tiles,
-            r.seg_rows, r.seg_keys, r.seg_count, r.seg_thr, nullptr, nullptr, 0);
+            r.seg_rows, r.seg_keys, r.seg_count, r.seg_thr, nullptr, nullptr, 0, nullptr);
         KY_HIP(hipGetLastError());
         if (scan_ms) KY_HIP(hipEventRecord(ev[1], st));
     }
