@@ -60,6 +60,7 @@ constexpr int MB = 2;           // row blocks (16 points each) per super-block
 constexpr int SPB = 16 * MB;    // points per super-block
 constexpr int QPW = 16 * NB;    // queries per wavefront
 constexpr int QPB = 4 * QPW;    // queries per block
+constexpr int GMAX_PER_TILE = 4 * (TILE / SPB);  // symmetric scan: lane groups x super-blocks a tile
 
 // ---------------------------------------------------------------- preparation ----
 
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     __shared__ __attribute__((aligned(16))) unsigned char tile[3][TILE * ROW_BYTES];
     // Symmetric scan: the starting thresholds of the points of each tile buffer's tile,
     // then the largest of each of its super-blocks'.
-    __shared__ __attribute__((aligned(16))) float tile_thr[SYM ? 3 : 1][SYM ? TILE + TILE / SPB : 4];
+    __shared__ __attribute__((aligned(16))) float tile_thr[SYM ? 3 : 1][SYM ? TILE + GMAX_PER_TILE : 4];
     // Per wave: the append stage, which a compaction's sort scratch overlays.
     __shared__ __attribute__((aligned(16))) uint32_t stage_key[4][SCAP];
     __shared__ __attribute__((aligned(16))) uint32_t stage_row[4][SCAP];
@@ -493,9 +494,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                                              16, 0, 0);
     };
     // The symmetric scan's thresholds of a tile's points (thr_init holds -inf past the
-    // last point), 64 by each of waves 0 and 1, and the super-blocks' largest (thr_max)
-    // by 4 lanes of wave 2, requested with the tile (one more load in flight for those
-    // waves).
+    // last point), 64 by each of waves 0 and 1, and the largest of each lane group's
+    // points in each super-block (thr_max) by 16 lanes of wave 2, requested with the
+    // tile (one more load in flight for those waves).
     auto fetch_thr = [&](uint64_t t0, float *__restrict__ dthr) {
         if (!SYM) return;
         const uint32_t fl = fresh_lane();
@@ -503,9 +504,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             __builtin_amdgcn_global_load_lds(
                 (const __attribute__((address_space(1))) void *)(reinterpret_cast<const unsigned char *>(thr_init + t0 + 64 * wave) + 4 * fl),
                 (__attribute__((address_space(3))) void *)(dthr + 64 * wave), 4, 0, 0);
-        else if (wave == 2 && fl < TILE / SPB)
+        else if (wave == 2 && fl < GMAX_PER_TILE)
             __builtin_amdgcn_global_load_lds(
-                (const __attribute__((address_space(1))) void *)(reinterpret_cast<const unsigned char *>(thr_max + t0 / SPB) + 4 * fl),
+                (const __attribute__((address_space(1))) void *)(reinterpret_cast<const unsigned char *>(thr_max + t0 / TILE * GMAX_PER_TILE) + 4 * fl),
                 (__attribute__((address_space(3))) void *)(dthr + TILE), 4, 0, 0);
     };
     // Tile indices advance by one and wrap (no division in the loop).
@@ -538,11 +539,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 // halves of K); those of the next super-block are read from LDS while this
                 // one's MFMAs and hit test run.
                 bf16x8 a0[MB], a1[MB];
-                // Symmetric scan: the largest threshold of the super-block's points, read
-                // with its fragments.
+                // Symmetric scan: the largest threshold of the super-block's points whose
+                // keys this lane holds, read with its fragments.
                 float pmax_next = 0.0f;
                 auto fragments = [&](int sb) {
-                    if (SYM) pmax_next = tt[TILE + sb];
+                    if (SYM) pmax_next = tt[TILE + sb * 4 + lane / 16];
 #pragma unroll
                     for (int g = 0; g < MB; g++) {
                         const unsigned prow = sb * SPB + g * 16 + (lane % 16);
@@ -615,8 +616,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     // The symmetric scan's second test: the smallest of the lane's NB keys of
                     // each of its points against the point's threshold, made only when the
                     // smallest of all the lane's keys (from the minima above) is at or below
-                    // the largest threshold of the super-block's points in some lane; most
-                    // super-blocks end at that one comparison.
+                    // the largest threshold of its points in some lane; most super-blocks
+                    // end at that one comparison.
                     // psites: bit e set when some lane's key e is at or below its point's threshold.
                     uint32_t psites = 0;
                     auto point_thresholds = [&](float (&pt)[4 * MB]) {
@@ -846,15 +847,20 @@ __global__ __launch_bounds__(64) void merge_tail(const uint32_t *__restrict__ se
     }
 }
 
-// The largest starting threshold of each super-block of SPB points (thr holds -inf
-// past the last point, through the last tile).
-__global__ __launch_bounds__(256) void superblock_max(const float *__restrict__ thr, uint64_t n_sb,
-                                                      float *__restrict__ out) {
+// The largest starting threshold of each lane group's points in each super-block:
+// entry 4 s + j holds the largest over the points of super-block s whose keys lanes
+// 16 j .. 16 j + 15 hold (point 16 g + 4 j + i, g < MB, i < 4; thr holds -inf past
+// the last point, through the last tile).
+__global__ __launch_bounds__(256) void group_max(const float *__restrict__ thr, uint64_t n_g,
+                                                 float *__restrict__ out) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n_sb) return;
+    if (i >= n_g) return;
+    const uint64_t p0 = (i / 4) * SPB + 4 * (i % 4);
     float v = -INFINITY;
 #pragma unroll
-    for (int j = 0; j < SPB; j++) v = fmaxf(v, thr[i * SPB + j]);
+    for (int g = 0; g < MB; g++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) v = fmaxf(v, thr[p0 + 16 * g + j]);
     out[i] = v;
 }
 
@@ -1161,7 +1167,7 @@ void bf16_take_round(A &a, const KnnShape &s, uint64_t count, KnnRound &r) {
         r.buf = a.template take<Entry>(count * BCAP);
         r.tcount = a.template take<uint32_t>(count);
         r.tlist = a.template take<Entry>(count * SYM_TCAP);
-        r.thr_max = a.template take<float>(div_up(count, TILE) * (TILE / SPB));
+        r.thr_max = a.template take<float>(div_up(count, TILE) * GMAX_PER_TILE);
         return;
     }
     bf16_split(s, count, r.main_blocks, r.n_seg, r.seg_tiles);
@@ -1272,8 +1278,8 @@ void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t l
         KY_CHECK(thr && lo == 0 && count == s.n, KY_E505, "neighbors: a symmetric scan of a partial round (internal)");
         fill_words(st, r.tcount, count, 0u);
         fill_words(st, reinterpret_cast<uint32_t *>(b.thr_est + count), TILE, 0xFF800000u);  // -inf
-        const uint64_t n_sb = div_up(count, TILE) * (TILE / SPB);
-        superblock_max<<<div_up(n_sb, 256), 256, 0, st>>>(thr, n_sb, r.thr_max);
+        const uint64_t n_g = div_up(count, TILE) * GMAX_PER_TILE;
+        group_max<<<div_up(n_g, 256), 256, 0, st>>>(thr, n_g, r.thr_max);
         if (scan_ms) KY_HIP(hipEventRecord(ev[0], st));
         search_bf16<true><<<blocks, 256, 0, st>>>(
             static_cast<const __bf16 *>(b.P), static_cast<const __bf16 *>(b.Q), s.n, nullptr, 0, count, b.perm, s.m,
