@@ -634,10 +634,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     if (cross && __ballot(fminf(fminf(lo_b[0], lo_b[1]), fminf(lo_b[2], lo_b[3])) <= pmax)) {
                         float pthr[4 * MB];
                         point_thresholds(pthr);
+                        auto point_min = [&](int e) {
+                            return fminf(fminf(fminf(acc[0][e], acc[1][e]), acc[2][e]), acc[3][e]);
+                        };
+                        // First whether any point has a key at or below its threshold: the
+                        // smallest of (key minimum - threshold) over the lane's points
+                        // against 0 (x - t <= 0 iff x <= t for finite keys; a flushed
+                        // difference only lets a point through), then, rarely, which.
+                        float dmin = point_min(0) - pthr[0];
 #pragma unroll
-                        for (int e = 0; e < 4 * MB; e++) {
-                            const float lo = fminf(fminf(fminf(acc[0][e], acc[1][e]), acc[2][e]), acc[3][e]);
-                            if (__ballot(lo <= pthr[e])) psites |= 1u << e;
+                        for (int e = 1; e < 4 * MB; e++) dmin = fminf(dmin, point_min(e) - pthr[e]);
+                        if (__ballot(dmin <= 0.0f)) {
+#pragma unroll
+                            for (int e = 0; e < 4 * MB; e++)
+                                if (__ballot(point_min(e) <= pthr[e])) psites |= 1u << e;
                         }
                     }
                     if (!hits && !psites) continue;
