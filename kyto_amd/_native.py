@@ -55,7 +55,7 @@ class KnnInfo(ctypes.Structure):
     _fields_ = [("candidates", c_uint64), ("certified", c_uint64), ("fallback", c_uint64),
                 ("search_ms", c_double), ("rerank_ms", c_double), ("fallback_ms", c_double),
                 ("total_ms", c_double), ("scan_kernel_ms", c_double), ("rounds", c_uint64),
-                ("scratch_bytes", c_uint64)]
+                ("scratch_bytes", c_uint64), ("rescanned", c_uint64), ("rescan_ms", c_double)]
 
 
 class StreamInfo(ctypes.Structure):

@@ -449,23 +449,45 @@ static bool cholesky_qr2(const Mat &a, Mat &q, Mat &r) {
     return true;
 }
 
-Mat extend_basis(const Mat &basis, const Mat &block, DeviceBasis *dev) {
-    size_t rows = block.rows;
-    double scale = frobenius(block);
-    if (scale == 0.0 || block.cols == 0) return Mat(rows, 0);
+// The block's scale (its Frobenius norm), 0 for a block that adds nothing.
+static double block_scale(const Mat &block) {
+    const double scale = block.cols == 0 ? 0.0 : frobenius(block);
     if (!std::isfinite(scale))
         throw Error(KY_E215, "pca: orthonormalising a Krylov block did not converge (a value that is not finite)");
-    Mat rest = block;
-    if (basis.cols > 0 && dev) {
+    return scale;
+}
+
+// The orthonormal new directions of `rest` (the block with the basis projected out).
+static Mat new_directions(const Mat &rest, double scale);
+
+Mat extend_basis(const Mat &basis, const Mat &block, DeviceBasis *dev) {
+    if (dev) {
         KY_CHECK(dev->cols == basis.cols, KY_E505, "pca: the device basis is out of step (internal)");
-        dev->project_out(rest);
-    } else if (basis.cols > 0) {
+        return extend_basis(*dev, block);
+    }
+    const double scale = block_scale(block);
+    if (scale == 0.0) return Mat(block.rows, 0);
+    Mat rest = block;
+    if (basis.cols > 0) {
         for (int pass = 0; pass < 2; pass++) {
             Mat proj = mul_tn(basis, rest);   // bc x bw
             Mat back = mul_nn(basis, proj);   // rows x bw
             for (size_t i = 0; i < rest.v.size(); i++) rest.v[i] -= back.v[i];
         }
     }
+    return new_directions(rest, scale);
+}
+
+Mat extend_basis(DeviceBasis &dev, const Mat &block) {
+    const double scale = block_scale(block);
+    if (scale == 0.0) return Mat(block.rows, 0);
+    Mat rest = block;
+    if (dev.cols > 0) dev.project_out(rest);
+    return new_directions(rest, scale);
+}
+
+static Mat new_directions(const Mat &rest, double scale) {
+    const size_t rows = rest.rows;
     size_t n = rest.cols;
     Mat q, r, ur;
     std::vector<double> s;
@@ -696,13 +718,14 @@ static void flip_signs(Mat &m) {
     }
 }
 
-void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<double> &eigenvalues,
-                   Mat &loadings, DeviceBasis *dev) {
-    loadings = Mat(basis.rows, 0);
+// The Rayleigh-Ritz step over a basis of `cols` columns (of `rows` genes): B^T C and
+// B U from the host matrices, or from `dev` when given.
+static void rayleigh_ritz_impl(size_t rows, size_t cols, const Mat *basis, const Mat *products, size_t k,
+                               std::vector<double> &eigenvalues, Mat &loadings, DeviceBasis *dev) {
+    loadings = Mat(rows, 0);
     eigenvalues.clear();
-    if (dev) KY_CHECK(dev->cols == basis.cols, KY_E505, "pca: the device basis is out of step (internal)");
-    if (basis.cols > 0) {
-        Mat t = dev ? dev->rayleigh_matrix() : mul_tn(basis, products);
+    if (cols > 0) {
+        Mat t = dev ? dev->rayleigh_matrix() : mul_tn(*basis, *products);
         size_t p = t.rows;
         Mat sym(p, p);
         for (size_t i = 0; i < p; i++)
@@ -715,7 +738,7 @@ void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<
         size_t taken = std::min(k, p);
         symmetric_eigen(sym, taken, d, chosen);
         for (size_t c = 0; c < taken; c++) eigenvalues.push_back(std::max(d[p - 1 - c], 0.0));
-        loadings = dev ? dev->times(chosen) : mul_nn(basis, chosen);
+        loadings = dev ? dev->times(chosen) : mul_nn(*basis, chosen);
     }
     size_t missing = k - eigenvalues.size();
     if (missing > 0) {
@@ -723,6 +746,16 @@ void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<
         eigenvalues.resize(k, 0.0);
     }
     flip_signs(loadings);
+}
+
+void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<double> &eigenvalues,
+                   Mat &loadings, DeviceBasis *dev) {
+    if (dev) KY_CHECK(dev->cols == basis.cols, KY_E505, "pca: the device basis is out of step (internal)");
+    rayleigh_ritz_impl(basis.rows, basis.cols, &basis, &products, k, eigenvalues, loadings, dev);
+}
+
+void rayleigh_ritz(DeviceBasis &dev, size_t k, std::vector<double> &eigenvalues, Mat &loadings) {
+    rayleigh_ritz_impl(dev.rows, dev.cols, nullptr, nullptr, k, eigenvalues, loadings, &dev);
 }
 
 }  // namespace ky

@@ -47,9 +47,13 @@ struct DeviceBasis {
 // pca_dense.rs:105-129. With `dev` (holding the same columns as `basis`), the
 // projections run on the device.
 Mat extend_basis(const Mat &basis, const Mat &block, DeviceBasis *dev = nullptr);
+// The same over the columns `dev` holds (no host copy of the basis).
+Mat extend_basis(DeviceBasis &dev, const Mat &block);
 
 // pca_dense.rs:141-174. With `dev`, B^T C and B U run on the device.
 void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<double> &eigenvalues,
                    Mat &loadings, DeviceBasis *dev = nullptr);
+// The same over the basis and products `dev` holds.
+void rayleigh_ritz(DeviceBasis &dev, size_t k, std::vector<double> &eigenvalues, Mat &loadings);
 
 }  // namespace ky

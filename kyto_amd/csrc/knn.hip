@@ -466,7 +466,8 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
                                               double bf_err_max, double bf_acc_rel, int floor_mode,
                                               const double *__restrict__ norms,
                                               double *__restrict__ out_keys, uint32_t *__restrict__ out_rows,
-                                              uint32_t *__restrict__ unc_list, uint32_t *__restrict__ unc_count) {
+                                              uint32_t *__restrict__ unc_list, uint32_t *__restrict__ unc_count,
+                                              uint32_t *__restrict__ short_list, uint32_t *__restrict__ short_count) {
     const unsigned wave = threadIdx.x / 64, lane = lane_id();
     const uint64_t i = (uint64_t)blockIdx.x * 4 + wave;
     if (i >= q_count) return;
@@ -566,7 +567,13 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
         out_keys[q * k + lane] = __longlong_as_double((long long)skey);
         out_rows[q * k + lane] = srow;
     }
-    if (lane == 0 && !cert) unc_list[atomicAdd(unc_count, 1u)] = (uint32_t)q;
+    // An uncertified row with fewer than m candidates goes to the short list when one is
+    // given (it is scanned again without a starting threshold), any other to the exact
+    // search.
+    if (lane == 0 && !cert) {
+        if (short_list && cnt < m) short_list[atomicAdd(short_count, 1u)] = (uint32_t)q;
+        else unc_list[atomicAdd(unc_count, 1u)] = (uint32_t)q;
+    }
 }
 
 // Exact search results for listed rows: copy the scan's output (per list position)
@@ -790,7 +797,8 @@ size_t knn_scratch_bytes(const KnnShape &s, bool graph, bool packed) {
     KnnPrep p;
     knn_take_prep(a, s, p);
     a.take<uint32_t>(s.nq);
-    a.take<uint32_t>(1);
+    a.take<uint32_t>(2);
+    if (s.precision == KY_PRECISION_BF16) a.take<uint32_t>(s.nq);
     size_t m1 = a.mark();
     if (s.metric != KY_METRIC_COSINE) a.take<double>(prep_partial_count(s));
     a.release(m1);
@@ -806,6 +814,10 @@ size_t knn_scratch_bytes(const KnnShape &s, bool graph, bool packed) {
     a.release(m1);
     if (s.nq % s.round) {
         take_round(a, s, s.nq % s.round, r);
+        a.release(m1);
+    }
+    if (s.precision == KY_PRECISION_BF16) {
+        a.take<unsigned char>(bf16_rescan_bytes(s, std::min(s.round, s.nq)));
         a.release(m1);
     }
     a.take<unsigned char>(exact_chunk_bytes(s, std::min(s.round, s.nq)));
@@ -900,15 +912,16 @@ void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t
     KnnPrep prep;
     knn_take_prep(a, s, prep);
     uint32_t *unc = a.take<uint32_t>(nq);
-    uint32_t *unc_count = a.take<uint32_t>(1);
+    uint32_t *unc_count = a.take<uint32_t>(2);  // [0]: uncertified rows, [1]: short rows
+    uint32_t *shrt = s.precision == KY_PRECISION_BF16 ? a.take<uint32_t>(nq) : nullptr;
     size_t m1 = a.mark();
     knn_prepare(ctx, s, pts, prep, s.metric == KY_METRIC_COSINE ? nullptr : a.take<double>(prep_partial_count(s)));
     a.release(m1);
     KY_CHECK(prep.finite, KY_E215, "neighbors: the embedding holds a value that is not finite");
-    double search_ms = 0, rerank_ms = 0, fallback_ms = 0, scan_ms = 0;
+    double search_ms = 0, rerank_ms = 0, fallback_ms = 0, scan_ms = 0, rescan_ms = 0;
     uint64_t rounds = 0;
-    uint32_t total = 0;
-    fill_words(st, unc_count, 1, 0u);
+    uint32_t total = 0, n_short = 0;
+    fill_words(st, unc_count, 2, 0u);
     if (prep.filter_ok) {
         Bf16Points b;
         if (s.precision == KY_PRECISION_BF16) {
@@ -938,7 +951,8 @@ void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t
             rr<<<div_up(count, 4), 256, 0, st>>>(
                 pts, n, d, r.rows, r.keys, r.count, s.precision == KY_PRECISION_BF16 ? r.thr : nullptr, round_rows,
                 round_rows ? 0 : lo, count, s.m, k, prep.slack, b.err, b.err_max, b.acc_rel,
-                s.precision == KY_PRECISION_FP32 ? 0 : 1, prep.norms, keys, rows, unc, unc_count);
+                s.precision == KY_PRECISION_FP32 ? 0 : 1, prep.norms, keys, rows, unc, unc_count, shrt,
+                unc_count + 1);
             KY_HIP(hipGetLastError());
             // The round's buffers are released for the next: wait for its work.
             KY_HIP(hipStreamSynchronize(st));
@@ -947,8 +961,36 @@ void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t
             rounds++;
             ctx.advance(count);
         }
-        KY_HIP(hipMemcpyAsync(&total, unc_count, sizeof(total), hipMemcpyDeviceToHost, st));
+        uint32_t counts[2];
+        KY_HIP(hipMemcpyAsync(counts, unc_count, sizeof(counts), hipMemcpyDeviceToHost, st));
         KY_HIP(hipStreamSynchronize(st));
+        n_short = counts[1];
+        // The short rows again, without starting thresholds (in rounds of the same size);
+        // their re-rank adds any still uncertified to the exact search's list.
+        const double t3 = now_ms();
+        const uint64_t chunk = std::min(s.round, s.nq);
+        for (uint64_t off = 0; off < n_short; off += chunk) {
+            const uint64_t count = std::min<uint64_t>(chunk, n_short - off);
+            ctx.check_cancel();
+            Bf16Rescan x;
+            KnnRound r;
+            bf16_take_rescan(a, s, count, x, r);
+            bf16_rescan(ctx, s, b, shrt + off, count, x, r);
+            const auto rr = prep.norms ? rerank<true> : rerank<false>;
+            rr<<<div_up(count, 4), 256, 0, st>>>(pts, n, d, r.rows, r.keys, r.count, r.thr, x.qorig, 0, count, s.m,
+                                                 k, prep.slack, b.err, b.err_max, b.acc_rel, 1, prep.norms, keys,
+                                                 rows, unc, unc_count, nullptr, nullptr);
+            KY_HIP(hipGetLastError());
+            KY_HIP(hipStreamSynchronize(st));
+            a.release(m1);
+        }
+        if (n_short) {
+            KY_HIP(hipMemcpyAsync(&total, unc_count, sizeof(total), hipMemcpyDeviceToHost, st));
+            KY_HIP(hipStreamSynchronize(st));
+        } else {
+            total = counts[0];
+        }
+        rescan_ms = now_ms() - t3;
     } else {
         // Norms too large for f32 keys: every query by the exact search.
         if (q_rows)
@@ -976,6 +1018,8 @@ void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t
         info->candidates = prep.filter_ok ? s.m : 0;
         info->fallback = total;
         info->certified = nq - total;
+        info->rescanned = n_short;
+        info->rescan_ms = rescan_ms;
         info->search_ms = search_ms;
         info->rerank_ms = rerank_ms;
         info->fallback_ms = fallback_ms;

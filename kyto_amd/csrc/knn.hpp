@@ -68,6 +68,12 @@ struct KnnRound {
     float *thr_max = nullptr;  // the largest starting threshold of each super-block of points
 };
 
+// The BF16 rescan's buffers (knn_bf16.hip): the listed rows' scan positions, sorted.
+struct Bf16Rescan {
+    uint32_t *inv = nullptr, *k2 = nullptr, *v2 = nullptr, *qpos = nullptr, *qorig = nullptr;
+    void *tmp = nullptr;
+};
+
 uint32_t candidates_for(uint32_t k, uint64_t n, int precision);
 int padded_width(uint32_t d);
 // The shape of a search on this device (precision switched to FP32 past 58
@@ -92,6 +98,14 @@ template <class A>
 void bf16_take_round(A &a, const KnnShape &s, uint64_t count, KnnRound &r);
 void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t lo, uint64_t count,
                 const KnnRound &r, double *scan_ms);
+// The candidates of `count` listed rows (device list `rows`) again, without starting
+// thresholds, into r (rows qorig[i] of x); buffers from bf16_take_rescan.
+template <class A>
+void bf16_take_rescan(A &a, const KnnShape &s, uint64_t count, Bf16Rescan &x, KnnRound &r);
+// The most device memory a rescan in rounds of `chunk` rows takes.
+size_t bf16_rescan_bytes(const KnnShape &s, uint64_t chunk);
+void bf16_rescan(Context &ctx, const KnnShape &s, const Bf16Points &b, const uint32_t *rows, uint64_t count,
+                 const Bf16Rescan &x, const KnnRound &r);
 
 // Exact neighbours of every row (or of the rows q_rows[0..nq)) of the packed n x d
 // embedding, with its scratch from `a`: candidates at `precision`, exact re-rank and

@@ -1142,7 +1142,7 @@ size_t bf16_prepare_bytes(const KnnShape &s) {
     return a.high;
 }
 
-// The split of one launch's query blocks (see bf16_round): the blocks that scan every
+// The split of one launch's query blocks (see bf16_scan): the blocks that scan every
 // point, and the tail's point segments.
 static void bf16_split(const KnnShape &s, uint64_t count, unsigned &main_blocks, unsigned &n_seg,
                        unsigned &seg_tiles) {
@@ -1152,8 +1152,10 @@ static void bf16_split(const KnnShape &s, uint64_t count, unsigned &main_blocks,
     main_blocks = blocks;
     n_seg = 1;
     seg_tiles = 0;
-    if (blocks > resident && blocks % resident != 0 && 2 * (blocks % resident) <= resident) {
-        const unsigned rem = blocks % resident;
+    // A launch of fewer query blocks than fill half the device (a short list of rows):
+    // every block is split the same way.
+    const unsigned rem = blocks > resident ? blocks % resident : blocks < resident / 2 ? blocks : 0;
+    if (rem != 0 && 2 * rem <= resident) {
         main_blocks = blocks - rem;
         n_seg = (unsigned)std::min<uint64_t>(std::min<unsigned>(MAX_TAIL_SEG, resident / rem), n_tiles);
         seg_tiles = div_up(n_tiles, n_seg);
@@ -1274,15 +1276,15 @@ void bf16_prepare(Context &ctx, const KnnShape &s, const float *pts, const KnnPr
 // alone at the end. Its query blocks are split instead over n_seg point segments
 // (blocks of their own in the same launch, filling the free slots), and merge_tail
 // joins their lists.
-void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t lo, uint64_t count,
-                const KnnRound &r, double *scan_ms) {
+// The queries: at sorted positions qpos[0..count) (ascending), or at qoff + i when qpos
+// is null; thr: their starting thresholds, or null for none.
+static void bf16_scan(Context &ctx, const KnnShape &s, const Bf16Points &b, const uint32_t *qpos, uint64_t lo,
+                      uint64_t count, const float *thr, const KnnRound &r, double *scan_ms) {
     hipStream_t st = ctx.stream;
     const unsigned blocks = div_up(count, QPB);
     const uint64_t main_q = std::min<uint64_t>(count, (uint64_t)r.main_blocks * QPB);
     const uint64_t tail_q = count - main_q;
     const unsigned grid = r.main_blocks + (blocks - r.main_blocks) * r.n_seg;
-    const uint32_t *qpos = s.list ? b.qpos + lo : nullptr;
-    const float *thr = b.thr_est ? b.thr_est + lo : nullptr;
     hipEvent_t ev[2] = {ctx.timer[0], ctx.timer[1]};
     if (r.sym) {
         KY_CHECK(thr && lo == 0 && count == s.n, KY_E505, "neighbors: a symmetric scan of a partial round (internal)");
@@ -1315,14 +1317,79 @@ void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t l
         KY_HIP(hipGetLastError());
     }
     // A query whose starting threshold left fewer than k rows at or below it ends with
-    // fewer than k candidates: the re-rank leaves it uncertified and the exact search
-    // takes it (knn.hip).
+    // fewer than k candidates: the re-rank leaves it uncertified, and the search takes it
+    // again (bf16_rescan, then the exact search; knn.hip).
     if (scan_ms) {
         KY_HIP(hipEventSynchronize(ev[1]));
         float ms = 0.0f;
         KY_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
         *scan_ms += ms;
     }
+}
+
+void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t lo, uint64_t count,
+                const KnnRound &r, double *scan_ms) {
+    bf16_scan(ctx, s, b, s.list ? b.qpos + lo : nullptr, lo, count, b.thr_est ? b.thr_est + lo : nullptr, r, scan_ms);
+}
+
+// ------------------------------------------------------------------- rescan ----
+// The rows a round's re-rank left with fewer than m candidates (a starting threshold
+// below their m-th key, or the symmetric scan's list of the row overflowed) are
+// scanned again as a list, without a starting threshold: each keeps its m smallest
+// pairs over every point, as a full scan without thresholds does.
+
+template <class A>
+void bf16_take_rescan(A &a, const KnnShape &s, uint64_t count, Bf16Rescan &x, KnnRound &r) {
+    KnnShape ls = s;
+    ls.list = true;
+    ls.nq = count;
+    ls.round = count;
+    x.inv = a.template take<uint32_t>(s.n);
+    x.k2 = a.template take<uint32_t>(count);
+    x.v2 = a.template take<uint32_t>(count);
+    x.qpos = a.template take<uint32_t>(count);
+    x.qorig = a.template take<uint32_t>(count);
+    x.tmp = a.template take<unsigned char>(sort_pairs_bytes(count));
+    r.rows = a.template take<uint32_t>(count * s.m);
+    r.keys = a.template take<float>(count * s.m);
+    r.count = a.template take<uint32_t>(count);
+    r.thr = a.template take<float>(count);
+    bf16_take_round(a, ls, count, r);
+}
+template void bf16_take_rescan<Arena>(Arena &, const KnnShape &, uint64_t, Bf16Rescan &, KnnRound &);
+template void bf16_take_rescan<SizeArena>(SizeArena &, const KnnShape &, uint64_t, Bf16Rescan &, KnnRound &);
+
+// The largest of the rescan's bytes for a round of `chunk` rows and for the largest
+// round that is split over point segments (whose buffers hold a query per segment).
+size_t bf16_rescan_bytes(const KnnShape &s, uint64_t chunk) {
+    size_t worst = 0;
+    const uint64_t split = std::min<uint64_t>(chunk, (uint64_t)(std::max<uint32_t>(s.resident, 1) / 2) * QPB);
+    for (const uint64_t count : {chunk, split}) {
+        if (count == 0) continue;
+        SizeArena a;
+        Bf16Rescan x;
+        KnnRound r;
+        bf16_take_rescan(a, s, count, x, r);
+        worst = std::max(worst, a.high);
+    }
+    return worst;
+}
+
+void bf16_rescan(Context &ctx, const KnnShape &s, const Bf16Points &b, const uint32_t *rows, uint64_t count,
+                 const Bf16Rescan &x, const KnnRound &r) {
+    hipStream_t st = ctx.stream;
+    KnnShape ls = s;
+    ls.list = true;
+    ls.nq = count;
+    ls.round = count;
+    // The rows' scan positions, ascending, and the rows in that order.
+    invert_perm<<<div_up(s.n, 256), 256, 0, st>>>(b.perm, s.n, x.inv);
+    query_positions<<<div_up(count, 256), 256, 0, st>>>(rows, count, x.inv, x.k2, x.v2);
+    KY_HIP(hipGetLastError());
+    sort_pairs(st, x.k2, x.qpos, x.v2, x.qorig, count, 32, x.tmp);
+    list_rows<<<div_up(count, 256), 256, 0, st>>>(rows, count, x.qorig);
+    KY_HIP(hipGetLastError());
+    bf16_scan(ctx, ls, b, x.qpos, 0, count, nullptr, r, nullptr);
 }
 
 }  // namespace ky

@@ -1294,11 +1294,10 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
     }
     for (size_t g = 0; g < genes; g++)
         for (size_t j = 0; j < width; j++) sketch.at(g, j) -= means[g] * omega_sums[j];
-    Mat basis(genes, 0), products(genes, 0);
-    // The basis and its products also resident on the device, for the dense steps'
-    // large products.
+    // The basis and its products resident on the device only, where the dense steps'
+    // large products run.
     DeviceBasis dev_basis(*csr.ctx, genes, width * (q + 1));
-    Mat block = extend_basis(basis, sketch, &dev_basis);
+    Mat block = extend_basis(dev_basis, sketch);
     std::vector<double> eigenvalues;
     Mat loadings;
     bool finished = false;
@@ -1313,7 +1312,7 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
         return bm;
     };
     auto finish_basis = [&]() {
-        rayleigh_ritz(basis, products, k, eigenvalues, loadings, &dev_basis);
+        rayleigh_ritz(dev_basis, k, eigenvalues, loadings);
         finished = true;
     };
     if (block.cols == 0) finish_basis();
@@ -1333,13 +1332,11 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
         td = now_ms();
         for (size_t g = 0; g < genes; g++)
             for (size_t j = 0; j < w; j++) product.at(g, j) -= means[g] * wsum[j];
-        basis = basis.beside(block);
-        products = products.beside(product);
         dev_basis.append(block, product);
         if (p == q) {
             finish_basis();
         } else {
-            block = extend_basis(basis, product, &dev_basis);
+            block = extend_basis(dev_basis, product);
             if (block.cols == 0) finish_basis();
         }
         dense_ms += now_ms() - td;

@@ -185,6 +185,36 @@ def test_scan_tail_split_over_point_segments(device):
         assert np.array_equal(g16.rows()[first:first + 400], np.take_along_axis(rows, o, 1))
 
 
+def test_short_rows_are_scanned_again_without_thresholds(device):
+    # 2,000 copies of one row: each copy has 1,999 rows at key 0, far more than a row's
+    # list of earlier rows holds in the symmetric scan, so most copies end their round
+    # with fewer than m candidates; they are scanned again as a list without starting
+    # thresholds (point segments over the device), and the ties then go to the exact
+    # search. A second group of 300 copies stays within the lists. The graph is the FP32
+    # path's on every row and the oracle's on sampled rows, copies included.
+    pts = _large_clustered(160_000, 50, 10, seed=41)
+    pts[10_000:12_000] = pts[5]
+    pts[90_000:90_300] = pts[7]
+    m = device.upload_dense(pts)
+    try:
+        g16 = device.neighbors(m, 15, precision="bf16")
+        g32 = device.neighbors(m, 15, precision="fp32")
+    finally:
+        m.free()
+    assert g16.info["rounds"] == 1
+    # (The earliest copies hold few earlier copies in their lists; the others overflow.)
+    assert g16.info["rescanned"] >= 1_500
+    assert np.array_equal(g16.rows(), g32.rows())
+    assert np.array_equal(g16.distances.view(np.uint32), g32.distances.view(np.uint32))
+    rows = np.unique(np.concatenate([np.arange(0, 160_000, 4001), np.arange(9_990, 10_020),
+                                     np.arange(11_990, 12_010), np.arange(90_290, 90_310)])).astype(np.uint32)
+    keys, want, _ = oracle.knn_exact_rows(pts, 14, rows)
+    o = np.argsort(want, axis=1, kind="stable")
+    assert np.array_equal(g16.rows()[rows], np.take_along_axis(want, o, 1))
+    want_dist = np.sqrt(np.take_along_axis(keys, o, 1)).astype(np.float32)
+    assert np.array_equal(g16.distances.reshape(-1, 14)[rows].view(np.uint32), want_dist.view(np.uint32))
+
+
 def test_symmetric_scan_equals_the_exact_graph(device):
     # One round over every row (above 2^17 rows) keys each pair of rows once, for both
     # rows: each row's candidates join its own scan of the later rows with the keys the
