@@ -284,8 +284,9 @@ typedef struct ky_knn_info {
     double scan_kernel_ms;  /* BF16: device time of the candidate scan launches (HIP events) */
     uint64_t rounds;        /* rounds of queries the candidate search ran in */
     uint64_t scratch_bytes; /* device scratch of the call (one region, every buffer of the search) */
-    uint64_t rescanned;     /* BF16: rows left with fewer than m candidates, scanned again without a
-                               starting threshold before any exact search */
+    uint64_t rescanned;     /* BF16: rows left with fewer than m candidates (at least 256 of them),
+                               scanned again from twice their starting threshold before any exact
+                               search */
     double rescan_ms;
 } ky_knn_info;
 

@@ -568,7 +568,7 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
         out_rows[q * k + lane] = srow;
     }
     // An uncertified row with fewer than m candidates goes to the short list when one is
-    // given (it is scanned again without a starting threshold), any other to the exact
+    // given (it is scanned again from a higher starting threshold), any other to the exact
     // search.
     if (lane == 0 && !cert) {
         if (short_list && cnt < m) short_list[atomicAdd(short_count, 1u)] = (uint32_t)q;
@@ -900,6 +900,9 @@ uint32_t check_search(const Dense &points, uint32_t n_components, uint32_t n_nei
     return used;
 }
 
+// Fewest short rows a BF16 rescan takes (fewer go to the exact search).
+constexpr uint32_t RESCAN_MIN_ROWS = 256;
+
 void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t d, uint32_t k, int metric,
                   int precision, const uint32_t *q_rows, uint64_t nq, uint64_t round, double *keys, uint32_t *rows,
                   ky_knn_info *info, bool timed) {
@@ -965,9 +968,17 @@ void exact_search(Context &ctx, Arena &a, const float *pts, uint64_t n, uint32_t
         KY_HIP(hipMemcpyAsync(counts, unc_count, sizeof(counts), hipMemcpyDeviceToHost, st));
         KY_HIP(hipStreamSynchronize(st));
         n_short = counts[1];
-        // The short rows again, without starting thresholds (in rounds of the same size);
-        // their re-rank adds any still uncertified to the exact search's list.
         const double t3 = now_ms();
+        // Fewer short rows than one query block take the exact search at once (a rescan's
+        // fixed costs, a sort and a launch over the device, are then above the exact
+        // search's).
+        if (n_short > 0 && n_short < RESCAN_MIN_ROWS) {
+            KY_HIP(hipMemcpyAsync(unc + counts[0], shrt, sizeof(uint32_t) * n_short, hipMemcpyDeviceToDevice, st));
+            counts[0] += n_short;
+            n_short = 0;
+        }
+        // The short rows again, from higher starting thresholds (in rounds of the same size);
+        // their re-rank adds any still uncertified to the exact search's list.
         const uint64_t chunk = std::min(s.round, s.nq);
         for (uint64_t off = 0; off < n_short; off += chunk) {
             const uint64_t count = std::min<uint64_t>(chunk, n_short - off);

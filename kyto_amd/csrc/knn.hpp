@@ -25,6 +25,7 @@ struct KnnShape {
     bool list = false;     // queries are a list of rows (else every row, in order)
     uint64_t round = 0;    // queries a round
     uint32_t resident = 0; // search_bf16 blocks resident on the device at once
+    uint64_t slots = 0;    // a BF16 launch's most query slots (queries x point segments; 0: any)
 };
 
 // The embedding prepared for the candidate search (neighbors_exact.rs:76-180).
@@ -72,6 +73,7 @@ struct KnnRound {
 struct Bf16Rescan {
     uint32_t *inv = nullptr, *k2 = nullptr, *v2 = nullptr, *qpos = nullptr, *qorig = nullptr;
     void *tmp = nullptr;
+    float *thr = nullptr;  // starting thresholds (null: none)
 };
 
 uint32_t candidates_for(uint32_t k, uint64_t n, int precision);
@@ -98,8 +100,8 @@ template <class A>
 void bf16_take_round(A &a, const KnnShape &s, uint64_t count, KnnRound &r);
 void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t lo, uint64_t count,
                 const KnnRound &r, double *scan_ms);
-// The candidates of `count` listed rows (device list `rows`) again, without starting
-// thresholds, into r (rows qorig[i] of x); buffers from bf16_take_rescan.
+// The candidates of `count` listed rows (device list `rows`) again, from twice their
+// starting thresholds, into r (rows qorig[i] of x); buffers from bf16_take_rescan.
 template <class A>
 void bf16_take_rescan(A &a, const KnnShape &s, uint64_t count, Bf16Rescan &x, KnnRound &r);
 // The most device memory a rescan in rounds of `chunk` rows takes.

@@ -214,6 +214,13 @@ __global__ void list_rows(const uint32_t *__restrict__ q_rows, uint64_t nq, uint
     if (i < nq) qorig[i] = q_rows[qorig[i]];
 }
 
+// The rescan's starting thresholds: `factor` times the first round's, by scan position.
+__global__ void rescan_thresholds(const uint32_t *__restrict__ qpos, uint64_t nq, const float *__restrict__ thr,
+                                  float factor, float *__restrict__ out) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < nq) out[i] = factor * thr[qpos[i]];
+}
+
 __global__ void bf16_delta(const double *__restrict__ nb, uint64_t n, double nb_max, double c, double absolute,
                            double *__restrict__ delta) {
     uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -337,7 +344,7 @@ constexpr uint64_t EST_MIN_ROWS = 1 << 17;  // threshold estimates from a sample
 constexpr uint32_t EST_STRIDE = 12;
 
 constexpr unsigned SCAP = 64;  // staged appends per wavefront
-constexpr unsigned MAX_TAIL_SEG = 16;                // point segments of a tail query block
+constexpr unsigned MAX_TAIL_SEG = 64;                // point segments of a tail query block
 constexpr unsigned MERGE_CAP = MAX_TAIL_SEG * 64;    // a tail query's merged lists (m <= 64)
 
 // The candidate scan. Points: the np rows of the scan order (P'). Queries: i < nq, query i being row qpos[i] of the scan order (row
@@ -1152,12 +1159,14 @@ static void bf16_split(const KnnShape &s, uint64_t count, unsigned &main_blocks,
     main_blocks = blocks;
     n_seg = 1;
     seg_tiles = 0;
-    // A launch of fewer query blocks than fill half the device (a short list of rows):
-    // every block is split the same way.
-    const unsigned rem = blocks > resident ? blocks % resident : blocks < resident / 2 ? blocks : 0;
+    // A rescan (a launch with a slot budget) of fewer query blocks than fill half the
+    // device: every block is split the same way, within the budget.
+    const unsigned rem = blocks > resident ? blocks % resident : s.slots && blocks < resident / 2 ? blocks : 0;
     if (rem != 0 && 2 * rem <= resident) {
         main_blocks = blocks - rem;
         n_seg = (unsigned)std::min<uint64_t>(std::min<unsigned>(MAX_TAIL_SEG, resident / rem), n_tiles);
+        // Within the launch's query slots (each segment of a query holds a buffer).
+        if (s.slots) n_seg = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(n_seg, s.slots / count));
         seg_tiles = div_up(n_tiles, n_seg);
         n_seg = div_up(n_tiles, seg_tiles);  // no empty segment
     }
@@ -1333,10 +1342,11 @@ void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t l
 }
 
 // ------------------------------------------------------------------- rescan ----
+constexpr float RESCAN_FACTOR = 2.0f;
 // The rows a round's re-rank left with fewer than m candidates (a starting threshold
 // below their m-th key, or the symmetric scan's list of the row overflowed) are
-// scanned again as a list, without a starting threshold: each keeps its m smallest
-// pairs over every point, as a full scan without thresholds does.
+// scanned again as a list, from twice the first round's starting threshold (none
+// below 2^17 rows): each keeps its m smallest pairs at or below it.
 
 template <class A>
 void bf16_take_rescan(A &a, const KnnShape &s, uint64_t count, Bf16Rescan &x, KnnRound &r) {
@@ -1344,12 +1354,14 @@ void bf16_take_rescan(A &a, const KnnShape &s, uint64_t count, Bf16Rescan &x, Kn
     ls.list = true;
     ls.nq = count;
     ls.round = count;
+    ls.slots = std::min(s.round, s.nq);  // the first rounds' buffers bound the rescan's
     x.inv = a.template take<uint32_t>(s.n);
     x.k2 = a.template take<uint32_t>(count);
     x.v2 = a.template take<uint32_t>(count);
     x.qpos = a.template take<uint32_t>(count);
     x.qorig = a.template take<uint32_t>(count);
     x.tmp = a.template take<unsigned char>(sort_pairs_bytes(count));
+    x.thr = s.n >= EST_MIN_ROWS ? a.template take<float>(count) : nullptr;
     r.rows = a.template take<uint32_t>(count * s.m);
     r.keys = a.template take<float>(count * s.m);
     r.count = a.template take<uint32_t>(count);
@@ -1382,6 +1394,7 @@ void bf16_rescan(Context &ctx, const KnnShape &s, const Bf16Points &b, const uin
     ls.list = true;
     ls.nq = count;
     ls.round = count;
+    ls.slots = std::min(s.round, s.nq);
     // The rows' scan positions, ascending, and the rows in that order.
     invert_perm<<<div_up(s.n, 256), 256, 0, st>>>(b.perm, s.n, x.inv);
     query_positions<<<div_up(count, 256), 256, 0, st>>>(rows, count, x.inv, x.k2, x.v2);
@@ -1389,7 +1402,15 @@ void bf16_rescan(Context &ctx, const KnnShape &s, const Bf16Points &b, const uin
     sort_pairs(st, x.k2, x.qpos, x.v2, x.qorig, count, 32, x.tmp);
     list_rows<<<div_up(count, 256), 256, 0, st>>>(rows, count, x.qorig);
     KY_HIP(hipGetLastError());
-    bf16_scan(ctx, ls, b, x.qpos, 0, count, nullptr, r, nullptr);
+    // Twice the first round's starting threshold (a key): a row its list could not hold
+    // keeps everything the first round would have, one whose threshold left fewer than m
+    // rows gains the rows up to twice its key (most reach m; the others go on to the
+    // exact search).
+    if (x.thr && b.thr_est) {
+        rescan_thresholds<<<div_up(count, 256), 256, 0, st>>>(x.qpos, count, b.thr_est, RESCAN_FACTOR, x.thr);
+        KY_HIP(hipGetLastError());
+    }
+    bf16_scan(ctx, ls, b, x.qpos, 0, count, x.thr && b.thr_est ? x.thr : nullptr, r, nullptr);
 }
 
 }  // namespace ky

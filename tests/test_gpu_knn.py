@@ -188,9 +188,9 @@ def test_scan_tail_split_over_point_segments(device):
 def test_short_rows_are_scanned_again_without_thresholds(device):
     # 2,000 copies of one row: each copy has 1,999 rows at key 0, far more than a row's
     # list of earlier rows holds in the symmetric scan, so most copies end their round
-    # with fewer than m candidates; they are scanned again as a list without starting
-    # thresholds (point segments over the device), and the ties then go to the exact
-    # search. A second group of 300 copies stays within the lists. The graph is the FP32
+    # with fewer than m candidates; they are scanned again as a list from twice their
+    # starting thresholds (point segments over the device), and the ties then go to the
+    # exact search. A second group of 300 copies stays within the lists. The graph is the FP32
     # path's on every row and the oracle's on sampled rows, copies included.
     pts = _large_clustered(160_000, 50, 10, seed=41)
     pts[10_000:12_000] = pts[5]
