@@ -50,6 +50,7 @@ namespace ky {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
 
 constexpr int KDIM = 64;        // augmented components
 constexpr int TILE = 128;       // points per tile
@@ -698,17 +699,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     }
                     // Keys at or below their point's threshold, staged for the points' lists
                     // (walked key by key, then column block by column block).
+                    // The keys of a point are walked with a uniform index into the lanes'
+                    // key vectors and thresholds (VGPR indexing), not unrolled: one copy of
+                    // the append code a column block (the kernel's code stays small enough
+                    // for the instruction cache).
                     if (psites) {
-                        float pthr[4 * MB];
-                        point_thresholds(pthr);
+                        f32x8 pthr;
+                        {
+                            float pt[4 * MB];
+                            point_thresholds(pt);
 #pragma unroll
-                        for (int e = 0; e < 4 * MB; e++) {
-                            if (!(psites & (1u << e))) continue;
+                            for (int e = 0; e < 4 * MB; e++) pthr[e] = pt[e];
+                        }
+                        while (psites) {
+                            const int e = __builtin_ctz(psites);
+                            psites &= psites - 1;
                             const uint32_t p = (uint32_t)(pbase + 16 * (e / 4) + (e % 4));
+                            const float pt = pthr[e];
 #pragma unroll
                             for (int b = 0; b < NB; b++) {
                                 const float x = acc[b][e];
-                                const bool h = x <= pthr[e];
+                                const bool h = x <= pt;
                                 const uint64_t mh = __ballot(h);
                                 if (!mh) continue;
                                 if (staged + (unsigned)__popcll(mh) > SCAP) flush(sk, sr, sd);
