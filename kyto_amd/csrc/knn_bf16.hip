@@ -431,6 +431,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         cb[b] = 0;
     }
     unsigned staged = 0;  // wave-uniform
+    // Whether the wave appended to its queries' buffers since the last compaction check
+    // (wave-uniform): without appends the counts and thresholds are unchanged, and so
+    // is the check's answer (none needs compacting).
+    bool appended = true;
     const bool wave_live = qbase < nq;
     // The staged appends to the buffers (one store instruction). In the symmetric scan
     // a stage entry with bit 15 of its destination set is a key for the point's own row
@@ -664,6 +668,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     // a column block are walked with a uniform index into a 16-register
                     // vector (VGPR indexing), not unrolled.
                     const uint64_t pbase = t0 + sb * SPB + 4 * (lane / 16);
+                    if (hits) appended = true;
 #pragma unroll
                     for (int b = 0; b < NB; b++) {
                         if (!__ballot(hit[b])) continue;
@@ -739,11 +744,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 // then tightens to their m-th key).
                 uint64_t need[NB];
                 bool any_need = false;
+                if (appended) {
 #pragma unroll
-                for (int b = 0; b < NB; b++) {
-                    const unsigned c = cb[b];
-                    need[b] = __ballot(c > BCAP - TILE || (c >= m && (thr[b] == INFINITY || c >= 2 * m + 64))) & 0xFFFFull;
-                    any_need = any_need || need[b];
+                    for (int b = 0; b < NB; b++) {
+                        const unsigned c = cb[b];
+                        need[b] = __ballot(c > BCAP - TILE || (c >= m && (thr[b] == INFINITY || c >= 2 * m + 64))) & 0xFFFFull;
+                        any_need = any_need || need[b];
+                    }
+                    appended = false;
                 }
                 if (any_need) {
                     flush(sk, sr, sd);

@@ -1109,17 +1109,14 @@ static void launch_row_project(Csr &csr, const float *M, uint32_t w, uint32_t ld
     if (t_clock)
         t_clock->begin(0, ctx.stream,
                        8.0 * csr.nnz + 8.0 * (csr.n_rows + 1.0) + 4.0 * csr.n_cols * w + 4.0 * csr.n_rows * w);
-    // M's column slice in LDS: 16 columns a slice up to 2,048 genes, 8 up to 4,096.
+    // M's column slice in LDS: 16 columns a slice up to 2,048 genes. Past that the
+    // gathers from L2 (row_project) beat 8-column slices (at 4M x 3,000: 13.5 against
+    // 18.7 ms a launch; each entry is then streamed 8 times).
     const unsigned lds_rows_per_block = 8192;
     const unsigned lds_blocks = 8 * div_up(div_up(csr.n_rows, lds_rows_per_block), 8);
     if (w <= 64 && ldm >= 64 && ldm % 4 == 0 && (size_t)csr.n_cols * 16 <= LDS_FLOATS) {
         const uint32_t n_slices = div_up(w, 16);
         row_project_lds<16><<<lds_blocks * n_slices, 1024, 0, ctx.stream>>>(
-            csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows, M, csr.n_cols, w, ldm, offsets, out, ldo,
-            n_slices, lds_rows_per_block);
-    } else if (w <= 64 && ldm >= 64 && ldm % 4 == 0 && (size_t)csr.n_cols * 8 <= LDS_FLOATS) {
-        const uint32_t n_slices = div_up(w, 8);
-        row_project_lds<8><<<lds_blocks * n_slices, 1024, 0, ctx.stream>>>(
             csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, csr.n_rows, M, csr.n_cols, w, ldm, offsets, out, ldo,
             n_slices, lds_rows_per_block);
     } else if (w <= 64)
