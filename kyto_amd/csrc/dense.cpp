@@ -388,7 +388,7 @@ static Mat gram(const Mat &a) {
 }
 
 // g = r^T r with r upper triangular; false when a pivot is not positive.
-static bool cholesky_upper(const Mat &g, Mat &r) {
+bool cholesky_factor(const Mat &g, Mat &r) {
     const size_t n = g.rows;
     r = Mat(n, n);
     for (size_t j = 0; j < n; j++) {
@@ -422,7 +422,7 @@ static void solve_right_upper(Mat &a, const Mat &r) {
 }
 
 // |r|_F |r^-1|_F, an upper bound of the 2-norm condition number of r.
-static double condition_bound(const Mat &r) {
+double condition_bound_of(const Mat &r) {
     const double lo = min_singular_bound(r);
     return lo > 0.0 ? frobenius(r) / lo : INFINITY;
 }
@@ -433,10 +433,10 @@ static double condition_bound(const Mat &r) {
 // Householder QR's two per column.
 static bool cholesky_qr2(const Mat &a, Mat &q, Mat &r) {
     Mat r1, r2;
-    if (!cholesky_upper(gram(a), r1) || condition_bound(r1) > 1e6) return false;
+    if (!cholesky_factor(gram(a), r1) || condition_bound_of(r1) > 1e6) return false;
     q = a;
     solve_right_upper(q, r1);
-    if (!cholesky_upper(gram(q), r2)) return false;
+    if (!cholesky_factor(gram(q), r2)) return false;
     solve_right_upper(q, r2);
     const size_t n = r1.rows;
     r = Mat(n, n);
@@ -459,6 +459,7 @@ static double block_scale(const Mat &block) {
 
 // The orthonormal new directions of `rest` (the block with the basis projected out).
 static Mat new_directions(const Mat &rest, double scale);
+static Mat keep_directions(const Mat &q, const Mat &r, double scale);
 
 Mat extend_basis(const Mat &basis, const Mat &block, DeviceBasis *dev) {
     if (dev) {
@@ -481,20 +482,41 @@ Mat extend_basis(const Mat &basis, const Mat &block, DeviceBasis *dev) {
 Mat extend_basis(DeviceBasis &dev, const Mat &block) {
     const double scale = block_scale(block);
     if (scale == 0.0) return Mat(block.rows, 0);
-    Mat rest = block;
-    if (dev.cols > 0) dev.project_out(rest);
-    return new_directions(rest, scale);
+    KY_CHECK(block.rows >= block.cols, KY_E505, "pca: a Krylov block wider than the genes (internal)");
+    if (block.cols > 64) {
+        Mat rest = block;
+        if (dev.cols > 0) dev.project_out(rest);
+        return new_directions(rest, scale);
+    }
+    // The projection and Cholesky QR twice on the device (the block stays there), the
+    // small factors on the host; a block the first factor finds ill conditioned goes
+    // to the host's Householder QR.
+    dev.project_out_resident(block);
+    Mat q, r;
+    if (!dev.cholesky_qr2(block.cols, q, r)) {
+        Mat rest = q;  // the projected block
+        return new_directions(rest, scale);
+    }
+    return keep_directions(q, r, scale);
 }
 
 static Mat new_directions(const Mat &rest, double scale) {
     const size_t rows = rest.rows;
     size_t n = rest.cols;
-    Mat q, r, ur;
-    std::vector<double> s;
+    Mat q, r;
     KY_CHECK(rows >= n, KY_E505, "pca: a Krylov block wider than the genes (internal)");
     // A well-conditioned block (every Krylov block of a PCA here) by Cholesky QR twice,
     // any other by Householder.
     if (!cholesky_qr2(rest, q, r)) householder_qr(rest, q, r);
+    return keep_directions(q, r, scale);
+}
+
+// The block's directions from its thin QR (q, r): q itself when r is provably of full
+// rank, else q times the right singular vectors of r above the drop tolerance.
+static Mat keep_directions(const Mat &q, const Mat &r, double scale) {
+    const size_t rows = q.rows, n = q.cols;
+    Mat ur;
+    std::vector<double> s;
     // Full rank (every singular value of R above the drop tolerance) keeps the whole
     // block, and the new columns may then be Q itself: span(Q U) = span(Q) for a square
     // orthogonal U, and the passes and the Rayleigh-Ritz step depend on the basis only

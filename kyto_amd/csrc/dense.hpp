@@ -25,6 +25,10 @@ struct Mat {
 };
 
 Mat mul_tn(const Mat &a, const Mat &b);
+// g = r^T r (r upper triangular); false when a pivot is not positive or not finite.
+bool cholesky_factor(const Mat &g, Mat &r);
+// |r|_F |r^-1|_F, an upper bound of the 2-norm condition number of upper triangular r.
+double condition_bound_of(const Mat &r);
 Mat mul_nn(const Mat &a, const Mat &b);
 
 // The basis B and its products C resident on the device (rows x cap each, row-major),
@@ -38,6 +42,14 @@ struct DeviceBasis {
     void append(const Mat &block, const Mat &product);
     // CGS2 against the held columns: rest -= B (B^T rest), twice.
     void project_out(Mat &rest);
+    // The block on the device (work_r) with the held columns projected out, CGS2 as
+    // project_out; it stays there for cholesky_qr2.
+    void project_out_resident(const Mat &block);
+    // Cholesky QR twice of the resident block (rows x w, w <= 64): q into `q` (host) and
+    // the triangular factor into `r`; false when the first factor's condition bound is
+    // above 1e6 or a pivot is not positive (then the host's Householder QR takes the
+    // block, downloaded into `q`).
+    bool cholesky_qr2(size_t w, Mat &q, Mat &r);
     // B^T C (cols x cols).
     Mat rayleigh_matrix();
     // B S (rows x S.cols).

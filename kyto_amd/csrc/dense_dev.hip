@@ -56,6 +56,24 @@ __global__ __launch_bounds__(256) void nn_rows(const double *__restrict__ A, uin
     C[r * ldc + j] = subtract ? C[r * ldc + j] - s : s;
 }
 
+// a <- a r^-1 for each row of a (m x n, n <= 64; r upper triangular, n x n dense):
+// one wavefront a row, lane j holding column j; column k is final once the columns
+// before it are subtracted, and is then broadcast to the lanes after it (forward
+// substitution in k order, fused multiply-adds).
+__global__ __launch_bounds__(256) void solve_rows_upper(double *__restrict__ a, uint64_t m, uint32_t n,
+                                                        const double *__restrict__ r) {
+    const uint64_t row = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    if (row >= m) return;
+    const unsigned j = threadIdx.x % 64;
+    double x = j < n ? a[row * n + j] : 0.0;
+    for (uint32_t k = 0; k < n; k++) {
+        const double vk = __shfl(x, (int)k) / r[(uint64_t)k * n + k];
+        if (j == k) x = vk;
+        if (j > k && j < n) x = fma(-vk, r[(uint64_t)k * n + j], x);
+    }
+    if (j < n) a[row * n + j] = x;
+}
+
 }  // namespace
 
 DeviceBasis::DeviceBasis(Context &c, size_t rows_, size_t cap_) : ctx(&c), rows(rows_), cap(cap_) {
@@ -113,6 +131,50 @@ void DeviceBasis::project_out(Mat &rest) {
         KY_HIP(hipGetLastError());
     }
     download(*ctx, rest.v.data(), work_r.ptr, rows * w * sizeof(double));
+}
+
+void DeviceBasis::project_out_resident(const Mat &rest) {
+    const uint32_t w = (uint32_t)rest.cols, bc = (uint32_t)cols;
+    upload_mat(*ctx, work_r, rest);
+    if (cols == 0 || w == 0) return;
+    work_p.reserve((size_t)bc * w);
+    for (int pass = 0; pass < 2; pass++) {
+        tn(*ctx, basis.ptr, cap, work_r.ptr, w, rows, bc, w, partial, work_p.ptr);
+        nn_rows<<<div_up(rows * w, 256), 256, 0, ctx->stream>>>(basis.ptr, cap, work_p.ptr, rows, bc, w, work_r.ptr,
+                                                                w, 1);
+        KY_HIP(hipGetLastError());
+    }
+}
+
+bool DeviceBasis::cholesky_qr2(size_t w, Mat &q, Mat &r) {
+    KY_CHECK(w >= 1 && w <= 64, KY_E505, "pca: a device Cholesky QR of more than 64 columns (internal)");
+    const uint32_t n = (uint32_t)w;
+    q = Mat(rows, w);
+    work_p.reserve(w * w);
+    Mat g(w, w), factor[2];
+    for (int pass = 0; pass < 2; pass++) {
+        // The Gram matrix of the resident block, its Cholesky factor on the host, the
+        // block times the factor's inverse on the device.
+        tn(*ctx, work_r.ptr, n, work_r.ptr, n, rows, n, n, partial, work_p.ptr);
+        download(*ctx, g.v.data(), work_p.ptr, w * w * sizeof(double));
+        if (!cholesky_factor(g, factor[pass]) || (pass == 0 && condition_bound_of(factor[0]) > 1e6)) {
+            download(*ctx, q.v.data(), work_r.ptr, rows * w * sizeof(double));
+            return false;
+        }
+        upload_mat(*ctx, work_p, factor[pass]);
+        solve_rows_upper<<<div_up(rows, 4), 256, 0, ctx->stream>>>(work_r.ptr, rows, n, work_p.ptr);
+        KY_HIP(hipGetLastError());
+    }
+    download(*ctx, q.v.data(), work_r.ptr, rows * w * sizeof(double));
+    // r = r2 r1 (both upper triangular).
+    r = Mat(w, w);
+    for (size_t i = 0; i < w; i++)
+        for (size_t j = i; j < w; j++) {
+            double v = 0.0;
+            for (size_t k = i; k <= j; k++) v += factor[1].at(i, k) * factor[0].at(k, j);
+            r.at(i, j) = v;
+        }
+    return true;
 }
 
 Mat DeviceBasis::rayleigh_matrix() {
