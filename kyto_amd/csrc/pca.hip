@@ -180,6 +180,57 @@ __global__ __launch_bounds__(1024) void build_columns(const uint64_t *__restrict
     }
 }
 
+// A gene's run in a chunk with at least this many entries is summed by a whole
+// wavefront (wave_run), a shorter one by a lane group (slice_walk).
+constexpr uint32_t LONG_RUN = 256;
+
+// One block per chunk: the chunk's genes ordered by the length of their runs, longest
+// first (a counting sort over lengths 0 .. chunk_rows: LDS counts, a descending scan,
+// then each gene placed at its length's next slot), and the number of runs of at
+// least LONG_RUN entries after them: order[c * (n_cols + 1) + i] for i < n_cols, then
+// the count. Genes of equal length land in any order among themselves; the column
+// product sums every run the same way wherever it stands, so its sums do not depend
+// on that order. LDS: chunk_rows + 1 counters.
+__global__ __launch_bounds__(1024) void order_runs(const uint32_t *__restrict__ col_ptr, uint32_t n_cols,
+                                                   uint32_t chunk_rows, uint32_t *__restrict__ order) {
+    extern __shared__ uint32_t slot[];
+    const unsigned c = blockIdx.x;
+    const uint32_t *cp = col_ptr + (size_t)c * (n_cols + 1);
+    uint32_t *out = order + (size_t)c * (n_cols + 1);
+    const uint32_t bins = chunk_rows + 1;
+    for (uint32_t i = threadIdx.x; i < bins; i += blockDim.x) slot[i] = 0;
+    __syncthreads();
+    for (uint32_t g = threadIdx.x; g < n_cols; g += blockDim.x) atomicAdd(&slot[min(cp[g + 1] - cp[g], chunk_rows)], 1u);
+    __syncthreads();
+    // Descending exclusive scan: slot[len] = genes with longer runs. Each thread takes
+    // a run of bins from the top, the runs' sums are scanned (Hillis-Steele), then each
+    // thread writes its run.
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (bins + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = min(threadIdx.x * per, bins), b1 = min(b0 + per, bins);
+    uint32_t local = 0;
+    for (uint32_t b = b0; b < b1; b++) local += slot[bins - 1 - b];
+    part[threadIdx.x] = local;
+    __syncthreads();
+    for (unsigned off = 1; off < blockDim.x; off <<= 1) {
+        const uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - local;
+    for (uint32_t b = b0; b < b1; b++) {
+        const uint32_t x = slot[bins - 1 - b];
+        slot[bins - 1 - b] = run;
+        run += x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) out[n_cols] = LONG_RUN <= chunk_rows ? slot[LONG_RUN - 1] : 0u;
+    __syncthreads();
+    for (uint32_t g = threadIdx.x; g < n_cols; g += blockDim.x)
+        out[atomicAdd(&slot[min(cp[g + 1] - cp[g], chunk_rows)], 1u)] = g;
+}
+
 // ------------------------------------------------------------------- Omega ----
 
 // W[r, j] = f32(Omega[cell_offset + r, j]) for j < l, and W[r, l] = 1.
@@ -614,24 +665,23 @@ __global__ __launch_bounds__(1024) void row_project_lds(const uint64_t *__restri
         });
 }
 
-// A gene's run in a chunk with at least this many entries is summed by a whole
-// wavefront (wave_run), a shorter one by a lane group (slice_walk).
-constexpr uint32_t LONG_RUN = 256;
-
 // column_product with W's column slice of the chunk's rows in LDS (chunk_rows * SW <=
 // LDS_FLOATS, W's columns up to n_slices * SW readable, rows 16-byte aligned). The
-// wavefronts first take the chunk's genes 64 at a time and sum each long run among them
-// whole, then sum the short runs as lane groups. partial as column_product's.
+// chunk's runs are taken longest first (order, from order_runs): the wavefronts first
+// sum the long runs one at a time, whole, then sum the short runs as lane groups, so
+// the runs left when the block's groups run out of work are its shortest. partial as
+// column_product's.
 template <int SW>
 __global__ __launch_bounds__(1024) void column_product_lds(const uint32_t *__restrict__ col_ptr,
                                                            const uint32_t *__restrict__ col_row,
-                                                           const float *__restrict__ col_val, uint32_t n_cols,
+                                                           const float *__restrict__ col_val,
+                                                           const uint32_t *__restrict__ order, uint32_t n_cols,
                                                            uint32_t n_chunks, uint32_t chunk_rows, uint64_t n_rows,
                                                            const float *__restrict__ W, uint32_t w, uint32_t ldw,
                                                            uint32_t n_slices, double *__restrict__ partial) {
     using S = SliceShape<SW>;
     __shared__ __attribute__((aligned(16))) float ws[LDS_FLOATS];
-    __shared__ uint32_t counter[2];  // next gene batch of the long runs, next short run
+    __shared__ uint32_t counter[2];  // next long run, next short run
     unsigned chunk, slice;
     slice_block(n_slices, chunk, slice);
     if (chunk >= n_chunks) return;
@@ -648,6 +698,8 @@ __global__ __launch_bounds__(1024) void column_product_lds(const uint32_t *__res
             *reinterpret_cast<const float4 *>(W + (r0 + i / (SW / 4)) * ldw + c0 + 4 * (i % (SW / 4)));
     __syncthreads();
     const uint32_t *cp = col_ptr + (size_t)chunk * (n_cols + 1);
+    const uint32_t *ord = order + (size_t)chunk * (n_cols + 1);
+    const uint32_t n_long = ord[n_cols];
     const uint32_t base = cp[0], count = cp[n_cols] - base;
     const __amdgpu_buffer_rsrc_t ri = entry_rsrc(col_row + base, count), rv = entry_rsrc(col_val + base, count);
     const unsigned lane = lane_id(), place = lane % S::LPR;
@@ -658,41 +710,30 @@ __global__ __launch_bounds__(1024) void column_product_lds(const uint32_t *__res
         for (int c = 0; c < 4; c++)
             if (col + c < w) o[col + c] = acc[c];
     };
-    // The long runs.
+    // The long runs, one a wavefront at a time.
     for (;;) {
-        uint32_t b0 = 0;
-        if (lane == 0) b0 = atomicAdd(&counter[0], 64u);
-        b0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)b0);
-        if (b0 >= n_cols) break;
-        const uint32_t g = b0 + lane;
-        uint32_t s = 0, e = 0;
-        if (g < n_cols) {
-            s = cp[g] - base;
-            e = cp[g + 1] - base;
-        }
-        uint64_t longs = __builtin_amdgcn_ballot_w64(e - s >= LONG_RUN);
-        while (longs) {
-            const int i = __builtin_ctzll(longs);
-            longs &= longs - 1;
-            const uint32_t rs = (uint32_t)__builtin_amdgcn_readlane((int)s, i);
-            const uint32_t re = (uint32_t)__builtin_amdgcn_readlane((int)e, i);
-            double acc[4];
-            wave_run<SW>(reinterpret_cast<const char *>(ws), ri, rv, rs, re, lane / S::LPR, place, acc);
-            if (lane < S::LPR) store(b0 + i, acc);
-        }
+        uint32_t i = 0;
+        if (lane == 0) i = atomicAdd(&counter[0], 1u);
+        i = (uint32_t)__builtin_amdgcn_readfirstlane((int)i);
+        if (i >= n_long) break;
+        const uint32_t g = ord[i];
+        double acc[4];
+        wave_run<SW>(reinterpret_cast<const char *>(ws), ri, rv, cp[g] - base, cp[g + 1] - base, lane / S::LPR,
+                     place, acc);
+        if (lane < S::LPR) store(g, acc);
     }
-    // The short runs.
+    // The short runs (run r of the walk is the chunk's (n_long + r)-th longest).
+    const uint32_t *short_ord = ord + n_long;
     const double init[4] = {0.0, 0.0, 0.0, 0.0};
     slice_walk<SW>(
-        ws, ri, rv, n_cols, &counter[1], wave_in_block() * S::RPW + lane / S::LPR, place, init,
-        [&](uint32_t g, uint32_t &s, uint32_t &e) {
+        ws, ri, rv, n_cols - n_long, &counter[1], wave_in_block() * S::RPW + lane / S::LPR, place, init,
+        [&](uint32_t r, uint32_t &s, uint32_t &e) {
+            const uint32_t g = short_ord[r];
             s = cp[g] - base;
             e = cp[g + 1] - base;
-            if (e - s < LONG_RUN) return true;
-            e = s;
-            return false;
+            return true;
         },
-        store);
+        [&](uint32_t r, const double (&acc)[4]) { store(short_ord[r], acc); });
 }
 
 // out[i] = sum over chunks c (in order) of partial[c * stride + i].
@@ -846,6 +887,10 @@ static void csr_finish(Csr &csr, const LayerMaps &maps) {
     build_columns<<<csr.n_chunks, 64 * waves, lds, st>>>(csr.indptr.ptr, csr.indices.ptr, csr.values.ptr, n,
                                                          csr.n_cols, csr.chunk_rows, csr.col_ptr.ptr,
                                                          csr.col_row.ptr, csr.col_val.ptr);
+    KY_HIP(hipGetLastError());
+    csr.col_order.alloc((size_t)csr.n_chunks * (csr.n_cols + 1));
+    order_runs<<<csr.n_chunks, 1024, sizeof(uint32_t) * (csr.chunk_rows + 1), st>>>(csr.col_ptr.ptr, csr.n_cols,
+                                                                                   csr.chunk_rows, csr.col_order.ptr);
     KY_HIP(hipGetLastError());
     KY_HIP(hipStreamSynchronize(st));
 }
@@ -1055,14 +1100,14 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
         // W's chunk rows in LDS, 16 columns a slice.
         const uint32_t n_slices = div_up(w, 16);
         column_product_lds<16><<<8 * div_up(csr.n_chunks, 8) * n_slices, 1024, 0, st>>>(
-            csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols, csr.n_chunks, csr.chunk_rows, csr.n_rows,
-            ctx.wbuf.ptr, w, ldw, n_slices, ctx.partials.ptr);
+            csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.col_order.ptr, csr.n_cols, csr.n_chunks,
+            csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, n_slices, ctx.partials.ptr);
     } else if (w <= 64 && ldw >= 64 && ldw % 4 == 0 && (size_t)csr.chunk_rows * 8 <= LDS_FLOATS) {
         // W's chunk rows in LDS, 8 columns a slice.
         const uint32_t n_slices = div_up(w, 8);
         column_product_lds<8><<<8 * div_up(csr.n_chunks, 8) * n_slices, 1024, 0, st>>>(
-            csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols, csr.n_chunks, csr.chunk_rows, csr.n_rows,
-            ctx.wbuf.ptr, w, ldw, n_slices, ctx.partials.ptr);
+            csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.col_order.ptr, csr.n_cols, csr.n_chunks,
+            csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, n_slices, ctx.partials.ptr);
     } else if (w <= 64)
         column_product<1><<<blocks, 256, 0, st>>>(csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols,
                                                   csr.n_chunks, csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, gpb,
@@ -1225,8 +1270,8 @@ size_t pca_device_bytes(uint64_t n, uint32_t g, uint64_t nnz, uint32_t k, uint32
     const size_t ld = padded_row((uint32_t)w1);
     const size_t cap = width * (q + 1);
     size_t total = 0;
-    // The layer: rows (indptr, indices, values) and chunked columns.
-    total += b(8 * (n + 1)) + 2 * b(4 * nnz) + b(4 * chunks * (g + 1)) + 2 * b(4 * nnz);
+    // The layer: rows (indptr, indices, values) and chunked columns with their run order.
+    total += b(8 * (n + 1)) + 2 * b(4 * nnz) + 2 * b(4 * chunks * (g + 1)) + 2 * b(4 * nnz);
     // Pass buffers: W (n rows of ld floats), the chunks' f64 partials and their
     // reduction, the column sums, the dense operand.
     total += b(4 * (n * ld + DENSE_PAD)) + b(8 * g * w1 * chunks * 2) + b(8 * (g * w1 + w1)) + b(8 * 4096 * w1);
