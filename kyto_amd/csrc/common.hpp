@@ -196,7 +196,7 @@ struct Csr {
     DevBuf<uint32_t> col_row;  // row within the chunk
     DevBuf<float> col_val;
     // For chunk c: its genes by the length of their runs, longest first, then the
-    // number of runs of at least 256 entries (col_order[c*(n_cols+1) + n_cols]).
+    // number of runs of at least 1,024 entries (col_order[c*(n_cols+1) + n_cols]).
     DevBuf<uint32_t> col_order;
 };
 
