@@ -230,6 +230,9 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
     __shared__ uint32_t bcnt[QB], trow[QB], qrow[QB], qskip[QB];
     __shared__ float qnorm[QB];
     __shared__ K tkey[QB];
+    // EXACT: a tile's points, staged by the block's coalesced loads (global_load_lds),
+    // each lane then reading its point's row from here.
+    __shared__ __attribute__((aligned(16))) float spts[EXACT ? TP * DP : 1];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
     const uint64_t i0 = (uint64_t)(blockIdx.x / n_seg) * QB;
     const unsigned seg = blockIdx.x % n_seg;
@@ -271,24 +274,44 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
         else sq[r * QS + c] = v;
     }
     __syncthreads();
+    // The block's queries are dealt to its waves in turn (query ql to wave ql % 4), so a
+    // short list of queries keeps every wave busy.
     unsigned nq = 0;  // this wave's live queries
     for (unsigned qi = 0; qi < 16; qi++)
-        if (qrow[wave * 16 + qi] != 0xFFFFFFFFu) nq = qi + 1;
+        if (qrow[qi * 4 + wave] != 0xFFFFFFFFu) nq = qi + 1;
+    // The points [t0, min(t0 + TP, p_end)) into spts, 64 floats a wave instruction (a
+    // lane past them reads the tile's first float; those slots belong to no point).
+    const auto stage_tile = [&](uint64_t t0) {
+        const uint32_t count = (uint32_t)(min<uint64_t>(TP, p_end - t0) * d);
+        const float *src = a + t0 * d;
+        for (uint32_t i = wave; i * 64 < count; i += 4) {
+            const uint32_t f = i * 64 + lane;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + (f < count ? f : 0u)),
+                                             (__attribute__((address_space(3))) void *)(spts + i * 64), 4, 0, 0);
+        }
+    };
+    if constexpr (EXACT)
+        if (p_begin < p_end) stage_tile(p_begin);
     for (uint64_t t0 = p_begin; t0 < p_end; t0 += TP) {
         const uint64_t p = t0 + lane;
         const bool valid = p < p_end;
         float pr[EXACT ? 1 : DP];
         double pd[EXACT ? DP : 1];  // EXACT: the point widened to f64, zero past d
         if constexpr (EXACT) {
-            // Unconditional loads (a lane past the segment reads its first point,
-            // components past d read component d - 1, then zeroed), so no load waits
-            // on a branch.
-            const uint64_t pp = valid ? p : p_begin;
+            // The tile's loads landed (every wave's), the rows read into registers
+            // (components past d read component d - 1, then zeroed; a lane past the
+            // segment reads slots of no point, its keys never kept), then the next
+            // tile's loads are issued into the same slots while this one is searched.
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            __syncthreads();
+            const float *row = spts + lane * d;
 #pragma unroll
             for (int c = 0; c < DP; c++) {
-                const float v = a[pp * d + min((unsigned)c, d - 1)];
+                const float v = row[min((unsigned)c, d - 1)];
                 pd[c] = c < (int)d ? (double)v : 0.0;
             }
+            __syncthreads();
+            if (t0 + TP < p_end) stage_tile(t0 + TP);
         } else if (valid) {
             const float4 *src = reinterpret_cast<const float4 *>(a + p * DP);
 #pragma unroll
@@ -306,7 +329,7 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
         const float po = (!EXACT && MODE == 0 && valid) ? offs[p] : 0.0f;
         const float pnv = (MODE == 2 && valid) ? pw.pn[p] : 0.0f;
         for (unsigned qi = 0; qi < nq; qi++) {
-            const unsigned ql = wave * 16 + qi;
+            const unsigned ql = qi * 4 + wave;
             const uint32_t qg = qrow[ql];
             K *bk = bkey + ql * CAP;
             uint32_t *br = brow + ql * CAP;
@@ -393,7 +416,7 @@ __global__ __launch_bounds__(256) void scan_points(const float *__restrict__ a, 
         }
     }
     for (unsigned qi = 0; qi < nq; qi++) {
-        const unsigned ql = wave * 16 + qi;
+        const unsigned ql = qi * 4 + wave;
         K *bk = bkey + ql * CAP;
         uint32_t *br = brow + ql * CAP;
         compact<K, CAP>(bk, br, &bcnt[ql], &tkey[ql], &trow[ql], keep);
