@@ -11,6 +11,7 @@
 #include "dense.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <numeric>
 #include <atomic>
@@ -533,53 +534,164 @@ static Mat keep_directions(const Mat &q, const Mat &r, double scale) {
     return mul_nn(q, ukeep);
 }
 
+// A barrier for the tasks of one pool run (spinning): every task of a run with no more
+// tasks than the pool's active threads is running at once, so all of them arrive.
+struct SpinBarrier {
+    alignas(64) std::atomic<unsigned> arrived{0};
+    alignas(64) std::atomic<unsigned> phase{0};
+    unsigned parties;
+    explicit SpinBarrier(unsigned n) : parties(n) {}
+    void wait() {
+        if (parties <= 1) return;
+        const unsigned ph = phase.load(std::memory_order_acquire);
+        if (arrived.fetch_add(1, std::memory_order_acq_rel) + 1 == parties) {
+            arrived.store(0, std::memory_order_relaxed);
+            phase.fetch_add(1, std::memory_order_release);
+        } else {
+            while (phase.load(std::memory_order_acquire) == ph) __builtin_ia32_pause();
+        }
+    }
+};
+
+// Householder reduction of the symmetric a (n x n, full row-major storage, updated in
+// place) to tridiagonal form: d the diagonal, e[j] the coupling of j and j + 1 (e[n -
+// 1] = 0), vs[j] / betas[j] the reflection of step j (beta 0: none). Step j: v = the
+// row below the diagonal with its first element moved to -sign(v0)|v| ... the usual
+// I - beta u u^T, p = beta A' v, K = beta/2 v.p, w = p - K v, A' -= v w^T + w v^T. The
+// update of step j and the product p of step j + 1 are one sweep over the trailing
+// rows (each row updated, then dotted with the next reflection's vector), cut over a
+// team of threads with one barrier a step: each thread forms the small vectors (w, the
+// next v) itself from the same values, so all agree to the bit, and every element is
+// computed as the one-pass-a-product form would (the same bits whatever the team).
+static void tridiagonalize(Mat &A, std::vector<double> &d, std::vector<double> &e,
+                           std::vector<std::vector<double>> &vs, std::vector<double> &betas) {
+    const size_t n = A.rows;
+    d.assign(n, 0.0);
+    e.assign(n, 0.0);
+    vs.assign(n, {});
+    betas.assign(n, 0.0);
+    if (n == 0) return;
+    d[0] = A.at(0, 0);
+    if (n <= 2) {
+        if (n == 2) {
+            d[1] = A.at(1, 1);
+            e[0] = A.at(1, 0);
+        }
+        return;
+    }
+    // Team: worth it while the trailing rows are many; the last steps run on one thread.
+    constexpr size_t TEAM_ROWS = 48;
+    Pool &pool = Pool::get();
+    const unsigned team = (unsigned)std::min<size_t>({(size_t)pool.active(), (size_t)8, n / TEAM_ROWS});
+    const unsigned T = std::max(1u, team);
+    SpinBarrier bar(T);
+    std::vector<double> pbuf[2] = {std::vector<double>(n), std::vector<double>(n)};
+    // v (m values, the row's part below the diagonal) into the reflection's vector; false
+    // when the row is zero (no reflection).
+    const auto form = [](const double *row, size_t m, std::vector<double> &v, double &beta, double &alpha) {
+        v.assign(row, row + m);
+        const double norm = std::sqrt(dot4(v.data(), v.data(), m));
+        if (norm == 0.0) {
+            beta = 0.0;
+            alpha = 0.0;
+            return false;
+        }
+        alpha = v[0] > 0 ? -norm : norm;
+        v[0] -= alpha;
+        beta = 2.0 / dot4(v.data(), v.data(), m);
+        return true;
+    };
+    const auto task = [&](size_t t) {
+        std::vector<double> v, vn, w(n), u(n);
+        double beta = 0.0, alpha = 0.0, betan = 0.0, alphan = 0.0;
+        unsigned live = T;  // the team's size at this step (1 once the trailing rows are few)
+        const auto range = [&](size_t lo, size_t &r0, size_t &r1) {
+            const size_t cnt = n - lo;
+            r0 = lo + cnt * t / live;
+            r1 = lo + cnt * (t + 1) / live;
+        };
+        bool have = form(A.row(0) + 1, n - 1, v, beta, alpha);
+        if (t == 0) {
+            vs[0] = v;
+            betas[0] = beta;
+            e[0] = have ? alpha : A.at(0, 1);
+        }
+        if (have) {
+            size_t r0, r1;
+            range(1, r0, r1);
+            for (size_t r = r0; r < r1; r++) pbuf[0][r - 1] = beta * dot4(A.row(r) + 1, v.data(), n - 1);
+        }
+        bar.wait();
+        for (size_t j = 0; j + 2 < n; j++) {
+            const size_t m = n - j - 1;  // trailing size of step j
+            if (live > 1 && m < TEAM_ROWS) {
+                // The rest on thread 0 (the barrier above saw every thread's last rows).
+                if (t != 0) return;
+                live = 1;
+            }
+            const double *pj = pbuf[j % 2].data();
+            double *pn = pbuf[(j + 1) % 2].data();
+            if (have) {
+                const double K = 0.5 * beta * dot4(v.data(), pj, m);
+                for (size_t i = 0; i < m; i++) w[i] = pj[i] - K * v[i];
+            }
+            // Row j + 1 (columns j + 1 ..) after this step's update.
+            const double *src = A.row(j + 1) + j + 1;
+            for (size_t c = 0; c < m; c++) u[c] = have ? src[c] - (v[0] * w[c] + w[0] * v[c]) : src[c];
+            const bool step_next = j + 3 < n;
+            const bool next = step_next && form(u.data() + 1, m - 1, vn, betan, alphan);
+            if (t == 0) {
+                d[j + 1] = u[0];
+                if (step_next) {
+                    vs[j + 1] = vn;
+                    betas[j + 1] = betan;
+                    e[j + 1] = next ? alphan : u[1];
+                } else {
+                    e[j + 1] = u[1];
+                }
+            }
+            // Rows j + 2 .. : columns j + 2 .. updated, then dotted with the next vector
+            // (four interleaved partial sums, as dot4).
+            size_t r0, r1;
+            range(j + 2, r0, r1);
+            for (size_t r = r0; r < r1; r++) {
+                double *ar = A.row(r) + j + 2;
+                const size_t mm = m - 1;
+                if (have) {
+                    const double vr = v[r - j - 1], wr = w[r - j - 1];
+                    const double *wc = w.data() + 1, *vc = v.data() + 1;
+                    for (size_t c = 0; c < mm; c++) ar[c] -= vr * wc[c] + wr * vc[c];
+                }
+                if (next) pn[r - j - 2] = betan * dot4(ar, vn.data(), mm);
+            }
+            if (live > 1) bar.wait();
+            std::swap(v, vn);
+            beta = betan;
+            alpha = alphan;
+            have = next;
+        }
+        if (t == 0) d[n - 1] = A.at(n - 1, n - 1);
+    };
+    if (T == 1)
+        task(0);
+    else
+        pool.run(T, task);
+    e[n - 1] = 0.0;
+}
+
 // Symmetric eigendecomposition a (n x n) = V diag(d) V^T: every eigenvalue (d
 // ascending) and the eigenvectors of the `want` largest (vout, n x want, largest
-// first). Householder reduction to tridiagonal form on full row-major storage (a
-// symmetric matrix-vector product and a rank-2 update per column, both along
-// contiguous rows), then the implicit QL method; only the wanted eigenvectors are
-// formed, from the recorded rotations and reflections.
+// first): the Householder tridiagonalisation above, then the implicit QL method; only
+// the wanted eigenvectors are formed, from the recorded rotations and reflections.
 static void symmetric_eigen(const Mat &a, size_t want, std::vector<double> &d, Mat &vout) {
     const size_t n = a.rows;
     d.assign(n, 0.0);
     std::vector<double> e(n, 0.0);
     if (n == 0) { vout = Mat(0, 0); return; }
     Mat A = a;
-    std::vector<std::vector<double>> vs(n);
-    std::vector<double> betas(n, 0.0);
-    std::vector<double> p(n), w(n);
-    for (size_t j = 0; j + 2 < n; j++) {
-        const size_t m = n - j - 1;  // trailing size
-        std::vector<double> &v = vs[j];
-        v.assign(A.row(j) + j + 1, A.row(j) + n);  // column j below the diagonal (= row j by symmetry)
-        const double norm = std::sqrt(dot4(v.data(), v.data(), m));
-        if (norm == 0.0) {
-            e[j] = 0.0;
-            continue;
-        }
-        const double alpha = v[0] > 0 ? -norm : norm;
-        v[0] -= alpha;
-        const double beta = 2.0 / dot4(v.data(), v.data(), m);
-        betas[j] = beta;
-        // p = beta A' v, K = beta/2 v.p, w = p - K v, A' -= v w^T + w v^T.
-        parallel_if(m, 2 * m, [&](size_t r0, size_t r1) {
-            for (size_t r = r0; r < r1; r++) p[r] = beta * dot4(A.row(j + 1 + r) + j + 1, v.data(), m);
-        }, 8);
-        const double K = 0.5 * beta * dot4(v.data(), p.data(), m);
-        for (size_t r = 0; r < m; r++) w[r] = p[r] - K * v[r];
-        parallel_if(m, 4 * m, [&](size_t r0, size_t r1) {
-            for (size_t r = r0; r < r1; r++) {
-                double *ar = A.row(j + 1 + r) + j + 1;
-                const double vr = v[r], wr = w[r];
-                for (size_t c = 0; c < m; c++) ar[c] -= vr * w[c] + wr * v[c];
-            }
-        });
-        A.at(j + 1, j) = alpha;
-        A.at(j, j + 1) = alpha;
-    }
-    for (size_t j = 0; j < n; j++) d[j] = A.at(j, j);
-    for (size_t j = 0; j + 1 < n; j++) e[j] = A.at(j + 1, j);
-    e[n - 1] = 0.0;
+    std::vector<std::vector<double>> vs;
+    std::vector<double> betas;
+    tridiagonalize(A, d, e, vs, betas);
     // Implicit QL on the tridiagonal matrix (e[i] couples i and i + 1), its rotations
     // recorded: the eigenvector matrix is Q R_1 ... R_N (Q the reflections above, R_t
     // the t-th rotation acting on columns i, i + 1).
@@ -642,46 +754,62 @@ static void symmetric_eigen(const Mat &a, size_t want, std::vector<double> &d, M
     // The eigenvalues ascending (ties in index order), then the `want` largest's
     // vectors, largest first: column j is Q R_1 ... R_N e_{s_j}, the rotations applied
     // to the unit vectors from the last back (each a pair of rows of the n x want
-    // block), then the reflections from the first back.
+    // block), then the reflections from the first back. Every column is its own
+    // computation: the block is cut into one slice a thread (whole 64-byte lines of each
+    // row: a multiple of 8 columns), the slices run on the pool.
     std::vector<size_t> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return d[x] < d[y]; });
     std::vector<double> ds(n);
     for (size_t k = 0; k < n; k++) ds[k] = d[order[k]];
     want = std::min(want, n);
-    std::vector<double> Y(n * want, 0.0);  // row-major n x want
-    for (size_t j = 0; j < want; j++) Y[order[n - 1 - j] * want + j] = 1.0;
-    for (size_t t = rotations.size(); t-- > 0;) {
-        const Rotation &rot = rotations[t];
-        double *__restrict y0 = &Y[rot.i * want];
-        double *__restrict y1 = &Y[(rot.i + 1) * want];
-        const double c = rot.c, sn = rot.s;
-        for (size_t j = 0; j < want; j++) {
-            const double a0 = y0[j], a1 = y1[j];
-            y0[j] = c * a0 + sn * a1;
-            y1[j] = c * a1 - sn * a0;
+    const size_t lines = (want + 7) / 8;  // 8-column lines a row
+    const size_t threads = n >= 64 ? std::min<size_t>(Pool::get().active(), lines) : 1;
+    const size_t sw = (lines + threads - 1) / threads * 8;  // slice width
+    const size_t slices = (lines * 8 + sw - 1) / sw;
+    const size_t ys = slices * sw;  // row stride
+    std::vector<double> store(n * ys + 8, 0.0);
+    double *Y = store.data();
+    while (reinterpret_cast<uintptr_t>(Y) % 64 != 0) Y++;  // rows on 64-byte lines
+    for (size_t j = 0; j < want; j++) Y[order[n - 1 - j] * ys + j] = 1.0;
+    const auto slice = [&](size_t sl) {
+        const size_t c0 = sw * sl;
+        for (size_t t = rotations.size(); t-- > 0;) {
+            const Rotation &rot = rotations[t];
+            double *__restrict y0 = Y + rot.i * ys + c0;
+            double *__restrict y1 = Y + (rot.i + 1) * ys + c0;
+            const double c = rot.c, sn = rot.s;
+            for (size_t j = 0; j < sw; j++) {
+                const double a0 = y0[j], a1 = y1[j];
+                y0[j] = c * a0 + sn * a1;
+                y1[j] = c * a1 - sn * a0;
+            }
         }
-    }
-    std::vector<double> acc(want);
-    for (size_t jj = n >= 2 ? n - 2 : 0; jj-- > 0;) {
-        if (betas[jj] == 0.0) continue;
-        const std::vector<double> &v = vs[jj];
-        const size_t m = n - jj - 1;
-        std::fill(acc.begin(), acc.end(), 0.0);
-        for (size_t r = 0; r < m; r++) {
-            const double vr = v[r];
-            const double *y = &Y[(jj + 1 + r) * want];
-            for (size_t j = 0; j < want; j++) acc[j] += vr * y[j];
+        std::vector<double> acc(sw);
+        for (size_t jj = n >= 2 ? n - 2 : 0; jj-- > 0;) {
+            if (betas[jj] == 0.0) continue;
+            const std::vector<double> &v = vs[jj];
+            const size_t m = n - jj - 1;
+            std::fill(acc.begin(), acc.end(), 0.0);
+            for (size_t r = 0; r < m; r++) {
+                const double vr = v[r];
+                const double *y = Y + (jj + 1 + r) * ys + c0;
+                for (size_t j = 0; j < sw; j++) acc[j] += vr * y[j];
+            }
+            for (size_t j = 0; j < sw; j++) acc[j] *= betas[jj];
+            for (size_t r = 0; r < m; r++) {
+                const double vr = v[r];
+                double *y = Y + (jj + 1 + r) * ys + c0;
+                for (size_t j = 0; j < sw; j++) y[j] -= acc[j] * vr;
+            }
         }
-        for (size_t j = 0; j < want; j++) acc[j] *= betas[jj];
-        for (size_t r = 0; r < m; r++) {
-            const double vr = v[r];
-            double *y = &Y[(jj + 1 + r) * want];
-            for (size_t j = 0; j < want; j++) y[j] -= acc[j] * vr;
-        }
-    }
+    };
+    if (slices > 1)
+        Pool::get().run(slices, slice);
+    else
+        for (size_t sl = 0; sl < slices; sl++) slice(sl);
     vout = Mat(n, want);
-    std::copy(Y.begin(), Y.end(), vout.v.begin());
+    for (size_t r = 0; r < n; r++) std::copy(Y + r * ys, Y + r * ys + want, vout.row(r));
     d = ds;
 }
 
