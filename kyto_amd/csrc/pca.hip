@@ -672,7 +672,8 @@ __global__ __launch_bounds__(1024) void row_project_lds(const uint64_t *__restri
 // chunk's runs are taken longest first (order, from order_runs): the wavefronts first
 // sum the long runs one at a time, whole, then sum the short runs as lane groups, so
 // the runs left when the block's groups run out of work are its shortest. partial as
-// column_product's.
+// column_product's. With col_sums, the slice's columns of W summed over the chunk's
+// rows too (f64, in a fixed order), into col_sums[chunk * w + column]. 1024 threads.
 template <int SW>
 __global__ __launch_bounds__(1024) void column_product_lds(const uint32_t *__restrict__ col_ptr,
                                                            const uint32_t *__restrict__ col_row,
@@ -680,10 +681,12 @@ __global__ __launch_bounds__(1024) void column_product_lds(const uint32_t *__res
                                                            const uint32_t *__restrict__ order, uint32_t n_cols,
                                                            uint32_t n_chunks, uint32_t chunk_rows, uint64_t n_rows,
                                                            const float *__restrict__ W, uint32_t w, uint32_t ldw,
-                                                           uint32_t n_slices, double *__restrict__ partial) {
+                                                           uint32_t n_slices, double *__restrict__ partial,
+                                                           double *__restrict__ col_sums) {
     using S = SliceShape<SW>;
     __shared__ __attribute__((aligned(16))) float ws[LDS_FLOATS];
     __shared__ uint32_t counter[2];  // next long run, next short run
+    __shared__ double wave_sums[16][SW];
     unsigned chunk, slice;
     slice_block(n_slices, chunk, slice);
     if (chunk >= n_chunks) return;
@@ -699,6 +702,25 @@ __global__ __launch_bounds__(1024) void column_product_lds(const uint32_t *__res
         reinterpret_cast<float4 *>(ws)[i] =
             *reinterpret_cast<const float4 *>(W + (r0 + i / (SW / 4)) * ldw + c0 + 4 * (i % (SW / 4)));
     __syncthreads();
+    if (col_sums) {
+        // Wavefront v sums rows [v R, (v + 1) R) of the slice (R = rows / 16 rounded up),
+        // lane l column l % SW of every (64 / SW)-th row from l / SW (each read a
+        // contiguous 256 bytes of the slice), the lanes of a column then added in a
+        // butterfly; the 16 wavefronts' sums in order by wavefront 0.
+        const unsigned v = wave_in_block(), l = lane_id();
+        const uint32_t R = (rows + 15) / 16, rend = min(rows, (v + 1) * R);
+        double t = 0.0;
+        for (uint32_t r = v * R + l / SW; r < rend; r += 64 / SW) t += (double)ws[r * SW + l % SW];
+#pragma unroll
+        for (int o = SW; o < 64; o <<= 1) t += __shfl_xor(t, o);
+        if (l < SW) wave_sums[v][l] = t;
+        __syncthreads();
+        if (v == 0 && l < SW && c0 + l < w) {
+            double total = 0.0;
+            for (int u = 0; u < 16; u++) total += wave_sums[u][l];
+            col_sums[(size_t)chunk * w + c0 + l] = total;
+        }
+    }
     const uint32_t *cp = col_ptr + (size_t)chunk * (n_cols + 1);
     const uint32_t *ord = order + (size_t)chunk * (n_cols + 1);
     const uint32_t n_long = ord[n_cols];
@@ -1094,22 +1116,31 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
     const unsigned groups = div_up(csr.n_cols, gpb);
     const unsigned slots = div_up(csr.n_chunks, 8);
     const unsigned blocks = 8 * slots * groups;
+    const bool lds16 = w <= 64 && ldw >= 64 && ldw % 4 == 0 && (size_t)csr.chunk_rows * 16 <= LDS_FLOATS;
+    const bool lds8 = !lds16 && w <= 64 && ldw >= 64 && ldw % 4 == 0 && (size_t)csr.chunk_rows * 8 <= LDS_FLOATS;
+    // The column sums of W: per chunk by the LDS-slice product (from its slice), else
+    // per segment of SEG rows by column_sums; then the tree's levels (each at most 1/64
+    // of the one before), then the total.
+    const uint32_t sum_parts = lds16 || lds8 ? csr.n_chunks : div_up(csr.n_rows, SEG);
+    const size_t sums_need = (size_t)sum_parts * w * 2 + 2 * w;
+    if (sums) ctx.sums.reserve(sums_need);
+    double *chunk_sums = sums && (lds16 || lds8) ? ctx.sums.ptr : nullptr;
     if (t_clock)
         t_clock->begin(1, st,
                        8.0 * csr.nnz + 4.0 * csr.n_chunks * (csr.n_cols + 1.0) + 4.0 * csr.n_rows * w +
                            8.0 * csr.n_chunks * csr.n_cols * w);
-    if (w <= 64 && ldw >= 64 && ldw % 4 == 0 && (size_t)csr.chunk_rows * 16 <= LDS_FLOATS) {
+    if (lds16) {
         // W's chunk rows in LDS, 16 columns a slice.
         const uint32_t n_slices = div_up(w, 16);
         column_product_lds<16><<<8 * div_up(csr.n_chunks, 8) * n_slices, 1024, 0, st>>>(
             csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.col_order.ptr, csr.n_cols, csr.n_chunks,
-            csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, n_slices, ctx.partials.ptr);
-    } else if (w <= 64 && ldw >= 64 && ldw % 4 == 0 && (size_t)csr.chunk_rows * 8 <= LDS_FLOATS) {
+            csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, n_slices, ctx.partials.ptr, chunk_sums);
+    } else if (lds8) {
         // W's chunk rows in LDS, 8 columns a slice.
         const uint32_t n_slices = div_up(w, 8);
         column_product_lds<8><<<8 * div_up(csr.n_chunks, 8) * n_slices, 1024, 0, st>>>(
             csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.col_order.ptr, csr.n_cols, csr.n_chunks,
-            csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, n_slices, ctx.partials.ptr);
+            csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, n_slices, ctx.partials.ptr, chunk_sums);
     } else if (w <= 64)
         column_product<1><<<blocks, 256, 0, st>>>(csr.col_ptr.ptr, csr.col_row.ptr, csr.col_val.ptr, csr.n_cols,
                                                   csr.n_chunks, csr.chunk_rows, csr.n_rows, ctx.wbuf.ptr, w, ldw, gpb,
@@ -1128,14 +1159,12 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
                 ctx.reduced.ptr);
     KY_HIP(hipMemcpyAsync(product, ctx.reduced.ptr, sizeof(double) * stride, hipMemcpyDeviceToHost, st));
     if (sums) {
-        const uint32_t n_seg = div_up(csr.n_rows, SEG);
-        // segment sums, then the tree's levels (each at most 1/64 of the one before), then the total
-        const size_t need = (size_t)n_seg * w * 2 + 2 * w;
-        ctx.sums.reserve(need);
-        column_sums<<<n_seg, 64 * div_up(w, 64), 0, st>>>(ctx.wbuf.ptr, csr.n_rows, w, ldw, ctx.sums.ptr);
-        KY_HIP(hipGetLastError());
-        double *tot = ctx.sums.ptr + need - w;
-        ordered_sum(st, ctx.sums.ptr, n_seg, w, ctx.sums.ptr + (size_t)n_seg * w, tot);
+        if (!chunk_sums) {
+            column_sums<<<sum_parts, 64 * div_up(w, 64), 0, st>>>(ctx.wbuf.ptr, csr.n_rows, w, ldw, ctx.sums.ptr);
+            KY_HIP(hipGetLastError());
+        }
+        double *tot = ctx.sums.ptr + sums_need - w;
+        ordered_sum(st, ctx.sums.ptr, sum_parts, w, ctx.sums.ptr + (size_t)sum_parts * w, tot);
         KY_HIP(hipGetLastError());
         KY_HIP(hipMemcpyAsync(sums, tot, sizeof(double) * w, hipMemcpyDeviceToHost, st));
     }
