@@ -451,7 +451,7 @@ static bool cholesky_qr2(const Mat &a, Mat &q, Mat &r) {
 }
 
 // The block's scale (its Frobenius norm), 0 for a block that adds nothing.
-static double block_scale(const Mat &block) {
+double block_scale_of(const Mat &block) {
     const double scale = block.cols == 0 ? 0.0 : frobenius(block);
     if (!std::isfinite(scale))
         throw Error(KY_E215, "pca: orthonormalising a Krylov block did not converge (a value that is not finite)");
@@ -467,7 +467,7 @@ Mat extend_basis(const Mat &basis, const Mat &block, DeviceBasis *dev) {
         KY_CHECK(dev->cols == basis.cols, KY_E505, "pca: the device basis is out of step (internal)");
         return extend_basis(*dev, block);
     }
-    const double scale = block_scale(block);
+    const double scale = block_scale_of(block);
     if (scale == 0.0) return Mat(block.rows, 0);
     Mat rest = block;
     if (basis.cols > 0) {
@@ -481,7 +481,7 @@ Mat extend_basis(const Mat &basis, const Mat &block, DeviceBasis *dev) {
 }
 
 Mat extend_basis(DeviceBasis &dev, const Mat &block) {
-    const double scale = block_scale(block);
+    const double scale = block_scale_of(block);
     if (scale == 0.0) return Mat(block.rows, 0);
     KY_CHECK(block.rows >= block.cols, KY_E505, "pca: a Krylov block wider than the genes (internal)");
     if (block.cols > 64) {
@@ -499,6 +499,32 @@ Mat extend_basis(DeviceBasis &dev, const Mat &block) {
         return new_directions(rest, scale);
     }
     return keep_directions(q, r, scale);
+}
+
+size_t extend_basis_resident(DeviceBasis &dev, size_t width, double scale) {
+    if (!std::isfinite(scale))
+        throw Error(KY_E215, "pca: orthonormalising a Krylov block did not converge (a value that is not finite)");
+    if (scale == 0.0 || width == 0) return 0;
+    KY_CHECK(dev.rows >= width, KY_E505, "pca: a Krylov block wider than the genes (internal)");
+    if (width > 64) {
+        Mat rest = dev.get_block(width);
+        if (dev.cols > 0) dev.project_out(rest);
+        const Mat q = new_directions(rest, scale);
+        dev.put_block(q);
+        return q.cols;
+    }
+    dev.project_resident(width);
+    Mat q, r;
+    if (!dev.cholesky_qr2(width, q, r, false)) {
+        const Mat out = new_directions(q, scale);  // q: the projected block
+        dev.put_block(out);
+        return out.cols;
+    }
+    // Full rank keeps Q itself, where it is (keep_directions' first test).
+    if (min_singular_bound(r) > DROP_TOLERANCE * scale) return width;
+    const Mat out = keep_directions(dev.get_block(width), r, scale);
+    dev.put_block(out);
+    return out.cols;
 }
 
 static Mat new_directions(const Mat &rest, double scale) {

@@ -25,6 +25,8 @@ struct Mat {
 };
 
 Mat mul_tn(const Mat &a, const Mat &b);
+// The Frobenius norm of a Krylov block (KY-E215 when it is not finite).
+double block_scale_of(const Mat &block);
 // g = r^T r (r upper triangular); false when a pivot is not positive or not finite.
 bool cholesky_factor(const Mat &g, Mat &r);
 // |r|_F |r^-1|_F, an upper bound of the 2-norm condition number of upper triangular r.
@@ -45,11 +47,22 @@ struct DeviceBasis {
     // The block on the device (work_r) with the held columns projected out, CGS2 as
     // project_out; it stays there for cholesky_qr2.
     void project_out_resident(const Mat &block);
-    // Cholesky QR twice of the resident block (rows x w, w <= 64): q into `q` (host) and
-    // the triangular factor into `r`; false when the first factor's condition bound is
-    // above 1e6 or a pivot is not positive (then the host's Householder QR takes the
-    // block, downloaded into `q`).
-    bool cholesky_qr2(size_t w, Mat &q, Mat &r);
+    // CGS2 of the resident block (work_r, rows x width) against the held columns.
+    void project_resident(size_t width);
+    // Cholesky QR twice of the resident block (rows x w, w <= 64), in place: the
+    // triangular factor into `r` and, with fetch_q, q into `q` (host); false when the
+    // first factor's condition bound is above 1e6 or a pivot is not positive (then the
+    // host's Householder QR takes the block, always downloaded into `q`).
+    bool cholesky_qr2(size_t w, Mat &q, Mat &r, bool fetch_q = true);
+    // The resident block (work_r, rows x width, contiguous): copied from device memory,
+    // uploaded from the host, downloaded, or its Frobenius norm.
+    void take_block(const double *src, size_t width);
+    void put_block(const Mat &block);
+    Mat get_block(size_t width);
+    double block_norm(size_t width);
+    // Columns [cols, cols + width) of B and C from device blocks (rows x width each,
+    // contiguous), queued on the stream.
+    void append_resident(const double *block, const double *product, size_t width);
     // B^T C (cols x cols).
     Mat rayleigh_matrix();
     // B S (rows x S.cols).
@@ -61,6 +74,11 @@ struct DeviceBasis {
 Mat extend_basis(const Mat &basis, const Mat &block, DeviceBasis *dev = nullptr);
 // The same over the columns `dev` holds (no host copy of the basis).
 Mat extend_basis(DeviceBasis &dev, const Mat &block);
+// The same for the block resident in dev.work_r (rows x width, Frobenius norm
+// `scale`): the new directions are left there, contiguous; returns their number.
+// They come back to the host only when the block needs the host's factorisations (more
+// than 64 columns, an ill-conditioned block or a rank-deficient one).
+size_t extend_basis_resident(DeviceBasis &dev, size_t width, double scale);
 
 // pca_dense.rs:141-174. With `dev`, B^T C and B U run on the device.
 void rayleigh_ritz(const Mat &basis, const Mat &products, size_t k, std::vector<double> &eigenvalues,

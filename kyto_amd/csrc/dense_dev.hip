@@ -74,6 +74,44 @@ __global__ __launch_bounds__(256) void solve_rows_upper(double *__restrict__ a, 
     if (j < n) a[row * n + j] = x;
 }
 
+// The larger of a and m, NaN once either is NaN.
+__device__ __forceinline__ double max_nan(double m, double a) { return (a > m || a != a) ? a : m; }
+
+// out[0] = max |x| and out[1] = sum of (x / max)^2 over count values (one block of
+// 256 threads: strided runs in order, then a fixed tree; 0 and 0 for an all-zero x; a
+// value that is not finite makes the norm out[0] sqrt(out[1]) not finite).
+__global__ __launch_bounds__(256) void scaled_squares(const double *__restrict__ x, uint64_t count,
+                                                      double *__restrict__ out) {
+    __shared__ double part[256];
+    const unsigned t = threadIdx.x;
+    double m = 0.0;
+    for (uint64_t i = t; i < count; i += 256) m = max_nan(m, fabs(x[i]));
+    part[t] = m;
+    __syncthreads();
+    for (unsigned o = 128; o > 0; o >>= 1) {
+        if (t < o) part[t] = max_nan(part[t], part[t + o]);
+        __syncthreads();
+    }
+    const double big = part[0];
+    __syncthreads();
+    double s = 0.0;
+    if (big > 0.0)
+        for (uint64_t i = t; i < count; i += 256) {
+            const double a = x[i] / big;
+            s += a * a;
+        }
+    part[t] = s;
+    __syncthreads();
+    for (unsigned o = 128; o > 0; o >>= 1) {
+        if (t < o) part[t] += part[t + o];
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = big;
+        out[1] = part[0];
+    }
+}
+
 }  // namespace
 
 DeviceBasis::DeviceBasis(Context &c, size_t rows_, size_t cap_) : ctx(&c), rows(rows_), cap(cap_) {
@@ -134,8 +172,12 @@ void DeviceBasis::project_out(Mat &rest) {
 }
 
 void DeviceBasis::project_out_resident(const Mat &rest) {
-    const uint32_t w = (uint32_t)rest.cols, bc = (uint32_t)cols;
     upload_mat(*ctx, work_r, rest);
+    project_resident(rest.cols);
+}
+
+void DeviceBasis::project_resident(size_t width) {
+    const uint32_t w = (uint32_t)width, bc = (uint32_t)cols;
     if (cols == 0 || w == 0) return;
     work_p.reserve((size_t)bc * w);
     for (int pass = 0; pass < 2; pass++) {
@@ -146,7 +188,44 @@ void DeviceBasis::project_out_resident(const Mat &rest) {
     }
 }
 
-bool DeviceBasis::cholesky_qr2(size_t w, Mat &q, Mat &r) {
+void DeviceBasis::take_block(const double *src, size_t width) {
+    work_r.reserve(std::max<size_t>(rows * width, 1));
+    if (width)
+        KY_HIP(hipMemcpyAsync(work_r.ptr, src, rows * width * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+void DeviceBasis::put_block(const Mat &block) {
+    work_r.reserve(std::max<size_t>(block.v.size(), 1));
+    if (!block.v.empty()) upload_parts(*ctx, {CopyPart{work_r.ptr, block.v.data(), block.v.size() * sizeof(double)}});
+}
+
+Mat DeviceBasis::get_block(size_t width) {
+    Mat out(rows, width);
+    if (width) download(*ctx, out.v.data(), work_r.ptr, rows * width * sizeof(double));
+    return out;
+}
+
+double DeviceBasis::block_norm(size_t width) {
+    if (width == 0) return 0.0;
+    work_p.reserve(2);
+    scaled_squares<<<1, 256, 0, ctx->stream>>>(work_r.ptr, rows * width, work_p.ptr);
+    KY_HIP(hipGetLastError());
+    double h[2];
+    download(*ctx, h, work_p.ptr, sizeof(h));
+    return h[0] * std::sqrt(h[1]);
+}
+
+void DeviceBasis::append_resident(const double *block, const double *product, size_t width) {
+    KY_CHECK(cols + width <= cap, KY_E505, "pca: a Krylov block that does not fit the device basis (internal)");
+    if (width == 0) return;
+    KY_HIP(hipMemcpy2DAsync(basis.ptr + cols, cap * sizeof(double), block, width * sizeof(double),
+                            width * sizeof(double), rows, hipMemcpyDeviceToDevice, ctx->stream));
+    KY_HIP(hipMemcpy2DAsync(products.ptr + cols, cap * sizeof(double), product, width * sizeof(double),
+                            width * sizeof(double), rows, hipMemcpyDeviceToDevice, ctx->stream));
+    cols += width;
+}
+
+bool DeviceBasis::cholesky_qr2(size_t w, Mat &q, Mat &r, bool fetch_q) {
     KY_CHECK(w >= 1 && w <= 64, KY_E505, "pca: a device Cholesky QR of more than 64 columns (internal)");
     const uint32_t n = (uint32_t)w;
     q = Mat(rows, w);
@@ -165,7 +244,7 @@ bool DeviceBasis::cholesky_qr2(size_t w, Mat &q, Mat &r) {
         solve_rows_upper<<<div_up(rows, 4), 256, 0, ctx->stream>>>(work_r.ptr, rows, n, work_p.ptr);
         KY_HIP(hipGetLastError());
     }
-    download(*ctx, q.v.data(), work_r.ptr, rows * w * sizeof(double));
+    if (fetch_q) download(*ctx, q.v.data(), work_r.ptr, rows * w * sizeof(double));
     // r = r2 r1 (both upper triangular).
     r = Mat(w, w);
     for (size_t i = 0; i < w; i++)

@@ -1102,9 +1102,52 @@ void ordered_sum(hipStream_t st, const double *partial, uint32_t n_parts, uint64
     KY_HIP(hipGetLastError());
 }
 
+// The next pass's operand from a resident f64 block q (genes x w, contiguous): its f32
+// copy with rows ld floats apart (zero past w), and the rounded means bm[j] = sum over
+// genes g in order of means[g] * f64(f32(q[g][j])) (one thread a column).
+__global__ void block_f32(const double *__restrict__ q, uint32_t genes, uint32_t w, uint32_t ld,
+                          float *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)genes * ld) return;
+    const uint32_t g = (uint32_t)(i / ld), j = (uint32_t)(i % ld);
+    out[i] = j < w ? (float)q[(uint64_t)g * w + j] : 0.0f;
+}
+
+__global__ void rounded_means(const double *__restrict__ q, const double *__restrict__ means, uint32_t genes,
+                              uint32_t w, double *__restrict__ bm) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= w) return;
+    constexpr uint32_t B = 16;  // loads issued together, then added in order
+    double s = 0.0;
+    uint32_t g = 0;
+    for (; g + B <= genes; g += B) {
+        double t[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) t[u] = means[g + u] * (double)(float)q[(uint64_t)(g + u) * w + j];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u++) s += t[u];
+    }
+    for (; g < genes; g++) s += means[g] * (double)(float)q[(uint64_t)g * w + j];
+    bm[j] = s;
+}
+
+// product[g][j] -= means[g] * sums[j] (the centring of X^T W).
+__global__ void center_product(double *__restrict__ product, uint32_t genes, uint32_t w,
+                               const double *__restrict__ means, const double *__restrict__ sums) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)genes * w) return;
+    product[i] -= means[i / w] * sums[i % w];
+}
+
 // W (n x w floats, row stride ldw) is in ctx.wbuf. Computes X^T W into `product`
-// (host, genes x w) and optionally the column sums of W into `sums` (host, w).
-static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *product, double *sums) {
+// (host, genes x w) and optionally the column sums of W into `sums` (host, w). With
+// `resident`, both stay on the device (product in ctx.reduced, the sums at
+// resident->sums) and nothing is downloaded.
+struct ProductOnDevice {
+    double *product = nullptr, *sums = nullptr;
+};
+static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *product, double *sums,
+                               ProductOnDevice *resident = nullptr) {
     Context &ctx = *csr.ctx;
     hipStream_t st = ctx.stream;
     const size_t stride = (size_t)csr.n_cols * w;
@@ -1123,8 +1166,9 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
     // of the one before), then the total.
     const uint32_t sum_parts = lds16 || lds8 ? csr.n_chunks : div_up(csr.n_rows, SEG);
     const size_t sums_need = (size_t)sum_parts * w * 2 + 2 * w;
-    if (sums) ctx.sums.reserve(sums_need);
-    double *chunk_sums = sums && (lds16 || lds8) ? ctx.sums.ptr : nullptr;
+    const bool want_sums = sums || resident;
+    if (want_sums) ctx.sums.reserve(sums_need);
+    double *chunk_sums = want_sums && (lds16 || lds8) ? ctx.sums.ptr : nullptr;
     if (t_clock)
         t_clock->begin(1, st,
                        8.0 * csr.nnz + 4.0 * csr.n_chunks * (csr.n_cols + 1.0) + 4.0 * csr.n_rows * w +
@@ -1157,8 +1201,8 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
     KY_HIP(hipGetLastError());
     ordered_sum(st, ctx.partials.ptr, csr.n_chunks, stride, ctx.partials.ptr + stride * csr.n_chunks,
                 ctx.reduced.ptr);
-    KY_HIP(hipMemcpyAsync(product, ctx.reduced.ptr, sizeof(double) * stride, hipMemcpyDeviceToHost, st));
-    if (sums) {
+    if (product) KY_HIP(hipMemcpyAsync(product, ctx.reduced.ptr, sizeof(double) * stride, hipMemcpyDeviceToHost, st));
+    if (want_sums) {
         if (!chunk_sums) {
             column_sums<<<sum_parts, 64 * div_up(w, 64), 0, st>>>(ctx.wbuf.ptr, csr.n_rows, w, ldw, ctx.sums.ptr);
             KY_HIP(hipGetLastError());
@@ -1166,7 +1210,12 @@ static void transposed_product(Csr &csr, uint32_t w, uint32_t ldw, double *produ
         double *tot = ctx.sums.ptr + sums_need - w;
         ordered_sum(st, ctx.sums.ptr, sum_parts, w, ctx.sums.ptr + (size_t)sum_parts * w, tot);
         KY_HIP(hipGetLastError());
-        KY_HIP(hipMemcpyAsync(sums, tot, sizeof(double) * w, hipMemcpyDeviceToHost, st));
+        if (sums) KY_HIP(hipMemcpyAsync(sums, tot, sizeof(double) * w, hipMemcpyDeviceToHost, st));
+        if (resident) resident->sums = tot;
+    }
+    if (resident) {
+        resident->product = ctx.reduced.ptr;
+        return;
     }
     KY_HIP(hipStreamSynchronize(st));
 }
@@ -1241,6 +1290,30 @@ void pca_product(Csr &csr, const double *q, uint32_t w, const double *block_mean
     ctx.wbuf.reserve((size_t)csr.n_rows * ld + DENSE_PAD);
     launch_row_project(csr, ctx.dense_f32.ptr, w, ld, ctx.dense_f64.ptr, ctx.wbuf.ptr, ld);
     transposed_product(csr, w, ld, product, sums);
+}
+
+// pca_product over a resident block q (genes x w, f64, contiguous on the device) with
+// the means on the device: the centred product (X - mu)^T (X Q32 - mu^T Q32) stays in
+// ctx.reduced (genes x w); returns it. Queued on the stream, nothing synchronised.
+static double *pca_product_resident(Csr &csr, const double *q, uint32_t w, const double *means) {
+    Context &ctx = *csr.ctx;
+    KY_CHECK(w >= 1 && w <= 256, KY_E215, "pca: block width out of range");
+    hipStream_t st = ctx.stream;
+    const uint32_t ld = padded_row(w);
+    const uint64_t cells = (uint64_t)csr.n_cols * ld;
+    ctx.dense_f32.reserve(cells + DENSE_PAD);
+    ctx.dense_f64.reserve(w);
+    block_f32<<<div_up(cells, 256), 256, 0, st>>>(q, csr.n_cols, w, ld, ctx.dense_f32.ptr);
+    rounded_means<<<div_up(w, 64), 64, 0, st>>>(q, means, csr.n_cols, w, ctx.dense_f64.ptr);
+    KY_HIP(hipGetLastError());
+    ctx.wbuf.reserve((size_t)csr.n_rows * ld + DENSE_PAD);
+    launch_row_project(csr, ctx.dense_f32.ptr, w, ld, ctx.dense_f64.ptr, ctx.wbuf.ptr, ld);
+    ProductOnDevice out;
+    transposed_product(csr, w, ld, nullptr, nullptr, &out);
+    center_product<<<div_up((uint64_t)csr.n_cols * w, 256), 256, 0, st>>>(out.product, csr.n_cols, w, means,
+                                                                           out.sums);
+    KY_HIP(hipGetLastError());
+    return out.product;
 }
 
 void pca_project(Csr &csr, const double *v, uint32_t k, const double *mean_projection, const double *means,
@@ -1368,9 +1441,14 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
     for (size_t g = 0; g < genes; g++)
         for (size_t j = 0; j < width; j++) sketch.at(g, j) -= means[g] * omega_sums[j];
     // The basis and its products resident on the device only, where the dense steps'
-    // large products run.
+    // large products run; so is each Krylov block (dev_basis.work_r) from its
+    // orthonormalisation to the next pass, and the pass's product.
     DeviceBasis dev_basis(*csr.ctx, genes, width * (q + 1));
-    Mat block = extend_basis(dev_basis, sketch);
+    dev_basis.work_r.reserve(genes * width);  // every block fits: no reallocation under queued copies
+    DevBuf<double> means_dev(genes);
+    upload_parts(run, {CopyPart{means_dev.ptr, means.data(), sizeof(double) * genes}});
+    dev_basis.put_block(sketch);
+    size_t block_cols = extend_basis_resident(dev_basis, sketch.cols, block_scale_of(sketch));
     std::vector<double> eigenvalues;
     Mat loadings;
     bool finished = false;
@@ -1388,29 +1466,26 @@ void pca_run(Csr &csr, uint32_t n_components, uint64_t seed, uint32_t q, float *
         rayleigh_ritz(dev_basis, k, eigenvalues, loadings);
         finished = true;
     };
-    if (block.cols == 0) finish_basis();
+    if (block_cols == 0) finish_basis();
     dense_ms += now_ms() - td;
     uint32_t passes = 0;
     for (uint32_t p = 1; p <= q && !finished; p++) {
-        const size_t w = block.cols;
-        std::vector<double> bm = rounded_means_of(block);
-        Mat product(genes, w);
-        std::vector<double> wsum(w);
+        const size_t w = block_cols;
         t0 = now_ms();
         run.check_cancel();
-        pca_product(csr, block.v.data(), (uint32_t)w, bm.data(), product.v.data(), wsum.data());
+        const double *product = pca_product_resident(csr, dev_basis.work_r.ptr, (uint32_t)w, means_dev.ptr);
+        KY_HIP(hipStreamSynchronize(run.stream));
         run.advance(n);
         pass_time(t0);
         passes++;
         td = now_ms();
-        for (size_t g = 0; g < genes; g++)
-            for (size_t j = 0; j < w; j++) product.at(g, j) -= means[g] * wsum[j];
-        dev_basis.append(block, product);
+        dev_basis.append_resident(dev_basis.work_r.ptr, product, w);
         if (p == q) {
             finish_basis();
         } else {
-            block = extend_basis(dev_basis, product);
-            if (block.cols == 0) finish_basis();
+            dev_basis.take_block(product, w);
+            block_cols = extend_basis_resident(dev_basis, w, dev_basis.block_norm(w));
+            if (block_cols == 0) finish_basis();
         }
         dense_ms += now_ms() - td;
     }
