@@ -930,6 +930,88 @@ __global__ __launch_bounds__(256) void merge_sym(const uint32_t *__restrict__ pe
         if (lane == 0) count[r] = 0;
         return;
     }
+    if (m <= 64) {
+        // In registers: slot 0 the row's own candidates (at most m), slots 1 .. the
+        // listed keys at or below T (a free slot holds (~0, ~0)). The m smallest (key,
+        // row) pairs are those at or below the m-th, found bit by bit (its key, then,
+        // among the pairs of that key, its row) by counting ballots; they are gathered
+        // one a lane through LDS and sorted in registers.
+        constexpr int SL = 1 + (int)(SYM_TCAP / 64);
+        const unsigned ns = 1 + (listed + 63) / 64;  // slots in use (wave-uniform)
+        uint32_t kk[SL], rr[SL];
+        bool ok[SL];
+        ok[0] = lane < own;
+        kk[0] = ok[0] ? order_f32(keys[r * m + lane]) : 0xFFFFFFFFu;
+        rr[0] = ok[0] ? rows[r * m + lane] : 0xFFFFFFFFu;
+        unsigned total = own;
+#pragma unroll
+        for (int sl = 1; sl < SL; sl++) {
+            ok[sl] = false;
+            kk[sl] = rr[sl] = 0xFFFFFFFFu;
+            if ((unsigned)sl < ns) {
+                const unsigned e = 64 * (sl - 1) + lane;
+                const Entry v = e < listed ? tlist[r * SYM_TCAP + e] : make_uint2(0, 0);
+                ok[sl] = e < listed && __uint_as_float(v.x) <= T;
+                if (ok[sl]) {
+                    kk[sl] = order_f32(__uint_as_float(v.x));
+                    rr[sl] = v.y;
+                }
+                total += (unsigned)__popcll(__ballot(ok[sl]));
+            }
+        }
+        // Keep (key, row) <= (K, R); every listed pair when there are at most m.
+        uint32_t K = 0xFFFFFFFFu, R = 0xFFFFFFFFu;
+        if (total > m) {
+            const auto count_if = [&](auto pred) {
+                unsigned c = 0;
+#pragma unroll
+                for (int sl = 0; sl < SL; sl++)
+                    if ((unsigned)sl < ns) c += (unsigned)__popcll(__ballot(ok[sl] && pred(kk[sl], rr[sl])));
+                return c;
+            };
+            // K: the largest key with fewer than m pairs below it (the m-th key).
+            K = 0;
+            for (int bit = 31; bit >= 0; bit--) {
+                const uint32_t t = K | (1u << bit);
+                if (count_if([&](uint32_t kv, uint32_t) { return kv < t; }) < m) K = t;
+            }
+            const unsigned need = m - count_if([&](uint32_t kv, uint32_t) { return kv < K; });
+            if (count_if([&](uint32_t kv, uint32_t) { return kv == K; }) > need) {
+                R = 0;
+                for (int bit = 31; bit >= 0; bit--) {
+                    const uint32_t t = R | (1u << bit);
+                    if (count_if([&](uint32_t kv, uint32_t rv) { return kv == K && rv < t; }) < need) R = t;
+                }
+            }
+        }
+        unsigned kept = 0;  // wave-uniform
+#pragma unroll
+        for (int sl = 0; sl < SL; sl++) {
+            if ((unsigned)sl >= ns) continue;
+            const bool keep = ok[sl] && (kk[sl] < K || (kk[sl] == K && rr[sl] <= R));
+            const uint64_t mask = __ballot(keep);
+            if (keep) {
+                const unsigned at = kept + mbcnt(mask);
+                sk[at] = kk[sl];
+                sr[at] = rr[sl];
+            }
+            kept += (unsigned)__popcll(mask);
+        }
+        wave_sync();
+        unsigned long long key = lane < kept ? (unsigned long long)sk[lane] : ~0ull;
+        uint32_t row = lane < kept ? sr[lane] : 0xFFFFFFFFu;
+        wave_sort64(key, row);
+        if (lane < m) {
+            rows[r * m + lane] = lane < kept ? perm[row] : 0xFFFFFFFFu;
+            keys[r * m + lane] = lane < kept ? unorder_f32((uint32_t)key) : INFINITY;
+        }
+        const uint32_t mth = (uint32_t)__shfl(key, (int)m - 1);
+        if (lane == 0) {
+            count[r] = kept;
+            thr[r] = total >= m ? unorder_f32(mth) : T;
+        }
+        return;
+    }
     for (unsigned e = lane; e < own; e += 64) {
         sk[e] = order_f32(keys[r * m + e]);
         sr[e] = rows[r * m + e];
