@@ -498,18 +498,39 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
     const unsigned cnt = cand_count[i];
     const unsigned t = lane & 7, g = lane >> 3;  // kyto lane t, candidate group g
     const float *qrow = pts + q * d;
+    // BF16 candidates (Euclidean; ascending keys): candidate c can be among the k
+    // nearest only if the least exact key its BF16 key allows, (sqrt(key_c - delta_q)
+    // - e)^2 with e = err_q + max err, is at most the most the k-th candidate's allows,
+    // (sqrt(key_k + delta_q) + e)^2 (the first k candidates' exact keys are all at or
+    // below it). The others' rows are not read: their exact keys would exceed the k
+    // smallest. `need`: one past the last candidate that can be among them.
+    unsigned need = cnt;
+    if (!COS && floor_mode == 1 && cnt > k) {
+        const double e = bf_err[q] + bf_err_max, dq = bf_err[n + q];
+        const double hi = sqrt((double)cand_keys[i * m + k - 1] + dq) + e;
+        const double upper = hi * hi * (1.0 + 1e-12);
+        bool may = false;
+        if (lane < cnt) {
+            const double s = (double)cand_keys[i * m + lane] - dq;
+            const double r = s > 0.0 ? sqrt(s) - e : -1.0;
+            may = !(r > 0.0 && r * r * (1.0 - 1e-12) > upper);
+        }
+        const uint64_t mask = ballot(may);
+        need = mask ? 64u - (unsigned)__clzll(mask) : cnt;
+    }
     // Lane t sums components t, t + 8, ... of each of its group's 8 candidates
     // (8r + g, r < 8); each 64-component chunk issues all its loads first. Loads are
-    // unconditional (a missing candidate reads the query's row, components past d
-    // read component d - 1) so that none waits on a branch; those values are unused.
+    // unconditional (a missing or dropped candidate reads the query's row, components
+    // past d read component d - 1) so that none waits on a branch; those values are
+    // unused.
     uint32_t rowv[8], lrow[8];
     double part[8];
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         const unsigned c = 8 * r + g;
-        const uint32_t cr = cand_rows[i * m + (c < cnt ? c : 0)];
-        rowv[r] = c < cnt ? cr : 0xFFFFFFFFu;
-        lrow[r] = c < cnt ? cr : (uint32_t)q;
+        const uint32_t cr = cand_rows[i * m + (c < need ? c : 0)];
+        rowv[r] = c < need ? cr : 0xFFFFFFFFu;
+        lrow[r] = c < need ? cr : (uint32_t)q;
         part[r] = 0.0;
     }
     for (unsigned j0 = 0; j0 < d; j0 += 64) {
@@ -549,7 +570,7 @@ __global__ __launch_bounds__(256) void rerank(const float *__restrict__ pts, uin
         double key = hq + __shfl_xor(hq, 1, 8);
         if constexpr (COS) key = cosine_key(key, norms[q], norms[lrow[r]]);
         if (t == (unsigned)r) {
-            mine = c < cnt ? (unsigned long long)__double_as_longlong(key) : ~0ull;
+            mine = c < need ? (unsigned long long)__double_as_longlong(key) : ~0ull;
             mine_row = rowv[r];
         }
     }

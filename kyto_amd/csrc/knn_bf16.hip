@@ -906,16 +906,15 @@ __global__ __launch_bounds__(256) void group_max(const float *__restrict__ thr, 
 // threshold, which is at least T; below T the union is whole. Its m smallest (key,
 // scan-order row) pairs at or below T replace the row's candidates, as rows (perm[.]).
 // A list that overflowed tcap leaves the row without candidates (the exact search takes
-// it).
+// it). m <= 64 (candidates_for).
 constexpr uint32_t SYM_TCAP = 320;     // list entries a row
-constexpr unsigned SYM_MERGE = 512;    // m + SYM_TCAP, rounded up to a power of two
 
 __global__ __launch_bounds__(256) void merge_sym(const uint32_t *__restrict__ perm, uint64_t n, uint32_t m,
                                                  const uint32_t *__restrict__ tcount, const Entry *__restrict__ tlist,
                                                  uint32_t *__restrict__ rows, float *__restrict__ keys,
                                                  uint32_t *__restrict__ count, float *__restrict__ thr) {
-    __shared__ uint32_t skey[4][SYM_MERGE];
-    __shared__ uint32_t srow[4][SYM_MERGE];
+    __shared__ uint32_t skey[4][64];
+    __shared__ uint32_t srow[4][64];
     const unsigned lane = lane_id(), wave = threadIdx.x / 64;
     const uint64_t r = (uint64_t)blockIdx.x * 4 + wave;
     if (r >= n) return;
@@ -930,7 +929,7 @@ __global__ __launch_bounds__(256) void merge_sym(const uint32_t *__restrict__ pe
         if (lane == 0) count[r] = 0;
         return;
     }
-    if (m <= 64) {
+    {
         // In registers: slot 0 the row's own candidates (at most m), slots 1 .. the
         // listed keys at or below T (a free slot holds (~0, ~0)). The m smallest (key,
         // row) pairs are those at or below the m-th, found bit by bit (its key, then,
@@ -1010,41 +1009,6 @@ __global__ __launch_bounds__(256) void merge_sym(const uint32_t *__restrict__ pe
             count[r] = kept;
             thr[r] = total >= m ? unorder_f32(mth) : T;
         }
-        return;
-    }
-    for (unsigned e = lane; e < own; e += 64) {
-        sk[e] = order_f32(keys[r * m + e]);
-        sr[e] = rows[r * m + e];
-    }
-    unsigned total = own;  // wave-uniform
-    for (unsigned e0 = 0; e0 < listed; e0 += 64) {
-        const unsigned e = e0 + lane;
-        const Entry v = e < listed ? tlist[r * SYM_TCAP + e] : make_uint2(0, 0);
-        const bool keep = e < listed && __uint_as_float(v.x) <= T;
-        const uint64_t mask = __ballot(keep);
-        if (keep) {
-            const unsigned at = total + mbcnt(mask);
-            sk[at] = order_f32(__uint_as_float(v.x));
-            sr[at] = v.y;
-        }
-        total += (unsigned)__popcll(mask);
-    }
-    unsigned N = 64;
-    while (N < total) N <<= 1;
-    for (unsigned e = total + lane; e < N; e += 64) {
-        sk[e] = 0xFFFFFFFFu;
-        sr[e] = 0xFFFFFFFFu;
-    }
-    wave_sync();
-    wave_bitonic<uint32_t>(sk, sr, N);
-    const unsigned kept = min(total, m);
-    for (unsigned e = lane; e < m; e += 64) {
-        rows[r * m + e] = e < kept ? perm[sr[e]] : 0xFFFFFFFFu;
-        keys[r * m + e] = e < kept ? unorder_f32(sk[e]) : INFINITY;
-    }
-    if (lane == 0) {
-        count[r] = kept;
-        thr[r] = total >= m ? unorder_f32(sk[m - 1]) : T;
     }
 }
 
@@ -1409,6 +1373,7 @@ static void bf16_scan(Context &ctx, const KnnShape &s, const Bf16Points &b, cons
             nullptr, r.tcount, static_cast<Entry *>(r.tlist), SYM_TCAP, r.thr_max);
         KY_HIP(hipGetLastError());
         if (scan_ms) KY_HIP(hipEventRecord(ev[1], st));
+        KY_CHECK(s.m <= 64, KY_E505, "knn: more than 64 candidates a row in the symmetric scan (internal)");
         merge_sym<<<div_up(count, 4), 256, 0, st>>>(b.perm, count, s.m, r.tcount, static_cast<const Entry *>(r.tlist),
                                                      r.rows, r.keys, r.count, r.thr);
         KY_HIP(hipGetLastError());
