@@ -338,11 +338,13 @@ __device__ __forceinline__ uint32_t fresh_lane() {
 
 constexpr int LEAD_TILES = 16;        // tiles scanned before a block's own position
 constexpr uint64_t EST_MIN_ROWS = 1 << 17;  // threshold estimates from a sample above this size
-// The sample: every 12th row of the scan order. With 32 subsets of n / 384 sampled
-// rows and the p-th smallest minimum as the estimate, about -ln(1 - p/32) * 384 rows
-// fall at or below it on average; p is the smallest giving 7 (k + 1) (p = 8 and 112
-// rows for k = 14, p = 14 and 221 rows for k = 29).
-constexpr uint32_t EST_STRIDE = 12;
+// The sample: every 16th row of the scan order. With 32 subsets of n / 512 sampled
+// rows and the p-th smallest minimum as the estimate, about -ln(1 - p/32) * 512 rows
+// fall at or below it on average; p is the smallest giving 7 (k + 1) (p = 6 and 106
+// rows for k = 14, p = 11 and 216 rows for k = 29). Measured at 1M (k = 14): every
+// 12th row costs 2.1 ms more estimate than it saves in rows scanned again (560 rows
+// at every 16th, 1.3 ms); every 20th leaves 1,628 rows short and scans slower.
+constexpr uint32_t EST_STRIDE = 16;
 
 constexpr unsigned SCAP = 64;  // staged appends per wavefront
 constexpr unsigned MAX_TAIL_SEG = 64;                // point segments of a tail query block
@@ -1408,11 +1410,15 @@ void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t l
 }
 
 // ------------------------------------------------------------------- rescan ----
-constexpr float RESCAN_FACTOR = 2.0f;
+// The short rows' starting thresholds times this: 1.25 at 1M x 50 (k = 14) rescans
+// the 560 short rows in 1.3 ms with none left short; 1.15 0.8 ms, 1.5 3.3 ms and 2
+// 6.0 ms (the keys kept grow fast with the threshold).
+constexpr float RESCAN_FACTOR = 1.25f;
 // The rows a round's re-rank left with fewer than m candidates (a starting threshold
 // below their m-th key, or the symmetric scan's list of the row overflowed) are
-// scanned again as a list, from twice the first round's starting threshold (none
-// below 2^17 rows): each keeps its m smallest pairs at or below it.
+// scanned again as a list, from RESCAN_FACTOR times the first round's starting
+// threshold (none below 2^17 rows): each keeps its m smallest pairs at or below it;
+// a row still short goes to the exact search.
 
 template <class A>
 void bf16_take_rescan(A &a, const KnnShape &s, uint64_t count, Bf16Rescan &x, KnnRound &r) {
