@@ -78,17 +78,28 @@ __global__ __launch_bounds__(256) void solve_rows_upper(double *__restrict__ a, 
 __device__ __forceinline__ double max_nan(double m, double a) { return (a > m || a != a) ? a : m; }
 
 // out[0] = max |x| and out[1] = sum of (x / max)^2 over count values (one block of
-// 256 threads: strided runs in order, then a fixed tree; 0 and 0 for an all-zero x; a
-// value that is not finite makes the norm out[0] sqrt(out[1]) not finite).
-__global__ __launch_bounds__(256) void scaled_squares(const double *__restrict__ x, uint64_t count,
-                                                      double *__restrict__ out) {
-    __shared__ double part[256];
+// 1024 threads: strided runs in order, 8 loads in flight a thread, then a fixed tree;
+// 0 and 0 for an all-zero x; a value that is not finite makes the norm out[0]
+// sqrt(out[1]) not finite).
+constexpr unsigned SQ_THREADS = 1024, SQ_U = 8;
+__global__ __launch_bounds__(SQ_THREADS) void scaled_squares(const double *__restrict__ x, uint64_t count,
+                                                             double *__restrict__ out) {
+    __shared__ double part[SQ_THREADS];
     const unsigned t = threadIdx.x;
     double m = 0.0;
-    for (uint64_t i = t; i < count; i += 256) m = max_nan(m, fabs(x[i]));
+    for (uint64_t i0 = t; i0 < count; i0 += SQ_THREADS * SQ_U) {
+        double v[SQ_U];
+#pragma unroll
+        for (unsigned u = 0; u < SQ_U; u++) {
+            const uint64_t i = i0 + (uint64_t)u * SQ_THREADS;
+            v[u] = i < count ? fabs(x[i]) : 0.0;
+        }
+#pragma unroll
+        for (unsigned u = 0; u < SQ_U; u++) m = max_nan(m, v[u]);
+    }
     part[t] = m;
     __syncthreads();
-    for (unsigned o = 128; o > 0; o >>= 1) {
+    for (unsigned o = SQ_THREADS / 2; o > 0; o >>= 1) {
         if (t < o) part[t] = max_nan(part[t], part[t + o]);
         __syncthreads();
     }
@@ -96,13 +107,19 @@ __global__ __launch_bounds__(256) void scaled_squares(const double *__restrict__
     __syncthreads();
     double s = 0.0;
     if (big > 0.0)
-        for (uint64_t i = t; i < count; i += 256) {
-            const double a = x[i] / big;
-            s += a * a;
+        for (uint64_t i0 = t; i0 < count; i0 += SQ_THREADS * SQ_U) {
+            double v[SQ_U];
+#pragma unroll
+            for (unsigned u = 0; u < SQ_U; u++) {
+                const uint64_t i = i0 + (uint64_t)u * SQ_THREADS;
+                v[u] = i < count ? x[i] / big : 0.0;
+            }
+#pragma unroll
+            for (unsigned u = 0; u < SQ_U; u++) s += v[u] * v[u];
         }
     part[t] = s;
     __syncthreads();
-    for (unsigned o = 128; o > 0; o >>= 1) {
+    for (unsigned o = SQ_THREADS / 2; o > 0; o >>= 1) {
         if (t < o) part[t] += part[t + o];
         __syncthreads();
     }
@@ -208,7 +225,7 @@ Mat DeviceBasis::get_block(size_t width) {
 double DeviceBasis::block_norm(size_t width) {
     if (width == 0) return 0.0;
     work_p.reserve(2);
-    scaled_squares<<<1, 256, 0, ctx->stream>>>(work_r.ptr, rows * width, work_p.ptr);
+    scaled_squares<<<1, SQ_THREADS, 0, ctx->stream>>>(work_r.ptr, rows * width, work_p.ptr);
     KY_HIP(hipGetLastError());
     double h[2];
     download(*ctx, h, work_p.ptr, sizeof(h));

@@ -1113,22 +1113,25 @@ __global__ void block_f32(const double *__restrict__ q, uint32_t genes, uint32_t
     out[i] = j < w ? (float)q[(uint64_t)g * w + j] : 0.0f;
 }
 
-__global__ void rounded_means(const double *__restrict__ q, const double *__restrict__ means, uint32_t genes,
-                              uint32_t w, double *__restrict__ bm) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= w) return;
-    constexpr uint32_t B = 16;  // loads issued together, then added in order
+// One block of 256 threads a column j: the column's terms means[g] * f64(f32(q[g][j]))
+// staged in LDS by the block (RM_GENES at a time), then added in gene order by one
+// thread.
+constexpr uint32_t RM_GENES = 4096;
+__global__ __launch_bounds__(256) void rounded_means(const double *__restrict__ q, const double *__restrict__ means,
+                                                     uint32_t genes, uint32_t w, double *__restrict__ bm) {
+    __shared__ double term[RM_GENES];
+    const uint32_t j = blockIdx.x;
     double s = 0.0;
-    uint32_t g = 0;
-    for (; g + B <= genes; g += B) {
-        double t[B];
-#pragma unroll
-        for (uint32_t u = 0; u < B; u++) t[u] = means[g + u] * (double)(float)q[(uint64_t)(g + u) * w + j];
-#pragma unroll
-        for (uint32_t u = 0; u < B; u++) s += t[u];
+    for (uint32_t g0 = 0; g0 < genes; g0 += RM_GENES) {
+        const uint32_t cnt = min(RM_GENES, genes - g0);
+        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x)
+            term[i] = means[g0 + i] * (double)(float)q[(uint64_t)(g0 + i) * w + j];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (uint32_t i = 0; i < cnt; i++) s += term[i];
+        __syncthreads();
     }
-    for (; g < genes; g++) s += means[g] * (double)(float)q[(uint64_t)g * w + j];
-    bm[j] = s;
+    if (threadIdx.x == 0) bm[j] = s;
 }
 
 // product[g][j] -= means[g] * sums[j] (the centring of X^T W).
@@ -1304,7 +1307,7 @@ static double *pca_product_resident(Csr &csr, const double *q, uint32_t w, const
     ctx.dense_f32.reserve(cells + DENSE_PAD);
     ctx.dense_f64.reserve(w);
     block_f32<<<div_up(cells, 256), 256, 0, st>>>(q, csr.n_cols, w, ld, ctx.dense_f32.ptr);
-    rounded_means<<<div_up(w, 64), 64, 0, st>>>(q, means, csr.n_cols, w, ctx.dense_f64.ptr);
+    rounded_means<<<w, 256, 0, st>>>(q, means, csr.n_cols, w, ctx.dense_f64.ptr);
     KY_HIP(hipGetLastError());
     ctx.wbuf.reserve((size_t)csr.n_rows * ld + DENSE_PAD);
     launch_row_project(csr, ctx.dense_f32.ptr, w, ld, ctx.dense_f64.ptr, ctx.wbuf.ptr, ld);
