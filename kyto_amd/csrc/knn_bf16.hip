@@ -340,10 +340,13 @@ constexpr int LEAD_TILES = 16;        // tiles scanned before a block's own posi
 constexpr uint64_t EST_MIN_ROWS = 1 << 17;  // threshold estimates from a sample above this size
 // The sample: every 16th row of the scan order. With 32 subsets of n / 512 sampled
 // rows and the p-th smallest minimum as the estimate, about -ln(1 - p/32) * 512 rows
-// fall at or below it on average; p is the smallest giving 7 (k + 1) (p = 6 and 106
-// rows for k = 14, p = 11 and 216 rows for k = 29). Measured at 1M (k = 14): every
-// 12th row costs 2.1 ms more estimate than it saves in rows scanned again (560 rows
-// at every 16th, 1.3 ms); every 20th leaves 1,628 rows short and scans slower.
+// fall at or below it on average; p is the smallest giving 5 (k + 1) (p = 5 and 87
+// rows for k = 14, p = 9 and 169 rows for k = 29). Measured at 1M (k = 14): every
+// 12th row costs 2.1 ms more estimate than it saves in rows scanned again; every
+// 20th leaves 1,628 rows short and scans slower. Aiming at 5 (k + 1) rows in place of
+// 7 (k + 1) keeps fewer keys (scan -3.2 ms at 1M, -26 ms at 4M) for more rows scanned
+// again at 1M (3,858 rows in 2.9 ms, 560 in 1.3) and fewer at 4M (1,835 against
+// 7,497: fewer lists overflow); 4.5 (k + 1) (p = 4) leaves 22,441 rows short.
 constexpr uint32_t EST_STRIDE = 16;
 
 constexpr unsigned SCAP = 64;  // staged appends per wavefront
@@ -1335,7 +1338,7 @@ void bf16_prepare(Context &ctx, const KnnShape &s, const float *pts, const KnnPr
         const uint64_t np = div_up(n, EST_STRIDE);
         uint32_t pick = 1;
         while (pick < EST_SUBSETS - 1 &&
-               -std::log1p(-(double)pick / EST_SUBSETS) * EST_SUBSETS * EST_STRIDE < 7.0 * (k + 1))
+               -std::log1p(-(double)pick / EST_SUBSETS) * EST_SUBSETS * EST_STRIDE < 5.0 * (k + 1))
             pick++;
         estimate_bf16<<<div_up(s.nq, EST_QPB), 256, 0, st>>>(P, Q, np, EST_STRIDE, b.qpos, 0, s.nq, pick, b.thr_est);
         KY_HIP(hipGetLastError());
@@ -1410,9 +1413,9 @@ void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t l
 }
 
 // ------------------------------------------------------------------- rescan ----
-// The short rows' starting thresholds times this: 1.25 at 1M x 50 (k = 14) rescans
-// the 560 short rows in 1.3 ms with none left short; 1.15 0.8 ms, 1.5 3.3 ms and 2
-// 6.0 ms (the keys kept grow fast with the threshold).
+// The short rows' starting thresholds times this: 1.25 at 1M x 50 (k = 14; the 560
+// short rows of a 7 (k + 1) target) rescans them in 1.3 ms with none left short; 1.15
+// 0.8 ms, 1.5 3.3 ms and 2 6.0 ms (the keys kept grow fast with the threshold).
 constexpr float RESCAN_FACTOR = 1.25f;
 // The rows a round's re-rank left with fewer than m candidates (a starting threshold
 // below their m-th key, or the symmetric scan's list of the row overflowed) are
