@@ -1413,10 +1413,10 @@ void bf16_round(Context &ctx, const KnnShape &s, const Bf16Points &b, uint64_t l
 }
 
 // ------------------------------------------------------------------- rescan ----
-// The short rows' starting thresholds times this: 1.25 at 1M x 50 (k = 14; the 560
-// short rows of a 7 (k + 1) target) rescans them in 1.3 ms with none left short; 1.15
-// 0.8 ms, 1.5 3.3 ms and 2 6.0 ms (the keys kept grow fast with the threshold).
-constexpr float RESCAN_FACTOR = 1.25f;
+// The short rows' starting thresholds times this: 1.15 at 1M x 50 (k = 14) rescans the
+// 3,858 short rows in 1.7 ms with none left short; 1.10 1.4 ms but 3 rows left for the
+// exact search, 1.25 2.9 ms, 1.35 4.1 ms (the keys kept grow fast with the threshold).
+constexpr float RESCAN_FACTOR = 1.15f;
 // The rows a round's re-rank left with fewer than m candidates (a starting threshold
 // below their m-th key, or the symmetric scan's list of the row overflowed) are
 // scanned again as a list, from RESCAN_FACTOR times the first round's starting
