@@ -963,6 +963,58 @@ __global__ __launch_bounds__(256) void merge_sym(const uint32_t *__restrict__ pe
                 total += (unsigned)__popcll(__ballot(ok[sl]));
             }
         }
+        // At most 64 listed pairs kept (the usual case): they are sorted in registers and
+        // merged with the row's own candidates, already ascending by (key, row) (a
+        // bitonic merge: one list reversed, the smaller of each pair, then a
+        // half-cleaner cascade), so no selection is needed.
+        const unsigned listed_kept = total - own;
+        if (listed_kept <= 64) {
+            unsigned at0 = 0;  // wave-uniform
+#pragma unroll
+            for (int sl = 1; sl < SL; sl++) {
+                if ((unsigned)sl >= ns) continue;
+                const uint64_t mask = __ballot(ok[sl]);
+                if (ok[sl]) {
+                    const unsigned at = at0 + mbcnt(mask);
+                    sk[at] = kk[sl];
+                    sr[at] = rr[sl];
+                }
+                at0 += (unsigned)__popcll(mask);
+            }
+            wave_sync();
+            unsigned long long b = lane < listed_kept ? (unsigned long long)sk[lane] : ~0ull;
+            uint32_t rb = lane < listed_kept ? sr[lane] : 0xFFFFFFFFu;
+            wave_sort64(b, rb);
+            unsigned long long a = ok[0] ? (unsigned long long)kk[0] : ~0ull;
+            uint32_t ra = rr[0];
+            const unsigned long long br = __shfl(b, 63 - (int)lane);
+            const uint32_t rbr = (uint32_t)__shfl((int)rb, 63 - (int)lane);
+            if (br < a || (br == a && rbr < ra)) {
+                a = br;
+                ra = rbr;
+            }
+#pragma unroll
+            for (unsigned stride = 32; stride > 0; stride >>= 1) {
+                const unsigned long long ok2 = __shfl_xor(a, stride);
+                const uint32_t orow = (uint32_t)__shfl_xor((int)ra, stride);
+                const bool other_less = ok2 < a || (ok2 == a && orow < ra);
+                if (other_less == ((lane & stride) == 0)) {
+                    a = ok2;
+                    ra = orow;
+                }
+            }
+            const unsigned kept2 = min(total, m);
+            if (lane < m) {
+                rows[r * m + lane] = lane < kept2 ? perm[ra] : 0xFFFFFFFFu;
+                keys[r * m + lane] = lane < kept2 ? unorder_f32((uint32_t)a) : INFINITY;
+            }
+            const uint32_t mth2 = (uint32_t)__shfl(a, (int)m - 1);
+            if (lane == 0) {
+                count[r] = kept2;
+                thr[r] = total >= m ? unorder_f32(mth2) : T;
+            }
+            return;
+        }
         // Keep (key, row) <= (K, R); every listed pair when there are at most m.
         uint32_t K = 0xFFFFFFFFu, R = 0xFFFFFFFFu;
         if (total > m) {
