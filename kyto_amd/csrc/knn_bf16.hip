@@ -1529,10 +1529,10 @@ void bf16_rescan(Context &ctx, const KnnShape &s, const Bf16Points &b, const uin
     sort_pairs(st, x.k2, x.qpos, x.v2, x.qorig, count, 32, x.tmp);
     list_rows<<<div_up(count, 256), 256, 0, st>>>(rows, count, x.qorig);
     KY_HIP(hipGetLastError());
-    // Twice the first round's starting threshold (a key): a row its list could not hold
-    // keeps everything the first round would have, one whose threshold left fewer than m
-    // rows gains the rows up to twice its key (most reach m; the others go on to the
-    // exact search).
+    // RESCAN_FACTOR times the first round's starting threshold (a key): a row its list
+    // could not hold keeps everything the first round would have, one whose threshold
+    // left fewer than m rows gains the rows up to the larger key (most reach m; the
+    // others go on to the exact search).
     if (x.thr && b.thr_est) {
         rescan_thresholds<<<div_up(count, 256), 256, 0, st>>>(x.qpos, count, b.thr_est, RESCAN_FACTOR, x.thr);
         KY_HIP(hipGetLastError());
