@@ -65,44 +65,52 @@ constexpr int GMAX_PER_TILE = 4 * (TILE / SPB);  // symmetric scan: lane groups 
 
 // ---------------------------------------------------------------- preparation ----
 
-// Row i of the scan order is original row perm[i]: P', Q' written at i, err and |b|
-// at the original row. One wavefront a row (lane j: component j; fixed butterflies
-// add o = |b|^2 and err^2 over the lanes), BROWS rows a block, one atomic a block for
-// the largest err and |b|.
+// Original row r is row inv[r] of the scan order: its P', Q' are written there, err and
+// |b| at r (the rows read in order, each 128-byte row of P' and Q' written whole). 16
+// lanes a row (lane t: components 4t .. 4t + 3; fixed butterflies over the 16 add o =
+// |b|^2 and err^2), 4 rows a wavefront at a time, BROWS rows a block, one atomic a
+// block for the largest err and |b|.
 constexpr unsigned BROWS = 64;
 
 __global__ __launch_bounds__(256) void bf16_rows(const float *__restrict__ x, const double *__restrict__ mean,
                                                  const double *__restrict__ norms,
-                                                 const float *__restrict__ cent, const uint32_t *__restrict__ perm,
+                                                 const float *__restrict__ cent, const uint32_t *__restrict__ inv,
                                                  uint64_t n, uint32_t d, uint32_t dp, __bf16 *__restrict__ P,
                                                  __bf16 *__restrict__ Q, double *__restrict__ err,
                                                  double *__restrict__ nb, unsigned long long *__restrict__ err_max,
                                                  unsigned long long *__restrict__ nb_max) {
     __shared__ double wmax[2][4];
-    const unsigned lane = lane_id(), wave = threadIdx.x / 64;
-    const double mu = (!norms && lane < d) ? mean[lane] : 0.0;
+    const unsigned lane = lane_id(), wave = threadIdx.x / 64, t = lane % 16;
+    double mu[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) mu[u] = (!norms && 4 * t + u < d) ? mean[4 * t + u] : 0.0;
     double emx = 0.0, nmx = 0.0;
-    for (unsigned t = wave; t < BROWS; t += 4) {
-        const uint64_t i = (uint64_t)blockIdx.x * BROWS + t;
-        if (i >= n) break;
-        const uint64_t r = perm[i];
-        __bf16 b = (__bf16)0.0f;
+    for (unsigned q0 = wave * 4; q0 < BROWS; q0 += 16) {
+        const uint64_t r = (uint64_t)blockIdx.x * BROWS + q0 + lane / 16;
+        const bool live = r < n;
+        const uint64_t i = live ? inv[r] : 0;
+        __bf16 b[4];
         double o = 0.0, e2 = 0.0;
-        if (lane < d) {
-            b = (__bf16)cent[r * dp + lane];
-            const double bv = (double)(float)b;
-            // The row b stands for: x - mean (Euclidean), or x / |x| (cosine; 0 for a
-            // zero row), in f64.
-            const double exact = norms ? (norms[r] != 0.0 ? (double)x[r * d + lane] / norms[r] : 0.0)
-                                       : (double)x[r * d + lane] - mu;
-            const double tt = bv - exact;
-            e2 = tt * tt;
-            o = bv * bv;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const unsigned c = 4 * t + u;
+            b[u] = (__bf16)0.0f;
+            if (live && c < d) {
+                b[u] = (__bf16)cent[r * dp + c];
+                const double bv = (double)(float)b[u];
+                // The row b stands for: x - mean (Euclidean), or x / |x| (cosine; 0 for
+                // a zero row), in f64.
+                const double exact = norms ? (norms[r] != 0.0 ? (double)x[r * d + c] / norms[r] : 0.0)
+                                           : (double)x[r * d + c] - mu[u];
+                const double tt = bv - exact;
+                e2 += tt * tt;
+                o += bv * bv;
+            }
         }
 #pragma unroll
-        for (int off = 32; off >= 1; off /= 2) {
-            o += __shfl_xor(o, off);
-            e2 += __shfl_xor(e2, off);
+        for (int off = 8; off >= 1; off /= 2) {
+            o += __shfl_xor(o, off, 16);
+            e2 += __shfl_xor(e2, off, 16);
         }
         const __bf16 hi = (__bf16)(float)o;
         const double r1 = o - (double)(float)hi;
@@ -110,28 +118,40 @@ __global__ __launch_bounds__(256) void bf16_rows(const float *__restrict__ x, co
         const double r2 = r1 - (double)(float)mid;
         const __bf16 lo = (__bf16)(float)r2;
         const __bf16 one = (__bf16)1.0f, zero = (__bf16)0.0f;
-        __bf16 pv = zero, qv = zero;
-        if (lane < d) {
-            pv = (__bf16)(-2.0f * (float)b);
-            qv = b;
-        } else if (lane < d + 3) {
-            const __bf16 part = lane == d ? hi : lane == d + 1 ? mid : lo;
-            pv = part;
-            qv = one;
-        } else if (lane < d + 6) {
-            const unsigned u = lane - (d + 3);
-            pv = one;
-            qv = u == 0 ? hi : u == 1 ? mid : lo;
+        __bf16 pv[4], qv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const unsigned c = 4 * t + u;
+            pv[u] = zero;
+            qv[u] = zero;
+            if (c < d) {
+                pv[u] = (__bf16)(-2.0f * (float)b[u]);
+                qv[u] = b[u];
+            } else if (c < d + 3) {
+                pv[u] = c == d ? hi : c == d + 1 ? mid : lo;
+                qv[u] = one;
+            } else if (c < d + 6) {
+                const unsigned v = c - (d + 3);
+                pv[u] = one;
+                qv[u] = v == 0 ? hi : v == 1 ? mid : lo;
+            }
         }
-        P[i * KDIM + lane] = pv;
-        Q[i * KDIM + lane] = qv;
-        if (lane == 0) {
-            const double e = sqrt(e2), nrm = sqrt(o);
-            err[r] = e;
-            nb[r] = nrm;
-            emx = fmax(emx, e);
-            nmx = fmax(nmx, nrm);
+        if (live) {
+            *reinterpret_cast<uint2 *>(P + i * KDIM + 4 * t) = __builtin_bit_cast(uint2, pv);
+            *reinterpret_cast<uint2 *>(Q + i * KDIM + 4 * t) = __builtin_bit_cast(uint2, qv);
+            if (t == 0) {
+                const double e = sqrt(e2), nrm = sqrt(o);
+                err[r] = e;
+                nb[r] = nrm;
+                emx = fmax(emx, e);
+                nmx = fmax(nmx, nrm);
+            }
         }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off /= 2) {
+        emx = fmax(emx, __shfl_xor(emx, off));
+        nmx = fmax(nmx, __shfl_xor(nmx, off));
     }
     if (lane == 0) {
         wmax[0][wave] = emx;
@@ -1245,7 +1265,7 @@ template void bf16_take_points<Arena>(Arena &, const KnnShape &, Bf16Points &);
 template void bf16_take_points<SizeArena>(SizeArena &, const KnnShape &, Bf16Points &);
 
 // The preparation's temporaries: Morton keys and values, the sort's scratch, |b| per
-// row; for a list, the inverse order and the queries' keys and values.
+// row, the inverse order; for a list, the queries' keys and values.
 template <class A>
 static void bf16_take_temp(A &a, const KnnShape &s, uint32_t *&keys, uint32_t *&vals, uint32_t *&skeys, void *&tmp,
                            double *&nb, uint32_t *&inv, uint32_t *&k2, uint32_t *&v2) {
@@ -1254,9 +1274,9 @@ static void bf16_take_temp(A &a, const KnnShape &s, uint32_t *&keys, uint32_t *&
     skeys = a.template take<uint32_t>(s.n);
     tmp = a.template take<unsigned char>(sort_pairs_bytes(std::max(s.n, s.nq)));
     nb = a.template take<double>(s.n);
-    inv = k2 = v2 = nullptr;
+    inv = a.template take<uint32_t>(s.n);
+    k2 = v2 = nullptr;
     if (s.list) {
-        inv = a.template take<uint32_t>(s.n);
         k2 = a.template take<uint32_t>(s.nq);
         v2 = a.template take<uint32_t>(s.nq);
     }
@@ -1350,7 +1370,8 @@ void bf16_prepare(Context &ctx, const KnnShape &s, const float *pts, const KnnPr
     sort_pairs(st, keys, skeys, vals, b.perm, n, 30, sort_tmp);
     __bf16 *P = static_cast<__bf16 *>(b.P), *Q = static_cast<__bf16 *>(b.Q);
     pad_rows<<<div_up(PAD_ROWS * KDIM, 256), 256, 0, st>>>(P, n, PAD_ROWS, s.d);
-    bf16_rows<<<div_up(n, BROWS), 256, 0, st>>>(pts, prep.mean, prep.norms, prep.cent, b.perm, n, s.d, s.dp, P, Q,
+    invert_perm<<<div_up(n, 256), 256, 0, st>>>(b.perm, n, inv);
+    bf16_rows<<<div_up(n, BROWS), 256, 0, st>>>(pts, prep.mean, prep.norms, prep.cent, inv, n, s.d, s.dp, P, Q,
                                                b.err, nb, b.small, b.small + 1);
     KY_HIP(hipGetLastError());
     unsigned long long hm[2];
@@ -1374,7 +1395,6 @@ void bf16_prepare(Context &ctx, const KnnShape &s, const float *pts, const KnnPr
     b.acc_rel = cb;
     // A list of queries: their scan positions, ascending, and their rows in that order.
     if (s.list) {
-        invert_perm<<<div_up(n, 256), 256, 0, st>>>(b.perm, n, inv);
         query_positions<<<div_up(s.nq, 256), 256, 0, st>>>(q_rows, s.nq, inv, k2, v2);
         KY_HIP(hipGetLastError());
         sort_pairs(st, k2, b.qpos, v2, b.qorig, s.nq, 32, sort_tmp);
